@@ -1,0 +1,251 @@
+"""Benchmark of the colored LU-SGS hot path on one MI355X.
+
+Workload (the configuration the metric is quoted on): colored LU-SGS, 3-D
+Euler (nvars = 5), FP64, on a 96^3 hex grid split into 6 Kuhn tets per hex
+(5,308,416 cells, nface = 4), cell order randomly permuted from the seed,
+normals / areas / volumes from jittered vertex coordinates, fields as in
+ucfd_sparse_amd.mesh.flow_fields.  The mesh is greedy-coloured and stored
+colour-contiguously (Morton order inside a colour).  One step = one
+legacy-API pass: parallel_pre_lusgs, ns_parallel_lower_sweep for every
+colour, ns_parallel_upper_sweep for every colour in reverse, and
+lusgs_parallel_ns_update - all on device-resident arrays, enqueued on one
+stream.  The right-hand side is never overwritten (the two-array API takes
+the lower sweep's input and the upper sweep's output as separate arrays), so
+no restore copy sits in the timed region.
+
+Prints one JSON line.  value = cell-iterations/s of the FP64 path.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "LU-SGS cell-iters/s & HBM GB/s, FP32-mixed vs FP64 at equal residual drop"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(F, V=5, D=3, s=8):
+    """Per-cell bytes of each legacy call: every input array element read
+    once, every output written once, indices 4 B (colour arrays excluded)."""
+    pre = s * (2 * F + 2)
+    sweep = 4 * F + s * (2 * F + F * D + 1 + 4 * V)
+    update = 3 * V * s
+    return pre, sweep, update
+
+
+def read_traffic(path):
+    """key = value lines written from the rocprofv3 --pmc runs (profiles/)."""
+    out = {}
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        line = line.split("#", 1)[0].strip()
+        if "=" in line:
+            k, v = (t.strip() for t in line.split("=", 1))
+            try:
+                out[k] = float(v)
+            except ValueError:
+                pass
+    return out
+
+
+def cpu_baseline(lay, seconds, nvars=5):
+    """The oracle's OpenMP coloured pass (liboracle_fast.so) on the host
+    cores, same renumbered arrays; a bounded number of iterations."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as O
+    lib = O.lib(fast=True)
+    m, f = lay.mesh, lay.flow
+    N, F = m.neles, m.nface
+    off = np.ascontiguousarray(lay.off, np.int32)
+    upts = f.upts.copy()
+    dub = f.rhsb.copy()
+    diag = np.empty(N)
+    args = lambda: (N, F, 3, 5, nvars, lay.ncolors, off.ctypes.data,  # noqa: E731
+                    m.nei_ele.ctypes.data, lay.icolor.ctypes.data, lay.lcolor.ctypes.data,
+                    m.fnorm_vol.ctypes.data, m.vec_fnorm.ctypes.data, upts.ctypes.data,
+                    dub.ctypes.data, diag.ctypes.data, f.fspr.ctypes.data, f.dt.ctypes.data)
+    dub[...] = f.rhsb
+    lib.orc_colored_iteration(*args())          # warm-up
+    n, t = 0, 0.0
+    while t < seconds or n < 2:
+        dub[...] = f.rhsb
+        t0 = time.perf_counter()
+        lib.orc_colored_iteration(*args())
+        t += time.perf_counter() - t0
+        n += 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": n * N / t, "unit": "cell-iter/s", "cores": threads, "kind": "port",
+            "sample": f"{n} coloured FP64 iterations of the full {N}-cell workload, "
+                      f"oracle/lusgs_oracle.c built -O3 -march=x86-64-v3 -fopenmp, "
+                      f"{threads} OpenMP threads, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--m", type=int, default=96, help="hex cells per axis (6 m^3 tets)")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures the colored LU-SGS path on one GPU only; multi-GPU runs "
+              "(domain decomposition with halo exchange) are not part of this build. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from ucfd_sparse_amd import lib as L
+    from ucfd_sparse_amd import mesh as M
+    from ucfd_sparse_amd.layout import colored_layout
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    mesh = M.tet_mesh(args.m, seed=args.seed)
+    flow = M.flow_fields(mesh, seed=args.seed)
+    lay = colored_layout(mesh, flow)
+    del mesh, flow
+    N, F = lay.mesh.neles, lay.mesh.nface
+    nc = lay.ncolors
+    log(f"mesh {N} cells, {nc} colours, setup {time.time() - t0:.1f} s")
+
+    dev = torch.device("cuda:0")
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    m, f = lay.mesh, lay.flow
+    nei, ic, lc = up(m.nei_ele), up(lay.icolor), up(lay.lcolor)
+    fnv, vfn, fsp, dt = up(m.fnorm_vol), up(m.vec_fnorm), up(f.fspr), up(f.dt)
+    upts, rhs = up(f.upts), up(f.rhsb)
+    dstar = torch.zeros_like(rhs)
+    dq = torch.zeros_like(rhs)
+    diag = torch.zeros(N, dtype=torch.float64, device=dev)
+    P = {k: v.data_ptr() for k, v in dict(nei=nei, ic=ic, lc=lc, fnv=fnv, vfn=vfn, fsp=fsp,
+                                          dt=dt, upts=upts, rhs=rhs, dstar=dstar, dq=dq,
+                                          diag=diag).items()}
+    off = [int(x) for x in lay.off]
+    lib = L.load()
+    L.call("ucfd_set_config", 5, 5, 3)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
+    L.call("ucfd_set_stream", stream.cuda_stream)
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(stream)
+        lib.parallel_pre_lusgs(N, F, 1.0, P["fnv"], P["dt"], P["diag"], P["fsp"])
+        if ev:
+            ev[1].record(stream)
+        for c in range(nc):
+            lib.ns_parallel_lower_sweep(off[c], off[c + 1], N, F, P["nei"], P["ic"], P["lc"],
+                                        P["fnv"], P["vfn"], P["upts"], P["rhs"], P["dstar"],
+                                        P["diag"], P["fsp"])
+        if ev:
+            ev[2].record(stream)
+        for c in reversed(range(nc)):
+            lib.ns_parallel_upper_sweep(off[c], off[c + 1], N, F, P["nei"], P["ic"], P["lc"],
+                                        P["fnv"], P["vfn"], P["upts"], P["dq"], P["dstar"],
+                                        P["diag"], P["fsp"])
+        if ev:
+            ev[3].record(stream)
+        lib.lusgs_parallel_ns_update(N, P["upts"], P["dq"])
+        if ev:
+            ev[4].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    L.call("ucfd_synchronize")
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(evs[k])
+    t_end.record(stream)
+    L.call("ucfd_synchronize")
+    torch.cuda.synchronize()
+    L.call("ucfd_set_stream", None)
+    total_ms = t_start.elapsed_time(t_end)
+    ms_step = total_ms / args.steps
+    pre_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    low_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    upp_ms = sum(e[2].elapsed_time(e[3]) for e in evs) / args.steps
+    upd_ms = sum(e[3].elapsed_time(e[4]) for e in evs) / args.steps
+    log(f"per step {ms_step:.3f} ms: pre {pre_ms:.3f} lower {low_ms:.3f} upper {upp_ms:.3f} "
+        f"update {upd_ms:.3f}")
+
+    b_pre, b_sweep, b_upd = algorithmic_bytes(F)
+    bytes_cell = b_pre + 2 * b_sweep + b_upd
+    cell_iters = N / (ms_step * 1e-3)
+    gbs = bytes_cell * N / (ms_step * 1e-3) / 1e9
+    # dominant kernel: the lower sweep (one launch per colour)
+    lower_launch_ms = low_ms / nc
+    lower_bytes_launch = b_sweep * N / nc
+    achieved = lower_bytes_launch / (lower_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tr = read_traffic(os.path.join(REPO, "profiles", "traffic_lower_sweep_cfg1.txt"))
+    if tr and "hbm_bytes_per_launch" in tr:
+        traffic = tr["hbm_bytes_per_launch"]
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(lay, args.cpu_seconds)
+
+    rec = {
+        "metric": METRIC,
+        "value": cell_iters,
+        "unit": "cell-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Kuhn-tet mesh and Euler fields, ucfd_sparse_amd.mesh)",
+        "config": {
+            "workload": f"colored LU-SGS FP64 legacy API pass (pre + lower + upper + update), "
+                        f"{args.m}^3 hex x 6 Kuhn tets, permuted, greedy coloured, "
+                        f"colour-contiguous layout",
+            "neles": N, "nface": F, "ncolors": nc, "seed": args.seed,
+            "parallelism": "single GPU",
+        },
+        "hbm_gbs_algorithmic": gbs,
+        "bytes_per_cell_iter": bytes_cell,
+        "phase_ms": {"pre": pre_ms, "lower": low_ms, "upper": upp_ms, "update": upd_ms},
+        "roofline": {
+            "kernel": "lower sweep (k_sweep<0,0,3,4>), one launch per colour",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": lower_bytes_launch,
+            "avg_launch_ms": lower_launch_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

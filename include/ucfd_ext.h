@@ -1,0 +1,68 @@
+/*
+ * Extensions of the MI355X LU-SGS library that have no counterpart in the
+ * reference's public headers.  They cover what a GPU caller needs around the
+ * legacy array API (stream control, configuration of the compile-time
+ * NVARS / NFVARS / NDIMS of src/lusgs, colouring and colour-contiguous
+ * renumbering).  All functions return 0 on success unless stated otherwise.
+ */
+#ifndef UCFD_EXT_H
+#define UCFD_EXT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- execution control ---------------------------------------------------
+ * By default every call is synchronous.  After ucfd_set_stream(s) (a
+ * hipStream_t), calls whose arrays are all device resident only enqueue on
+ * s; ucfd_synchronize() waits for s and reports kernel-detected errors.
+ * ucfd_set_stream(NULL) restores the synchronous default. */
+int ucfd_set_stream(void *hip_stream);
+void *ucfd_get_stream(void);
+int ucfd_synchronize(void);
+/* number of visible HIP devices (0: every compute entry point will fail) */
+int ucfd_device_available(void);
+/* free the staging workspace used for host-pointer calls */
+int ucfd_release_workspace(void);
+
+/* ---- configuration --------------------------------------------------------
+ * nvars: rows touched by update (NVARS of the reference build,
+ * Makefile.inc:21); nfvars = ndims + 2: rows touched by the NS sweeps;
+ * ndims in {2, 3}.  Defaults 5/5/3, or the environment variables
+ * UCFD_NVARS / UCFD_NFVARS / UCFD_NDIMS. */
+int ucfd_set_config(int nvars, int nfvars, int ndims);
+int ucfd_get_config(int *nvars, int *nfvars, int *ndims);
+
+/* ---- colouring (host) -----------------------------------------------------
+ * First-fit greedy colouring in ascending cell id on the symmetrised face
+ * graph (self faces ignored).  Writes lcolor[neles] (1-based), icolor[neles]
+ * (cells colour by colour, ascending id within a colour; nullable) and
+ * color_off[ncolors+1] (colour c+1 is icolor[color_off[c] .. color_off[c+1]);
+ * nullable).  Returns ncolors, or < 0 on bad input.  On an unpermuted
+ * structured hex grid the result is the red-black colouring of
+ * examples/utils/pbutils.c:98-121. */
+int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
+                         int *lcolor, int *color_off);
+/* face pairs with equal colours (0 = valid colouring) */
+long long ucfd_coloring_conflicts(int neles, int nface, const int *nei_ele,
+                                  const int *lcolor);
+
+/* ---- colour-contiguous renumbering (host) ---------------------------------
+ * perm[old] = new with cells sorted by (lcolor, key, id); key (nullable) is a
+ * locality key such as a Morton code of the cell centre.  After permuting
+ * every per-cell array with perm (and nei_ele with ucfd_permute_nei), icolor
+ * becomes the identity and each colour is a contiguous range, so the sweeps
+ * read every array coalesced.  Returns ncolors. */
+int ucfd_color_reorder(int neles, const int *lcolor, const unsigned long long *key,
+                       int *perm, int *color_off);
+/* out[r*neles + perm[i]] = in[r*neles + i], r < nrows, esize 4 or 8 bytes */
+int ucfd_permute_rows(int neles, int nrows, int esize, const int *perm,
+                      const void *in, void *out);
+/* out[f*neles + perm[i]] = perm[nei[f*neles + i]] */
+int ucfd_permute_nei(int neles, int nface, const int *perm, const int *nei, int *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UCFD_EXT_H */

@@ -1,0 +1,135 @@
+"""Greedy colouring and colour-contiguous renumbering (host routines of
+liblusgs.so; no GPU needed).  Pinned by the red-black known answer of
+examples/utils/pbutils.py:39-56 on unpermuted hex grids, by an independent
+Python first-fit, by the colourings stored in the reference fixtures, and by
+validity / bound / round-trip invariants."""
+import os
+
+import numpy as np
+import pytest
+
+from refbridge import greedy_coloring_py
+from ucfd_sparse_amd import lib as L
+from ucfd_sparse_amd import mesh as M
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def color(nei):
+    nei = np.ascontiguousarray(nei, np.int32)
+    F, N = nei.shape
+    lcolor = np.empty(N, np.int32)
+    icolor = np.empty(N, np.int32)
+    off = np.empty(2 * F + 4, np.int32)
+    nc = L.load().ucfd_greedy_coloring(N, F, nei.ctypes.data, icolor.ctypes.data,
+                                       lcolor.ctypes.data, off.ctypes.data)
+    assert nc > 0
+    return nc, lcolor, icolor, off[:nc + 1]
+
+
+def pbutils_red_black(m, n, l):
+    """Known answer restating examples/utils/pbutils.py:55-72 (make_coloring):
+    even (x+y+z) cells first, then odd, each in ascending id."""
+    i = np.arange(m * n * l)
+    x, y, z = i % m, (i // m) % n, i // (m * n)
+    even = ((x + y + z) & 1) == 0
+    return np.concatenate((i[even], i[~even])).astype(np.int32), np.where(even, 1, 2)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (5, 3, 2), (6, 5, 4), (32, 32, 32)])
+def test_red_black_known_answer(dims):
+    mesh = M.hex_mesh(*dims, seed=0, permute=False)
+    nc, lcolor, icolor, off = color(mesh.nei_ele)
+    ic_ref, lc_ref = pbutils_red_black(*dims)
+    assert nc == 2
+    assert np.array_equal(lcolor, lc_ref)
+    assert np.array_equal(icolor, ic_ref)
+
+
+@pytest.mark.parametrize("kind,arg,seed", [("hex", (8, 8, 8), 1), ("hex", (9, 7, 5), 2),
+                                           ("tet", 4, 3), ("tet", 7, 5)])
+def test_greedy_matches_independent_statement(kind, arg, seed):
+    mesh = M.hex_mesh(*arg, seed=seed) if kind == "hex" else M.tet_mesh(arg, seed=seed)
+    nc, lcolor, icolor, off = color(mesh.nei_ele)
+    lc_py, ic_py = greedy_coloring_py(mesh.nei_ele)
+    assert np.array_equal(lcolor, lc_py)
+    assert np.array_equal(icolor, ic_py)
+    assert nc <= mesh.nface + 1
+    assert L.load().ucfd_coloring_conflicts(mesh.neles, mesh.nface, mesh.nei_ele.ctypes.data,
+                                            lcolor.ctypes.data) == 0
+    assert np.array_equal(np.diff(off), np.bincount(lcolor)[1:])
+
+
+@pytest.mark.parametrize("name", ["hex8p", "tet3p", "tet5p", "cfg0_hex32"])
+def test_greedy_matches_fixture_coloring(name):
+    z = np.load(os.path.join(GOLD, f"lusgs_{name}.npz"))
+    kind, params, seed = str(z["kind"]), [int(v) for v in z["params"]], int(z["seed"])
+    mesh = (M.hex_mesh(*params[:3], seed=seed, permute=bool(params[3])) if kind == "hex"
+            else M.tet_mesh(params[0], seed=seed, permute=bool(params[1])))
+    nc, lcolor, icolor, off = color(mesh.nei_ele)
+    assert np.array_equal(lcolor, z["lcolor"].astype(np.int32))
+
+
+def test_config0_coloring_has_six_colors():
+    mesh = M.hex_mesh(32, 32, 32, seed=0)
+    nc, lcolor, icolor, off = color(mesh.nei_ele)
+    assert nc == 6
+    assert np.array_equal(icolor, np.argsort(lcolor, kind="stable"))
+
+
+def test_greedy_is_deterministic_and_valid_on_nonsymmetric_input():
+    rng = np.random.default_rng(5)
+    N, F = 300, 4
+    nei = rng.integers(0, N, size=(F, N)).astype(np.int32)     # arbitrary, not symmetric
+    a = color(nei)
+    b = color(nei)
+    assert all(np.array_equal(x, y) for x, y in zip(a[1:], b[1:]))
+    assert L.load().ucfd_coloring_conflicts(N, F, nei.ctypes.data, a[1].ctypes.data) == 0
+
+
+def test_reorder_round_trip_and_contiguity():
+    mesh = M.tet_mesh(5, seed=8)
+    N, F = mesh.neles, mesh.nface
+    nc, lcolor, icolor, off = color(mesh.nei_ele)
+    lib = L.load()
+    perm = np.empty(N, np.int32)
+    off2 = np.empty(nc + 1, np.int32)
+    key = np.ascontiguousarray(mesh.key, np.uint64)
+    assert lib.ucfd_color_reorder(N, lcolor.ctypes.data, key.ctypes.data, perm.ctypes.data,
+                                  off2.ctypes.data) == nc
+    assert np.array_equal(np.sort(perm), np.arange(N))
+    assert np.array_equal(off, off2)
+    lc_new = np.empty(N, np.int32)
+    lib.ucfd_permute_rows(N, 1, 4, perm.ctypes.data, lcolor.ctypes.data, lc_new.ctypes.data)
+    assert np.all(np.diff(lc_new) >= 0)                        # colours contiguous, ascending
+    nei_new = np.empty_like(mesh.nei_ele)
+    lib.ucfd_permute_nei(N, F, perm.ctypes.data, mesh.nei_ele.ctypes.data, nei_new.ctypes.data)
+    # unpermute: old = inv[new]
+    inv = np.empty(N, np.int32)
+    inv[perm] = np.arange(N, dtype=np.int32)
+    assert np.array_equal(inv[nei_new[:, perm]], mesh.nei_ele)
+    vfn_new = np.empty_like(mesh.vec_fnorm)
+    lib.ucfd_permute_rows(N, F * 3, 8, perm.ctypes.data, mesh.vec_fnorm.ctypes.data,
+                          vfn_new.ctypes.data)
+    assert np.array_equal(vfn_new[..., perm], mesh.vec_fnorm)
+    # within a colour the locality key is ascending
+    k_new = np.empty(N, np.uint64)
+    k_new[perm] = key
+    for c in range(nc):
+        assert np.all(np.diff(k_new[off[c]:off[c + 1]].astype(np.int64)) >= 0)
+    # the renumbered colouring is still valid
+    assert lib.ucfd_coloring_conflicts(N, F, nei_new.ctypes.data, lc_new.ctypes.data) == 0
+
+
+@pytest.mark.slow
+def test_benchmark_mesh_colorings_fixture():
+    path = os.path.join(GOLD, "colorings.npz")
+    if not os.path.exists(path):
+        pytest.skip("colorings.npz not generated")
+    import hashlib
+    z = np.load(path)
+    for name, m, seed in (("cfg4_tet55", 55, 4), ("cfg3_tet70", 70, 3), ("cfg1_tet96", 96, 1)):
+        mesh = M.tet_mesh(m, seed=seed)
+        nc, lcolor, icolor, off = color(mesh.nei_ele)
+        assert np.array_equal(np.bincount(lcolor)[1:], z[name + "_counts"])
+        assert hashlib.sha256(lcolor.tobytes()).hexdigest() == str(z[name + "_digest"])

@@ -1,0 +1,359 @@
+"""GPU parity of the FP64 legacy LU-SGS API against the CPU oracle (itself
+pinned bit for bit to the compiled reference) and the golden fixtures.
+
+Bar: FP64 results within 1e-12 relative of the oracle (the kernels keep the
+reference's operation order without FMA contraction, so they are expected
+to agree bit for bit; the test reports the largest difference), integer
+colouring bit-exact.  All calls go through the C-ABI (ctypes over
+liblusgs.so), with host arrays (staged by the library) and with device
+arrays (torch tensors, used in place)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from refbridge import greedy_coloring_py
+from ucfd_sparse_amd import lib as L
+from ucfd_sparse_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(REPO, "tests", "golden")
+RTOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if L.device_count() == 0:
+        pytest.fail("no GPU visible: the gpu tests must run on the MI355X")
+    L.call("ucfd_set_config", 5, 5, 3)
+    L.call("ucfd_set_stream", None)
+    yield
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+def assert_close(a, b, what):
+    e = rel_err(a, b)
+    say(f"  {what}: max rel diff {e:.3e} (bitwise equal: {np.array_equal(a, b)})")
+    assert e <= RTOL, f"{what}: {e}"
+
+
+def load_case(name):
+    z = np.load(os.path.join(GOLD, f"lusgs_{name}.npz"))
+    kind, params, seed = str(z["kind"]), [int(v) for v in z["params"]], int(z["seed"])
+    mesh = (M.hex_mesh(*params[:3], seed=seed, permute=bool(params[3])) if kind == "hex"
+            else M.tet_mesh(params[0], seed=seed, permute=bool(params[1])))
+    flow = M.flow_fields(mesh, seed=seed)
+    return z, mesh, flow
+
+
+def colored_pass_lib(mesh, flow, lcolor, icolor, to_dev=None, two_array=False):
+    """pre / lower per colour / upper per colour / update through the C-ABI."""
+    N, F = mesh.neles, mesh.nface
+    off = O.color_offsets(lcolor)
+    cv = (lambda a: a) if to_dev is None else to_dev
+    nei, ic, lc = cv(mesh.nei_ele), cv(icolor), cv(lcolor)
+    fnv, vfn, fsp, dt = cv(mesh.fnorm_vol), cv(mesh.vec_fnorm), cv(flow.fspr), cv(flow.dt)
+    upts = cv(flow.upts.copy())
+    diag = cv(np.zeros(N))
+    rhsb = cv(flow.rhsb.copy())
+    dub = cv(flow.rhsb.copy()) if not two_array else cv(np.zeros_like(flow.rhsb))
+    L.call("parallel_pre_lusgs", N, F, 1.0, fnv, dt, diag, fsp)
+    nc = len(off) - 1
+    for c in range(nc):
+        if two_array:
+            L.call("ns_parallel_lower_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
+                   fnv, vfn, upts, rhsb, dub, diag, fsp)
+        else:
+            L.call("parallel_ns_lower_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
+                   fnv, vfn, upts, dub, diag, fsp)
+    dstar = np.array(dub.cpu() if hasattr(dub, "cpu") else dub, copy=True)
+    for c in reversed(range(nc)):
+        if two_array:
+            L.call("ns_parallel_upper_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
+                   fnv, vfn, upts, rhsb, dub, diag, fsp)
+        else:
+            L.call("parallel_ns_upper_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
+                   fnv, vfn, upts, dub, diag, fsp)
+    x = rhsb if two_array else dub
+    L.call("lusgs_parallel_ns_update", N, upts, x)
+
+    def h(a):
+        return np.array(a.cpu() if hasattr(a, "cpu") else a)
+    return h(diag), dstar, h(x), h(upts)
+
+
+def torch_dev():
+    import torch
+
+    def cv(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    return cv
+
+
+@pytest.mark.parametrize("name", ["hex8p", "hex6x5x4", "tet3p", "tet5p", "cfg0_hex32"])
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_colored_matches_oracle_and_fixture(name, where):
+    say(f"case {name} ({where} arrays)")
+    z, mesh, flow = load_case(name)
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    to_dev = torch_dev() if where == "device" else None
+    diag, dstar, dq, upts = colored_pass_lib(mesh, flow, lcolor, icolor, to_dev)
+    od, os_, oq, ou = O.colored_pass(mesh.nei_ele, icolor, lcolor, mesh.fnorm_vol,
+                                     mesh.vec_fnorm, flow.fspr, flow.dt, flow.upts, flow.rhsb)
+    assert_close(diag, od, "diag")
+    assert_close(dstar, os_, "dQ*")
+    assert_close(dq, oq, "dQ")
+    assert_close(upts, ou, "upts")
+    idx = z["sample"]
+    assert_close(dq[:, idx], z["c_dq"], "dQ vs reference fixture")
+
+
+def test_two_array_form_equals_in_place_form():
+    z, mesh, flow = load_case("tet5p")
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    a = colored_pass_lib(mesh, flow, lcolor, icolor, torch_dev(), two_array=False)
+    b = colored_pass_lib(mesh, flow, lcolor, icolor, torch_dev(), two_array=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", ["hex8p", "tet3p", "tet5p"])
+def test_serial_matches_oracle(name):
+    say(f"serial case {name}")
+    z, mesh, flow = load_case(name)
+    N, F = mesh.neles, mesh.nface
+    cv = torch_dev()
+    nei, fnv, vfn, fsp, dt = (cv(mesh.nei_ele), cv(mesh.fnorm_vol), cv(mesh.vec_fnorm),
+                              cv(flow.fspr), cv(flow.dt))
+    upts, dub, diag = cv(flow.upts.copy()), cv(flow.rhsb.copy()), cv(np.zeros(N))
+    L.call("serial_pre_lusgs", N, F, 1.0, fnv, dt, diag, fsp)
+    L.call("serial_ns_lower_sweep", N, F, nei, fnv, vfn, upts, dub, diag, fsp)
+    dstar = dub.cpu().numpy().copy()
+    L.call("serial_ns_upper_sweep", N, F, nei, fnv, vfn, upts, dub, diag, fsp)
+    L.call("serial_ns_lusgs_update", N, upts, dub)
+    od, os_, oq, ou = O.serial_pass(mesh.nei_ele, mesh.fnorm_vol, mesh.vec_fnorm, flow.fspr,
+                                    flow.dt, flow.upts, flow.rhsb)
+    assert_close(dstar, os_, "serial dQ*")
+    assert_close(dub.cpu().numpy(), oq, "serial dQ")
+    assert_close(upts.cpu().numpy(), ou, "serial upts")
+    idx = z["sample"]
+    assert_close(dub.cpu().numpy()[:, idx], z["s_dq"], "serial dQ vs reference fixture")
+    # Python two-array spelling, host arrays
+    upts_h, rhsb_h, dub_h, diag_h = flow.upts.copy(), flow.rhsb.copy(), np.zeros_like(flow.rhsb), \
+        np.zeros(N)
+    L.call("serial_pre_lusgs", N, F, 1.0, mesh.fnorm_vol, flow.dt, diag_h, flow.fspr)
+    L.call("ns_serial_lower_sweep", N, F, mesh.nei_ele, mesh.fnorm_vol, mesh.vec_fnorm, upts_h,
+           rhsb_h, dub_h, diag_h, flow.fspr)
+    L.call("ns_serial_upper_sweep", N, F, mesh.nei_ele, mesh.fnorm_vol, mesh.vec_fnorm, upts_h,
+           rhsb_h, dub_h, diag_h, flow.fspr)
+    L.call("lusgs_serial_ns_update", N, upts_h, rhsb_h)
+    assert_close(rhsb_h, oq, "serial two-array dQ")
+    assert_close(upts_h, ou, "serial two-array upts")
+
+
+def test_config0_ten_iterations_match_reference_fixture():
+    say("config 0: 32^3 hex, 10 iterations, rhsb restored each iteration")
+    z, mesh, flow = load_case("cfg0_hex32")
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    cv = torch_dev()
+    u = flow.upts.copy()
+    for _ in range(10):
+        f2 = M.Flow(u, flow.fspr, flow.dt, flow.rhsb)
+        _, _, _, u = colored_pass_lib(mesh, f2, lcolor, icolor, cv)
+    assert_close(u[:, z["sample"]], z["c_uptsN"], "upts after 10 iterations vs reference")
+
+
+def reorder(mesh, flow, lcolor):
+    """Colour-contiguous renumbering with a Morton locality key."""
+    lib = L.load()
+    N, F = mesh.neles, mesh.nface
+    perm = np.empty(N, np.int32)
+    nc = int(lcolor.max())
+    off = np.empty(nc + 1, np.int32)
+    key = np.ascontiguousarray(mesh.key, np.uint64)
+    lib.ucfd_color_reorder(N, lcolor.ctypes.data, key.ctypes.data, perm.ctypes.data,
+                           off.ctypes.data)
+
+    def p(a):
+        out = np.empty_like(a)
+        out[..., perm] = a
+        return out
+    nei = np.empty_like(mesh.nei_ele)
+    lib.ucfd_permute_nei(N, F, perm.ctypes.data, mesh.nei_ele.ctypes.data, nei.ctypes.data)
+    m2 = M.Mesh(nei, p(mesh.fnorm_vol), p(mesh.vec_fnorm), p(mesh.orig_id), p(mesh.key))
+    f2 = M.Flow(p(flow.upts), p(flow.fspr), p(flow.dt), p(flow.rhsb))
+    return m2, f2, p(lcolor), np.arange(N, dtype=np.int32), perm
+
+
+def test_color_contiguous_layout_gives_identical_results():
+    z, mesh, flow = load_case("tet5p")
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    a = colored_pass_lib(mesh, flow, lcolor, icolor, torch_dev())
+    m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
+    b = colored_pass_lib(m2, f2, lc2, ic2, torch_dev())
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y[..., perm])
+
+
+def test_bad_neighbour_index_is_reported_not_faulted():
+    z, mesh, flow = load_case("tet3p")
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    nei = mesh.nei_ele.copy()
+    nei[1, 5] = mesh.neles + 1000            # out of range: the kernel must skip and flag it
+    N, F = mesh.neles, mesh.nface
+    diag = np.zeros(N)
+    L.call("parallel_pre_lusgs", N, F, 1.0, mesh.fnorm_vol, flow.dt, diag, flow.fspr)
+    dub = flow.rhsb.copy()
+    rc = L.load().parallel_ns_lower_sweep(0, N, N, F, nei.ctypes.data, icolor.ctypes.data,
+                                          lcolor.ctypes.data, mesh.fnorm_vol.ctypes.data,
+                                          mesh.vec_fnorm.ctypes.data, flow.upts.ctypes.data,
+                                          dub.ctypes.data, diag.ctypes.data,
+                                          flow.fspr.ctypes.data)
+    assert rc == 3
+    # the library keeps working after the reported error
+    test_two_array_form_equals_in_place_form()
+
+
+def test_async_stream_mode_matches_sync():
+    import torch
+    z, mesh, flow = load_case("tet5p")
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    a = colored_pass_lib(mesh, flow, lcolor, icolor, torch_dev())
+    s = torch.cuda.Stream()
+    L.call("ucfd_set_stream", s.cuda_stream)
+    try:
+        cv = torch_dev()
+        N, F = mesh.neles, mesh.nface
+        off = O.color_offsets(lcolor)
+        nei, ic, lc = cv(mesh.nei_ele), cv(icolor), cv(lcolor)
+        fnv, vfn, fsp, dt = cv(mesh.fnorm_vol), cv(mesh.vec_fnorm), cv(flow.fspr), cv(flow.dt)
+        upts, dub, diag = cv(flow.upts.copy()), cv(flow.rhsb.copy()), cv(np.zeros(N))
+        torch.cuda.synchronize()
+        L.call("parallel_pre_lusgs", N, F, 1.0, fnv, dt, diag, fsp)
+        for c in range(len(off) - 1):
+            L.call("parallel_ns_lower_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
+                   fnv, vfn, upts, dub, diag, fsp)
+        for c in reversed(range(len(off) - 1)):
+            L.call("parallel_ns_upper_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
+                   fnv, vfn, upts, dub, diag, fsp)
+        L.call("lusgs_parallel_ns_update", N, upts, dub)
+        L.call("ucfd_synchronize")
+    finally:
+        L.call("ucfd_set_stream", None)
+    assert np.array_equal(upts.cpu().numpy(), a[3])
+    assert np.array_equal(dub.cpu().numpy(), a[2])
+
+
+def test_determinism():
+    z, mesh, flow = load_case("hex8p")
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    a = colored_pass_lib(mesh, flow, lcolor, icolor, torch_dev())
+    b = colored_pass_lib(mesh, flow, lcolor, icolor, torch_dev())
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_two_dimensional_quads():
+    """NDIMS = 2 (nfvars = 4) through ucfd_set_config, on a quad grid."""
+    h = M.hex_mesh(9, 7, 1, seed=6)
+    N = h.neles
+    m2 = M.Mesh(np.ascontiguousarray(h.nei_ele[:4]), np.ascontiguousarray(h.fnorm_vol[:4]),
+                np.ascontiguousarray(h.vec_fnorm[:4, :2]), h.orig_id, h.key)
+    flow = M.flow_fields(m2, seed=6)
+    lcolor, icolor = greedy_coloring_py(m2.nei_ele)
+    L.call("ucfd_set_config", 4, 4, 2)
+    try:
+        a = colored_pass_lib(m2, flow, lcolor, icolor, torch_dev())
+    finally:
+        L.call("ucfd_set_config", 5, 5, 3)
+    od, os_, oq, ou = O.colored_pass(m2.nei_ele, icolor, lcolor, m2.fnorm_vol, m2.vec_fnorm,
+                                     flow.fspr, flow.dt, flow.upts, flow.rhsb)
+    assert_close(a[2], oq, "2-D dQ")
+    assert_close(a[3], ou, "2-D upts")
+
+
+def test_c_abi_program_matches_oracle():
+    exe = os.path.join(REPO, "tests", "abi", "legacy_calls.x")
+    if not os.path.exists(exe):
+        pytest.fail("tests/abi/legacy_calls.x not built (run __graft_entry__.build())")
+    out = os.path.join(REPO, "gpurun_out", "legacy_calls.bin")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REPO, "ucfd_sparse_amd"))
+    r = subprocess.run([exe, out, "6", "5", "4"], capture_output=True, text=True, timeout=120,
+                       env=env)
+    say(r.stdout.strip())
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    m, n, l, N = np.frombuffer(raw[:16], np.int32)
+    pos = 16
+
+    def take(dtype, count):
+        nonlocal pos
+        a = np.frombuffer(raw[pos:pos + count * np.dtype(dtype).itemsize], dtype)
+        pos += count * np.dtype(dtype).itemsize
+        return a
+    nei = take(np.int32, 6 * N).reshape(6, N)
+    icolor = take(np.int32, N)
+    lcolor = take(np.int32, N)
+    fnv = take(np.float64, 6 * N).reshape(6, N)
+    vfn = take(np.float64, 18 * N).reshape(6, 3, N)
+    fspr = take(np.float64, 6 * N).reshape(6, N)
+    dt = take(np.float64, N)
+    upts0 = take(np.float64, 5 * N).reshape(5, N)
+    rhs0 = take(np.float64, 5 * N).reshape(5, N)
+    od, os_, oq, ou = O.colored_pass(nei, icolor, lcolor, fnv, vfn, fspr, dt, upts0, rhs0)
+    assert_close(take(np.float64, N), od, "C caller diag")
+    assert_close(take(np.float64, 5 * N).reshape(5, N), os_, "C caller dQ*")
+    assert_close(take(np.float64, 5 * N).reshape(5, N), oq, "C caller dQ")
+    assert_close(take(np.float64, 5 * N).reshape(5, N), ou, "C caller upts")
+    take(np.float64, 5 * N)
+    assert_close(take(np.float64, 5 * N).reshape(5, N), oq, "C caller two-array dQ")
+    assert_close(take(np.float64, 5 * N).reshape(5, N), ou, "C caller two-array upts")
+    sd, ss, sq, su = O.serial_pass(nei, fnv, vfn, fspr, dt, upts0, rhs0)
+    assert_close(take(np.float64, 5 * N).reshape(5, N), sq, "C caller serial dQ")
+    assert_close(take(np.float64, 5 * N).reshape(5, N), su, "C caller serial upts")
+
+
+def test_config1_size_fp64_matches_oracle():
+    """Full benchmark size (96^3 x 6 tets = 5,308,416 cells), colour-contiguous
+    layout, one iteration against the OpenMP oracle."""
+    import torch
+    say("config 1 size: generating 5.3M-cell tet mesh")
+    mesh = M.tet_mesh(96, seed=1)
+    flow = M.flow_fields(mesh, seed=1)
+    N = mesh.neles
+    lcolor = np.empty(N, np.int32)
+    icolor = np.empty(N, np.int32)
+    L.load().ucfd_greedy_coloring(N, 4, mesh.nei_ele.ctypes.data, icolor.ctypes.data,
+                                  lcolor.ctypes.data, None)
+    m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
+    del mesh, flow
+    say("config 1 size: GPU pass")
+    diag, dstar, dq, upts = colored_pass_lib(m2, f2, lc2, ic2, torch_dev())
+    torch.cuda.empty_cache()
+    say("config 1 size: oracle pass")
+    od, os_, oq, ou = O.colored_pass(m2.nei_ele, ic2, lc2, m2.fnorm_vol, m2.vec_fnorm, f2.fspr,
+                                     f2.dt, f2.upts, f2.rhsb)
+    assert_close(dq, oq, "config 1 dQ")
+    assert_close(upts, ou, "config 1 upts")

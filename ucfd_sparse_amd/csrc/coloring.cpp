@@ -1,0 +1,167 @@
+// Greedy multi-colouring and colour-contiguous renumbering of a cell mesh.
+//
+// The reference's own colouring routine is absent from its tree; its only
+// colouring code is the red-black helper examples/utils/pbutils.c:98-121.
+// The routine here is first-fit in ascending cell id: cell i takes the
+// smallest colour (1-based) not used by an already coloured face neighbour
+// (a neighbour with a smaller id), on the symmetrised face graph; boundary
+// self-faces are ignored.  On an unpermuted structured hex grid this is
+// exactly the red-black colouring and icolor order of pbutils, and it never
+// needs more than nface + 1 colours.  icolor lists the cells colour by
+// colour, ascending id within a colour (the convention of
+// examples/omp3d.c:173-175).
+#include "ucfd_internal.h"
+#include "ucfd_ext.h"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+// symmetrised adjacency (CSR) of the face graph, self faces dropped
+void sym_graph(int N, int F, const int *nei, std::vector<int> &ptr, std::vector<int> &adj)
+{
+    std::vector<int> deg(N, 0);
+    for (int f = 0; f < F; ++f)
+        for (int i = 0; i < N; ++i) {
+            const int nb = nei[(size_t)f*N + i];
+            if (nb < 0 || nb >= N || nb == i) continue;
+            deg[i]++;
+            deg[nb]++;
+        }
+    ptr.assign(N + 1, 0);
+    for (int i = 0; i < N; ++i) ptr[i + 1] = ptr[i] + deg[i];
+    adj.assign(ptr[N], 0);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int f = 0; f < F; ++f)
+        for (int i = 0; i < N; ++i) {
+            const int nb = nei[(size_t)f*N + i];
+            if (nb < 0 || nb >= N || nb == i) continue;
+            adj[pos[i]++] = nb;
+            adj[pos[nb]++] = i;
+        }
+}
+
+}  // namespace
+
+// Returns the number of colours (>= 1), or a negative value on bad input.
+UCFD_API int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
+                                  int *lcolor, int *color_off)
+{
+    if (neles < 0 || nface < 1 || !nei_ele || !lcolor) return -1;
+    if (neles == 0) {
+        if (color_off) color_off[0] = 0;
+        return 0;
+    }
+    const int N = neles;
+    std::vector<int> ptr, adj;
+    sym_graph(N, nface, nei_ele, ptr, adj);
+    std::vector<int> mark(2*nface + 8, -1);
+    int ncol = 0;
+    for (int i = 0; i < N; ++i) {
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+            const int nb = adj[e];
+            if (nb < i) {
+                const int c = lcolor[nb];
+                if (c >= (int)mark.size()) mark.resize(2*c + 2, -1);
+                mark[c] = i;
+            }
+        }
+        int c = 1;
+        while (c < (int)mark.size() && mark[c] == i) ++c;
+        if (c >= (int)mark.size()) mark.resize(2*c + 2, -1);
+        lcolor[i] = c;
+        ncol = std::max(ncol, c);
+    }
+    std::vector<int> off(ncol + 2, 0);
+    for (int i = 0; i < N; ++i) off[lcolor[i] + 1]++;
+    for (int c = 1; c <= ncol; ++c) off[c + 1] += off[c];
+    if (icolor) {
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (int i = 0; i < N; ++i) icolor[pos[lcolor[i]]++] = i;
+    }
+    if (color_off)
+        for (int c = 0; c <= ncol; ++c) color_off[c] = off[c + 1];
+    return ncol;
+}
+
+// Number of face pairs (i, nb != i) with lcolor[i] == lcolor[nb]; 0 means a
+// valid colouring.  Out-of-range neighbours count as conflicts.
+UCFD_API long long ucfd_coloring_conflicts(int neles, int nface, const int *nei_ele,
+                                           const int *lcolor)
+{
+    long long bad = 0;
+    const int N = neles;
+    for (int f = 0; f < nface; ++f)
+        for (int i = 0; i < N; ++i) {
+            const int nb = nei_ele[(size_t)f*N + i];
+            if (nb < 0 || nb >= N) { ++bad; continue; }
+            if (nb != i && lcolor[nb] == lcolor[i]) ++bad;
+        }
+    return bad;
+}
+
+// Colour-contiguous renumbering: perm[old] = new position, with cells sorted
+// by (lcolor, key, old id).  key may be NULL (then ascending id within a
+// colour, i.e. new order = icolor).  color_off[c] .. color_off[c+1] is the
+// new range of colour c+1 (ncolors+1 entries).  Returns ncolors.
+UCFD_API int ucfd_color_reorder(int neles, const int *lcolor, const unsigned long long *key,
+                                int *perm, int *color_off)
+{
+    if (neles < 0 || !lcolor || !perm) return -1;
+    const int N = neles;
+    std::vector<int> idx(N);
+    std::iota(idx.begin(), idx.end(), 0);
+    int ncol = 0;
+    for (int i = 0; i < N; ++i) ncol = std::max(ncol, lcolor[i]);
+    if (key)
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+            if (lcolor[a] != lcolor[b]) return lcolor[a] < lcolor[b];
+            return key[a] < key[b];
+        });
+    else
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return lcolor[a] < lcolor[b]; });
+    for (int q = 0; q < N; ++q) perm[idx[q]] = q;
+    if (color_off) {
+        std::vector<int> off(ncol + 2, 0);
+        for (int i = 0; i < N; ++i) off[lcolor[i] + 1]++;
+        for (int c = 1; c <= ncol; ++c) off[c + 1] += off[c];
+        for (int c = 0; c <= ncol; ++c) color_off[c] = off[c + 1];
+    }
+    return ncol;
+}
+
+// out[r*N + perm[i]] = in[r*N + i] for rows r < nrows (element size esize).
+UCFD_API int ucfd_permute_rows(int neles, int nrows, int esize, const int *perm, const void *in,
+                               void *out)
+{
+    if (neles < 0 || nrows < 0 || (esize != 4 && esize != 8) || !perm || !in || !out) return 1;
+    const size_t N = (size_t)neles;
+    for (int r = 0; r < nrows; ++r) {
+        if (esize == 8) {
+            const double *a = (const double *)in + r*N;
+            double *b = (double *)out + r*N;
+            for (size_t i = 0; i < N; ++i) b[perm[i]] = a[i];
+        } else {
+            const int *a = (const int *)in + r*N;
+            int *b = (int *)out + r*N;
+            for (size_t i = 0; i < N; ++i) b[perm[i]] = a[i];
+        }
+    }
+    return 0;
+}
+
+// Renumbered neighbour table: out[f*N + perm[i]] = perm[nei[f*N + i]].
+UCFD_API int ucfd_permute_nei(int neles, int nface, const int *perm, const int *nei, int *out)
+{
+    if (neles < 0 || nface < 1 || !perm || !nei || !out) return 1;
+    const size_t N = (size_t)neles;
+    for (int f = 0; f < nface; ++f)
+        for (size_t i = 0; i < N; ++i) {
+            const int nb = nei[f*N + i];
+            out[f*N + perm[i]] = (nb >= 0 && (size_t)nb < N) ? perm[nb] : nb;
+        }
+    return 0;
+}

@@ -1,0 +1,293 @@
+// Legacy LU-SGS array API (the one the reference examples and pyBaram call):
+// {serial,parallel}_pre_lusgs, the C in-place sweeps {serial,parallel}_ns_*_sweep,
+// the Python two-array sweeps ns_{serial,parallel}_*_sweep and the update
+// spellings.  Signatures are recovered from the reference call sites (the
+// implementation files are absent from the reference):
+//   examples/omp3d.c:180-189, examples/mpi3d.c:166-173,
+//   examples/omp3d.py:22-35,76-94, examples/mpi3d.py:21-34,71-81.
+//
+// Colored calls launch one kernel over the cells icolor[n0:ne); a neighbour is
+// "lower" iff lcolor[nb] < lcolor[i] (upper: >).  Serial calls keep the
+// sequential semantics of src/lusgs/lusgs.c:175,227 (lower iff nb < i) and run
+// them level by level: a cell's level is one more than the highest level of
+// its lower neighbours, so each level is independent and the result equals
+// the sequential sweep exactly.
+#include "ucfd_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+#include "coloredlusgs.h"
+#include "lusgs.h"
+
+namespace {
+
+bool bad_dims(int neles, int nface) { return neles < 0 || nface < 1 || nface > 32; }
+
+int do_pre(const char *fn, int neles, int nface, double factor, const double *fnorm_vol,
+           const double *dt, double *diag, const double *fspr)
+{
+    if (bad_dims(neles, nface)) return ucfd_report(UCFD_ERR_ARG, fn, "bad neles/nface");
+    if (neles == 0) return UCFD_OK;
+    const size_t N = (size_t)neles, F = (size_t)nface;
+    UcfdStage sg(fn);
+    const int a_fnv = sg.add(fnorm_vol, F*N*8, UCFD_IN);
+    const int a_dt = sg.add(dt, N*8, UCFD_IN);
+    const int a_dg = sg.add(diag, N*8, UCFD_OUT);
+    const int a_sp = sg.add(fspr, F*N*8, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return rc;
+    rc = ucfd_launch_pre(ucfd_stream(), neles, nface, factor, (const double *)sg.dev(a_fnv),
+                         (const double *)sg.dev(a_dt), (double *)sg.dev(a_dg),
+                         (const double *)sg.dev(a_sp));
+    if (rc) ucfd_report(rc, fn, "kernel launch failed");
+    return sg.leave(rc);
+}
+
+// One coloured sweep over icolor[n0:ne).  lower: out = (in - 0.5 df)/diag;
+// upper: out = in - 0.5 df/diag.  in == out is the in-place C form.
+int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, const int *nei,
+               const int *icolor, const int *lcolor, const double *fnorm_vol,
+               const double *vec_fnorm, const double *upts, const double *in, double *out,
+               const double *diag, const double *fspr)
+{
+    if (bad_dims(neles, nface) || n0 < 0 || ne < n0 || ne > neles)
+        return ucfd_report(UCFD_ERR_ARG, fn, "bad n0/ne/neles/nface");
+    if (ne == n0) return UCFD_OK;
+    const UcfdConfig &c = ucfd_config();
+    const size_t N = (size_t)neles, F = (size_t)nface, D = (size_t)c.ndims, V = (size_t)c.nfvars;
+    UcfdStage sg(fn);
+    const int a_nei = sg.add(nei, F*N*4, UCFD_IN);
+    const int a_ic = sg.add(icolor, N*4, UCFD_IN);
+    const int a_lc = sg.add(lcolor, N*4, UCFD_IN);
+    const int a_fnv = sg.add(fnorm_vol, F*N*8, UCFD_IN);
+    const int a_vfn = sg.add(vec_fnorm, F*D*N*8, UCFD_IN);
+    const int a_u = sg.add(upts, V*N*8, UCFD_IN);
+    const int a_in = sg.add(in, V*N*8, UCFD_IN);
+    const int a_out = sg.add(out, V*N*8, UCFD_INOUT);
+    const int a_dg = sg.add(diag, N*8, UCFD_IN);
+    const int a_sp = sg.add(fspr, F*N*8, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return rc;
+    rc = ucfd_launch_sweep(ucfd_stream(), dir, UCFD_SWEEP_COLORED, n0, ne, neles, nface, c.ndims,
+                           c.nfvars, (const int *)sg.dev(a_nei), (const int *)sg.dev(a_ic),
+                           (const int *)sg.dev(a_lc), (const double *)sg.dev(a_fnv),
+                           (const double *)sg.dev(a_vfn), (const double *)sg.dev(a_u),
+                           (const double *)sg.dev(a_in), (double *)sg.dev(a_out),
+                           (const double *)sg.dev(a_dg), (const double *)sg.dev(a_sp),
+                           ucfd_device_flag());
+    if (rc) ucfd_report(rc, fn, "kernel launch failed");
+    return sg.leave(rc);
+}
+
+// Dependency levels of the sequential sweep: lower (nb < i) or upper (nb > i).
+// order[] lists cells level by level, off[l]..off[l+1] is level l.
+void build_levels(int dir, int N, int F, const int *nei, std::vector<int> &order,
+                  std::vector<int> &off)
+{
+    std::vector<int> lev(N, 0);
+    int maxl = 0;
+    if (dir == 0) {
+        for (int i = 0; i < N; ++i) {
+            int l = 0;
+            for (int f = 0; f < F; ++f) {
+                const int nb = nei[(size_t)f*N + i];
+                if (nb >= 0 && nb < i) l = std::max(l, lev[nb] + 1);
+            }
+            lev[i] = l;
+            maxl = std::max(maxl, l);
+        }
+    } else {
+        for (int i = N - 1; i >= 0; --i) {
+            int l = 0;
+            for (int f = 0; f < F; ++f) {
+                const int nb = nei[(size_t)f*N + i];
+                if (nb > i && nb < N) l = std::max(l, lev[nb] + 1);
+            }
+            lev[i] = l;
+            maxl = std::max(maxl, l);
+        }
+    }
+    off.assign(maxl + 2, 0);
+    for (int i = 0; i < N; ++i) off[lev[i] + 1]++;
+    for (int l = 0; l <= maxl; ++l) off[l + 1] += off[l];
+    order.resize(N);
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int i = 0; i < N; ++i) order[pos[lev[i]]++] = i;
+}
+
+int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
+              const double *fnorm_vol, const double *vec_fnorm, const double *upts,
+              const double *in, double *out, const double *diag, const double *fspr)
+{
+    if (bad_dims(neles, nface)) return ucfd_report(UCFD_ERR_ARG, fn, "bad neles/nface");
+    if (neles == 0) return UCFD_OK;
+    const UcfdConfig &c = ucfd_config();
+    const size_t N = (size_t)neles, F = (size_t)nface, D = (size_t)c.ndims, V = (size_t)c.nfvars;
+    int rc = ucfd_ensure_device(fn);
+    if (rc) return rc;
+
+    // the level schedule is built on the host from nei_ele
+    std::vector<int> hnei;
+    const int *nei_h = nei;
+    if (!nei) return ucfd_report(UCFD_ERR_ARG, fn, "NULL nei_ele");
+    if (ucfd_is_device_ptr(nei)) {
+        hnei.resize(F*N);
+        rc = ucfd_check_hip(hipMemcpyAsync(hnei.data(), nei, F*N*4, hipMemcpyDeviceToHost,
+                                           ucfd_stream()), fn, "hipMemcpyAsync(nei_ele)");
+        if (!rc) rc = ucfd_check_hip(hipStreamSynchronize(ucfd_stream()), fn, "sync");
+        if (rc) return rc;
+        nei_h = hnei.data();
+    }
+    std::vector<int> order, off;
+    build_levels(dir, neles, nface, nei_h, order, off);
+
+    UcfdStage sg(fn);
+    const int a_nei = sg.add(nei, F*N*4, UCFD_IN);
+    const int a_ord = sg.add(order.data(), N*4, UCFD_IN);
+    const int a_fnv = sg.add(fnorm_vol, F*N*8, UCFD_IN);
+    const int a_vfn = sg.add(vec_fnorm, F*D*N*8, UCFD_IN);
+    const int a_u = sg.add(upts, V*N*8, UCFD_IN);
+    const int a_in = sg.add(in, V*N*8, UCFD_IN);
+    const int a_out = sg.add(out, V*N*8, UCFD_INOUT);
+    const int a_dg = sg.add(diag, N*8, UCFD_IN);
+    const int a_sp = sg.add(fspr, F*N*8, UCFD_IN);
+    rc = sg.enter();
+    if (rc) return rc;
+    const int nlev = (int)off.size() - 1;
+    for (int l = 0; l < nlev && !rc; ++l) {
+        rc = ucfd_launch_sweep(ucfd_stream(), dir, UCFD_SWEEP_ORDERED, off[l], off[l + 1], neles,
+                               nface, c.ndims, c.nfvars, (const int *)sg.dev(a_nei),
+                               (const int *)sg.dev(a_ord), nullptr,
+                               (const double *)sg.dev(a_fnv), (const double *)sg.dev(a_vfn),
+                               (const double *)sg.dev(a_u), (const double *)sg.dev(a_in),
+                               (double *)sg.dev(a_out), (const double *)sg.dev(a_dg),
+                               (const double *)sg.dev(a_sp), ucfd_device_flag());
+    }
+    if (rc) ucfd_report(rc, fn, "kernel launch failed");
+    // the level order is a host array, so it is staged and leave() always
+    // waits for the kernels before this frame releases it
+    return sg.leave(rc);
+}
+
+int do_update(const char *fn, int neles, double *upts, const double *x)
+{
+    if (neles < 0) return ucfd_report(UCFD_ERR_ARG, fn, "bad neles");
+    if (neles == 0) return UCFD_OK;
+    const size_t n = (size_t)neles*(size_t)ucfd_config().nvars;
+    UcfdStage sg(fn);
+    const int a_u = sg.add(upts, n*8, UCFD_INOUT);
+    const int a_x = sg.add(x, n*8, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return rc;
+    rc = ucfd_launch_axpy_rows(ucfd_stream(), (int64_t)n, (double *)sg.dev(a_u),
+                               (const double *)sg.dev(a_x));
+    if (rc) ucfd_report(rc, fn, "kernel launch failed");
+    return sg.leave(rc);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ coloured
+
+UCFD_API int parallel_pre_lusgs(int neles, int nface, double factor, double *fnorm_vol,
+                                double *dt, double *diag, double *fspr)
+{
+    return do_pre("parallel_pre_lusgs", neles, nface, factor, fnorm_vol, dt, diag, fspr);
+}
+
+UCFD_API int parallel_ns_lower_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                     int *icolor, int *lcolor, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *dub, double *diag,
+                                     double *fspr)
+{
+    return do_colored("parallel_ns_lower_sweep", 0, n0, ne, neles, nface, nei_ele, icolor, lcolor,
+                      fnorm_vol, vec_fnorm, upts, dub, dub, diag, fspr);
+}
+
+UCFD_API int parallel_ns_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                     int *icolor, int *lcolor, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *dub, double *diag,
+                                     double *fspr)
+{
+    return do_colored("parallel_ns_upper_sweep", 1, n0, ne, neles, nface, nei_ele, icolor, lcolor,
+                      fnorm_vol, vec_fnorm, upts, dub, dub, diag, fspr);
+}
+
+UCFD_API int ns_parallel_lower_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                     int *icolor, int *lcolor, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *rhsb, double *dub,
+                                     double *diag, double *fspr)
+{
+    return do_colored("ns_parallel_lower_sweep", 0, n0, ne, neles, nface, nei_ele, icolor, lcolor,
+                      fnorm_vol, vec_fnorm, upts, rhsb, dub, diag, fspr);
+}
+
+UCFD_API int ns_parallel_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                     int *icolor, int *lcolor, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *rhsb, double *dub,
+                                     double *diag, double *fspr)
+{
+    return do_colored("ns_parallel_upper_sweep", 1, n0, ne, neles, nface, nei_ele, icolor, lcolor,
+                      fnorm_vol, vec_fnorm, upts, dub, rhsb, diag, fspr);
+}
+
+UCFD_API int lusgs_parallel_ns_update(int neles, double *upts, double *x)
+{
+    return do_update("lusgs_parallel_ns_update", neles, upts, x);
+}
+
+UCFD_API int parallel_ns_lusgs_update(int neles, double *upts, double *x)
+{
+    return do_update("parallel_ns_lusgs_update", neles, upts, x);
+}
+
+// -------------------------------------------------------------------- serial
+
+UCFD_API int serial_pre_lusgs(int neles, int nface, double factor, double *fnorm_vol, double *dt,
+                              double *diag, double *fspr)
+{
+    return do_pre("serial_pre_lusgs", neles, nface, factor, fnorm_vol, dt, diag, fspr);
+}
+
+UCFD_API int serial_ns_lower_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                   double *vec_fnorm, double *upts, double *dub, double *diag,
+                                   double *fspr)
+{
+    return do_serial("serial_ns_lower_sweep", 0, neles, nface, nei_ele, fnorm_vol, vec_fnorm, upts,
+                     dub, dub, diag, fspr);
+}
+
+UCFD_API int serial_ns_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                   double *vec_fnorm, double *upts, double *dub, double *diag,
+                                   double *fspr)
+{
+    return do_serial("serial_ns_upper_sweep", 1, neles, nface, nei_ele, fnorm_vol, vec_fnorm, upts,
+                     dub, dub, diag, fspr);
+}
+
+UCFD_API int ns_serial_lower_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                   double *vec_fnorm, double *upts, double *rhsb, double *dub,
+                                   double *diag, double *fspr)
+{
+    return do_serial("ns_serial_lower_sweep", 0, neles, nface, nei_ele, fnorm_vol, vec_fnorm, upts,
+                     rhsb, dub, diag, fspr);
+}
+
+UCFD_API int ns_serial_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                   double *vec_fnorm, double *upts, double *rhsb, double *dub,
+                                   double *diag, double *fspr)
+{
+    return do_serial("ns_serial_upper_sweep", 1, neles, nface, nei_ele, fnorm_vol, vec_fnorm, upts,
+                     dub, rhsb, diag, fspr);
+}
+
+UCFD_API int serial_ns_lusgs_update(int neles, double *upts, double *dub)
+{
+    return do_update("serial_ns_lusgs_update", neles, upts, dub);
+}
+
+UCFD_API int lusgs_serial_ns_update(int neles, double *upts, double *rhsb)
+{
+    return do_update("lusgs_serial_ns_update", neles, upts, rhsb);
+}

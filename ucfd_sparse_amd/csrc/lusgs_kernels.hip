@@ -1,0 +1,195 @@
+// FP64 LU-SGS kernels behind the legacy array API: pre (diagonal), the
+// lower / upper sweep over one colour (or one dependency level) and the
+// solution update.
+//
+// Sweep: one thread per cell of the launched range; the cell reads its own
+// row of every nface-major array (coalesced when the cells of a colour are
+// stored contiguously) and gathers neighbour states through nei_ele.  Cells
+// of one colour have no face neighbour of the same colour, so a launch has no
+// intra-launch dependency and no atomics.  Arithmetic follows
+// src/lusgs/lusgs.c:157-257 in operation order; this file is compiled with
+// -ffp-contract=off so no FMA is introduced.
+//
+// Every gathered index is bounds checked; an out-of-range entry raises
+// UCFD_KFLAG_INDEX in the device error word and the face (or cell) is
+// skipped, so a malformed index array cannot make the kernel fault.
+#include "flux.h"
+#include "ucfd_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void raise_flag(int *flag, int bit)
+{
+    atomicOr(flag, bit);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pre(int N, int nface, double factor,
+                                                const double *__restrict__ fnv,
+                                                const double *__restrict__ dt,
+                                                double *__restrict__ diag,
+                                                const double *__restrict__ fspr)
+{
+    const int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= N) return;
+    // lusgs.c:33 with a0 = dsrc = 0, then lusgs.c:102-110 with kappa = rcp_vol = 1
+    const double d = 1.0/(dt[i]*factor) + 0.0 + 0.0;
+    double sd = 0.0;
+    for (int f = 0; f < nface; ++f)
+        sd += 0.5*fspr[(size_t)f*N + i]*fnv[(size_t)f*N + i];
+    diag[i] = d + sd;
+}
+
+// DIR 0: lower, out_i = (in_i - 0.5 df)/diag_i
+// DIR 1: upper, out_i = in_i - (0.5 df)/diag_i
+// MODE 0 (colored): neighbour taken iff lcolor[nb] </> lcolor[i]
+// MODE 1 (ordered): neighbour taken iff nb </> i
+template <int DIR, int MODE, int ND, int FC>
+__global__ __launch_bounds__(kBlock) void k_sweep(
+    int n0, int cnt, int N, int nface, const int *__restrict__ nei,
+    const int *__restrict__ order, const int *__restrict__ lcolor,
+    const double *__restrict__ fnv, const double *__restrict__ vfn,
+    const double *__restrict__ upts, const double *in, double *out,
+    const double *__restrict__ diag, const double *__restrict__ fspr, int *flag)
+{
+    constexpr int NV = ND + 2;
+    const int t = blockIdx.x*kBlock + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = order[n0 + t];
+    if ((unsigned)i >= (unsigned)N) {
+        raise_flag(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    const int F = FC ? FC : nface;
+    int ci = 0;
+    if (MODE == 0) ci = lcolor[i];
+
+    double df[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) df[k] = 0.0;
+
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        const int nb = nei[(size_t)f*N + i];
+        if ((unsigned)nb >= (unsigned)N) {
+            raise_flag(flag, UCFD_KFLAG_INDEX);
+            continue;
+        }
+        bool take;
+        if (MODE == 0) {
+            const int cn = lcolor[nb];
+            take = DIR == 0 ? (cn < ci) : (cn > ci);
+        } else {
+            take = DIR == 0 ? (nb < i) : (nb > i);
+        }
+        if (!take) continue;
+        double u[NV], du[NV], nf[ND], dfj[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            u[k] = upts[(size_t)k*N + nb];
+            du[k] = out[(size_t)k*N + nb];
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d) nf[d] = vfn[(size_t)(f*ND + d)*N + i];
+        const double fv = fnv[(size_t)f*N + i];
+        const double lam = fspr[(size_t)f*N + i];
+        ucfd::diff_flux<double, ND>(u, du, nf, dfj);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) df[k] += (dfj[k] - lam*du[k])*fv;
+    }
+    const double dg = diag[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const size_t o = (size_t)k*N + i;
+        if (DIR == 0) out[o] = (in[o] - 0.5*df[k])/dg;
+        else          out[o] = in[o] - (0.5*df[k]/dg);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy(int64_t n, double *__restrict__ y,
+                                                 const double *__restrict__ x)
+{
+    const int64_t stride = (int64_t)gridDim.x*kBlock;
+    for (int64_t j = (int64_t)blockIdx.x*kBlock + threadIdx.x; j < n; j += stride)
+        y[j] += x[j];
+}
+
+template <int DIR, int MODE, int ND>
+int launch_sweep_f(hipStream_t s, int n0, int cnt, int N, int nface, const int *nei,
+                   const int *order, const int *lcolor, const double *fnv, const double *vfn,
+                   const double *upts, const double *in, double *out, const double *diag,
+                   const double *fspr, int *flag)
+{
+    const dim3 grid((cnt + kBlock - 1)/kBlock), block(kBlock);
+    switch (nface) {
+    case 4:
+        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 4>), grid, block, 0, s, n0, cnt, N, nface, nei,
+                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+        break;
+    case 6:
+        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 6>), grid, block, 0, s, n0, cnt, N, nface, nei,
+                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+        break;
+    default:
+        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 0>), grid, block, 0, s, n0, cnt, N, nface, nei,
+                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+        break;
+    }
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}
+
+template <int DIR, int MODE>
+int launch_sweep_d(hipStream_t s, int ndims, int n0, int cnt, int N, int nface, const int *nei,
+                   const int *order, const int *lcolor, const double *fnv, const double *vfn,
+                   const double *upts, const double *in, double *out, const double *diag,
+                   const double *fspr, int *flag)
+{
+    if (ndims == 3)
+        return launch_sweep_f<DIR, MODE, 3>(s, n0, cnt, N, nface, nei, order, lcolor, fnv, vfn,
+                                            upts, in, out, diag, fspr, flag);
+    return launch_sweep_f<DIR, MODE, 2>(s, n0, cnt, N, nface, nei, order, lcolor, fnv, vfn, upts,
+                                        in, out, diag, fspr, flag);
+}
+
+}  // namespace
+
+int ucfd_launch_pre(hipStream_t s, int neles, int nface, double factor, const double *fnorm_vol,
+                    const double *dt, double *diag, const double *fspr)
+{
+    if (neles <= 0) return UCFD_OK;
+    hipLaunchKernelGGL(k_pre, dim3((neles + kBlock - 1)/kBlock), dim3(kBlock), 0, s, neles,
+                       nface, factor, fnorm_vol, dt, diag, fspr);
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}
+
+int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int neles, int nface,
+                      int ndims, int nfvars, const int *nei, const int *order, const int *lcolor,
+                      const double *fnorm_vol, const double *vec_fnorm, const double *upts,
+                      const double *in, double *out, const double *diag, const double *fspr,
+                      int *flag)
+{
+    const int cnt = ne - n0;
+    if (cnt <= 0) return UCFD_OK;
+    (void)nfvars;
+    if (dir == 0 && mode == UCFD_SWEEP_COLORED)
+        return launch_sweep_d<0, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    if (dir == 1 && mode == UCFD_SWEEP_COLORED)
+        return launch_sweep_d<1, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    if (dir == 0)
+        return launch_sweep_d<0, 1>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    return launch_sweep_d<1, 1>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                vec_fnorm, upts, in, out, diag, fspr, flag);
+}
+
+int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x)
+{
+    if (n <= 0) return UCFD_OK;
+    int64_t blocks = (n + kBlock - 1)/kBlock;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_axpy, dim3((unsigned)blocks), dim3(kBlock), 0, s, n, y, x);
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}

@@ -1,0 +1,349 @@
+/*
+ * Seeded synthetic meshes and flow fields for the LU-SGS workloads.
+ *
+ * Every random draw is counter based: value = mix64(seed, stream, index), so a
+ * draw depends only on (seed, stream, index) and never on evaluation order.
+ * Flow-field draws are indexed by the ORIGINAL (pre-permutation) cell id, so a
+ * physical cell keeps its state whatever numbering the mesh is stored in.
+ *
+ * Array layouts follow the legacy LU-SGS API (nface-major, C order):
+ *   nei_ele[f*N + i], fnorm_vol[f*N + i], fspr[f*N + i],
+ *   vec_fnorm[(f*ndims + d)*N + i], upts/rhsb[k*N + i], dt[i].
+ * Boundary faces point to the cell itself (examples/utils/pbutils.c:43-84).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GEN_EXPORT __attribute__((visibility("default")))
+
+enum {
+    ST_PERM = 1, ST_VOLJIT = 2, ST_RHO = 3, ST_VEL = 4, ST_PRES = 5,
+    ST_DT = 6, ST_RHSB = 7, ST_VERT = 8, ST_NU = 9, ST_PERT = 10,
+    ST_AFACE = 11, ST_VEC = 12
+};
+
+static inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t ctr64(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return mix64(seed * 0x9E3779B97F4A7C15ULL
+                 + mix64(stream * 0xD1B54A32D192ED03ULL + idx + 1));
+}
+
+/* uniform in [0, 1) with 53 random bits */
+static inline double u01(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return (double)(ctr64(seed, stream, idx) >> 11) * 0x1.0p-53;
+}
+
+/* uniform in (0, 1) (never 0, for the logarithm of Box-Muller) */
+static inline double u01_open(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return ((double)(ctr64(seed, stream, idx) >> 11) + 0.5) * 0x1.0p-53;
+}
+
+static inline double normal01(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    const double a = u01_open(seed, stream, 2*idx);
+    const double b = u01(seed, stream, 2*idx + 1);
+    return sqrt(-2.0*log(a)) * cos(6.283185307179586476925*b);
+}
+
+GEN_EXPORT double ucfdgen_uniform(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return u01(seed, stream, idx);
+}
+
+GEN_EXPORT double ucfdgen_normal(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return normal01(seed, stream, idx);
+}
+
+/*
+ * Random permutation of [0, n): perm[old] = new.  Fisher-Yates driven by the
+ * counter stream ST_PERM; perm = identity when permute == 0.
+ */
+GEN_EXPORT void ucfdgen_permutation(int64_t n, uint64_t seed, int permute, int32_t *perm)
+{
+    int32_t *a = (int32_t *)malloc(sizeof(int32_t)*(size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) a[i] = (int32_t)i;
+    if (permute) {
+        for (int64_t i = n - 1; i > 0; --i) {
+            int64_t j = (int64_t)(ctr64(seed, ST_PERM, (uint64_t)i) % (uint64_t)(i + 1));
+            int32_t t = a[i]; a[i] = a[j]; a[j] = t;
+        }
+    }
+    /* a[new] = old */
+    for (int64_t q = 0; q < n; ++q) perm[a[q]] = (int32_t)q;
+    free(a);
+}
+
+static uint64_t morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    uint64_t k = 0;
+    for (int b = 0; b < 21; ++b) {
+        k |= ((uint64_t)((x >> b) & 1u)) << (3*b);
+        k |= ((uint64_t)((y >> b) & 1u)) << (3*b + 1);
+        k |= ((uint64_t)((z >> b) & 1u)) << (3*b + 2);
+    }
+    return k;
+}
+
+/*
+ * Structured m x n x l hex grid in the unit cube, nface = 6, faces ordered
+ * -x,+x,-y,+y,-z,+z (pbutils.c:27-32), linear id x + m*y + m*n*z
+ * (pbutils.py:6-8) before the optional seeded permutation.
+ *
+ * fnorm_vol = face_area / cell_volume with the volume jittered by a factor
+ * U[1-voljit, 1+voljit] per cell.  vec_fnorm holds the matching outward
+ * axis normals.  Outputs, all in the permuted numbering:
+ *   nei_ele[6*N], fnorm_vol[6*N], vec_fnorm[6*3*N],
+ *   orig_id[N]   (original id of each stored cell; nullable),
+ *   key[N]       (Morton key of the cell position; nullable).
+ */
+GEN_EXPORT int ucfdgen_hex(int m, int n, int l, uint64_t seed, int permute, double voljit,
+                           int32_t *nei_ele, double *fnorm_vol, double *vec_fnorm,
+                           int32_t *orig_id, uint64_t *key)
+{
+    if (m <= 0 || n <= 0 || l <= 0) return 1;
+    const int64_t N = (int64_t)m*n*l, mn = (int64_t)m*n;
+    int32_t *perm = (int32_t *)malloc(sizeof(int32_t)*N);
+    if (!perm) return 2;
+    ucfdgen_permutation(N, seed, permute, perm);
+
+    const double hx = 1.0/m, hy = 1.0/n, hz = 1.0/l;
+    const double area[6] = {hy*hz, hy*hz, hx*hz, hx*hz, hx*hy, hx*hy};
+    const double nrm[6][3] = {{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},{0,0,-1},{0,0,1}};
+
+    for (int z = 0; z < l; ++z)
+    for (int y = 0; y < n; ++y)
+    for (int x = 0; x < m; ++x) {
+        const int64_t o = x + (int64_t)m*y + mn*z;
+        const int64_t nb[6] = {
+            x > 0     ? o - 1  : o,  x + 1 < m ? o + 1  : o,
+            y > 0     ? o - m  : o,  y + 1 < n ? o + m  : o,
+            z > 0     ? o - mn : o,  z + 1 < l ? o + mn : o};
+        const int64_t i = perm[o];
+        const double vol = hx*hy*hz*(1.0 + voljit*(2.0*u01(seed, ST_VOLJIT, (uint64_t)o) - 1.0));
+        for (int f = 0; f < 6; ++f) {
+            nei_ele[f*N + i] = perm[nb[f]];
+            fnorm_vol[f*N + i] = area[f]/vol;
+            for (int d = 0; d < 3; ++d) vec_fnorm[(f*3 + d)*N + i] = nrm[f][d];
+        }
+        if (orig_id) orig_id[i] = (int32_t)o;
+        if (key) key[i] = morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
+    }
+    free(perm);
+    return 0;
+}
+
+/* Kuhn (Freudenthal) split of a cube into 6 tets: tet s walks the cube's
+ * main diagonal adding unit steps in the axis order KUHN[s]. */
+static const int KUHN[6][3] = {{0,1,2},{0,2,1},{1,0,2},{1,2,0},{2,0,1},{2,1,0}};
+
+static int kuhn_index(int a, int b, int c)
+{
+    for (int s = 0; s < 6; ++s)
+        if (KUHN[s][0] == a && KUHN[s][1] == b && KUHN[s][2] == c) return s;
+    return -1;
+}
+
+static void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1]*b[2] - a[2]*b[1];
+    c[1] = a[2]*b[0] - a[0]*b[2];
+    c[2] = a[0]*b[1] - a[1]*b[0];
+}
+
+/*
+ * m^3 hex grid of the unit cube, each cube split into 6 Kuhn tets
+ * (N = 6 m^3, nface = 4, conforming).  Original tet id = 6*cube + s with
+ * cube = x + m*y + m*m*z.  Face k of a tet is the face opposite its vertex
+ * p_k.  Grid vertices are jittered by U[-jit*h, +jit*h] per axis, boundary
+ * vertices only tangentially, so the domain stays the unit cube.  Normals,
+ * areas and volumes come from the jittered coordinates.
+ *
+ * Outputs (permuted numbering): nei_ele[4N], fnorm_vol[4N], vec_fnorm[12N],
+ * vol[N], centroid[3N], orig_id[N], key[N], face_area[4N]
+ * (all but the first three nullable).
+ */
+GEN_EXPORT int ucfdgen_kuhn_tets(int m, uint64_t seed, int permute, double jit,
+                                 int32_t *nei_ele, double *fnorm_vol, double *vec_fnorm,
+                                 double *vol_out, double *centroid, int32_t *orig_id,
+                                 uint64_t *key, double *face_area)
+{
+    if (m <= 0) return 1;
+    const int64_t nc = (int64_t)m*m*m, N = 6*nc, mv = m + 1;
+    int32_t *perm = (int32_t *)malloc(sizeof(int32_t)*N);
+    double *vx = (double *)malloc(sizeof(double)*3*(size_t)(mv*mv*mv));
+    if (!perm || !vx) { free(perm); free(vx); return 2; }
+    ucfdgen_permutation(N, seed, permute, perm);
+
+    const double h = 1.0/m;
+    for (int64_t k = 0; k < mv; ++k)
+    for (int64_t j = 0; j < mv; ++j)
+    for (int64_t i = 0; i < mv; ++i) {
+        const int64_t v = i + mv*j + mv*mv*k;
+        const int64_t g[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) {
+            double dd = jit*h*(2.0*u01(seed, ST_VERT, (uint64_t)(3*v + d)) - 1.0);
+            if (g[d] == 0 || g[d] == m) dd = 0.0;
+            vx[3*v + d] = g[d]*h + dd;
+        }
+    }
+
+    int rc = 0;
+    #pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nc; ++c) {
+        const int x = (int)(c % m), y = (int)((c / m) % m), z = (int)(c / ((int64_t)m*m));
+        for (int s = 0; s < 6; ++s) {
+            const int a = KUHN[s][0], b = KUHN[s][1], cc = KUHN[s][2];
+            int pg[4][3];
+            pg[0][0] = x; pg[0][1] = y; pg[0][2] = z;
+            for (int d = 0; d < 3; ++d) pg[1][d] = pg[0][d] + (d == a);
+            for (int d = 0; d < 3; ++d) pg[2][d] = pg[1][d] + (d == b);
+            for (int d = 0; d < 3; ++d) pg[3][d] = pg[2][d] + (d == cc);
+            double p[4][3];
+            for (int q = 0; q < 4; ++q) {
+                const int64_t v = pg[q][0] + mv*pg[q][1] + mv*mv*pg[q][2];
+                for (int d = 0; d < 3; ++d) p[q][d] = vx[3*v + d];
+            }
+            const int64_t o = 6*c + s, i = perm[o];
+
+            double e1[3], e2[3], e3[3], cr[3];
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = p[1][d] - p[0][d]; e2[d] = p[2][d] - p[0][d]; e3[d] = p[3][d] - p[0][d];
+            }
+            cross3(e1, e2, cr);
+            const double vol = fabs(cr[0]*e3[0] + cr[1]*e3[1] + cr[2]*e3[2])/6.0;
+            if (!(vol > 0.0)) rc = 3;
+
+            /* neighbours: (cube offset axis/sign, Kuhn order) per face */
+            int ncube[4][3], nord[4];
+            for (int d = 0; d < 3; ++d) {
+                ncube[0][d] = pg[0][d] + (d == a);   /* opposite p0: cube + e_a, (b,c,a) */
+                ncube[1][d] = pg[0][d];              /* opposite p1: same cube, (b,a,c) */
+                ncube[2][d] = pg[0][d];              /* opposite p2: same cube, (a,c,b) */
+                ncube[3][d] = pg[0][d] - (d == cc);  /* opposite p3: cube - e_c, (c,a,b) */
+            }
+            nord[0] = kuhn_index(b, cc, a);
+            nord[1] = kuhn_index(b, a, cc);
+            nord[2] = kuhn_index(a, cc, b);
+            nord[3] = kuhn_index(cc, a, b);
+
+            for (int f = 0; f < 4; ++f) {
+                const int *q = ncube[f];
+                int64_t nbo = o;
+                if (q[0] >= 0 && q[0] < m && q[1] >= 0 && q[1] < m && q[2] >= 0 && q[2] < m)
+                    nbo = 6*(q[0] + (int64_t)m*q[1] + (int64_t)m*m*q[2]) + nord[f];
+                nei_ele[f*N + i] = perm[nbo];
+
+                /* face = the three vertices other than p_f; outward from p_f */
+                int t[3], nt = 0;
+                for (int r = 0; r < 4; ++r) if (r != f) t[nt++] = r;
+                double u1[3], u2[3], nn[3], w[3];
+                for (int d = 0; d < 3; ++d) {
+                    u1[d] = p[t[1]][d] - p[t[0]][d];
+                    u2[d] = p[t[2]][d] - p[t[0]][d];
+                    w[d]  = p[f][d] - p[t[0]][d];
+                }
+                cross3(u1, u2, nn);
+                double sgn = (nn[0]*w[0] + nn[1]*w[1] + nn[2]*w[2]) > 0.0 ? -1.0 : 1.0;
+                const double len = sqrt(nn[0]*nn[0] + nn[1]*nn[1] + nn[2]*nn[2]);
+                const double area = 0.5*len;
+                for (int d = 0; d < 3; ++d) vec_fnorm[(f*3 + d)*N + i] = sgn*nn[d]/len;
+                fnorm_vol[f*N + i] = area/vol;
+                if (face_area) face_area[f*N + i] = area;
+            }
+            if (vol_out) vol_out[i] = vol;
+            if (centroid)
+                for (int d = 0; d < 3; ++d)
+                    centroid[d*N + i] = 0.25*(p[0][d] + p[1][d] + p[2][d] + p[3][d]);
+            if (orig_id) orig_id[i] = (int32_t)o;
+            if (key) key[i] = morton3((uint32_t)x, (uint32_t)y, (uint32_t)z)*8 + (uint64_t)s;
+        }
+    }
+    free(perm);
+    free(vx);
+    return rc;
+}
+
+/*
+ * Flow fields for the Euler LU-SGS workloads (gamma = 1.4):
+ *   rho ~ U[0.8,1.2], u,v,w ~ U[-0.3,0.3], p ~ U[0.7,1.3]  -> conservative upts
+ *   fspr[f,i] = |u.n_f| + sqrt(gamma p / rho)   (cell's own state)
+ *   dt ~ U[1e-3, 1e-2],  rhsb ~ N(0, sigma = 1e-3)
+ * Draws are indexed by orig_id[i] (identity if orig_id is NULL).
+ * upts and rhsb have nvars rows of stride N; rows >= ndims+2 are set to 0.
+ * Any output pointer may be NULL.
+ */
+GEN_EXPORT int ucfdgen_flow(int64_t N, int nface, int ndims, int nvars, uint64_t seed,
+                            const int32_t *orig_id, const double *vec_fnorm,
+                            double *upts, double *fspr, double *dt, double *rhsb)
+{
+    const double gamma = 1.4;
+    const int nfv = ndims + 2;
+    if (ndims < 2 || ndims > 3 || nvars < nfv) return 1;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        const uint64_t o = orig_id ? (uint64_t)orig_id[i] : (uint64_t)i;
+        const double rho = 0.8 + 0.4*u01(seed, ST_RHO, o);
+        double vel[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < ndims; ++d) vel[d] = -0.3 + 0.6*u01(seed, ST_VEL, 3*o + d);
+        const double p = 0.7 + 0.6*u01(seed, ST_PRES, o);
+        const double c = sqrt(gamma*p/rho);
+        double ke = 0.0;
+        for (int d = 0; d < ndims; ++d) ke += vel[d]*vel[d];
+        if (upts) {
+            upts[i] = rho;
+            for (int d = 0; d < ndims; ++d) upts[(1 + d)*N + i] = rho*vel[d];
+            upts[(nfv - 1)*N + i] = p/(gamma - 1.0) + 0.5*rho*ke;
+            for (int k = nfv; k < nvars; ++k) upts[k*N + i] = 0.0;
+        }
+        if (fspr) {
+            for (int f = 0; f < nface; ++f) {
+                double un = 0.0;
+                for (int d = 0; d < ndims; ++d) un += vel[d]*vec_fnorm[(f*ndims + d)*N + i];
+                fspr[f*N + i] = fabs(un) + c;
+            }
+        }
+        if (dt) dt[i] = 1e-3 + 9e-3*u01(seed, ST_DT, o);
+        if (rhsb) {
+            for (int k = 0; k < nfv; ++k) rhsb[k*N + i] = 1e-3*normal01(seed, ST_RHSB, nfv*o + k);
+            for (int k = nfv; k < nvars; ++k) rhsb[k*N + i] = 0.0;
+        }
+    }
+    return 0;
+}
+
+/*
+ * Face-pair map: pair[f*N + i] = the face index f' of cell nb = nei_ele[f*N+i]
+ * with nei_ele[f'*N + nb] == i, or -1 for a boundary (self) face.  Returns
+ * the number of interior faces with no matching back face (0 on a conforming
+ * mesh).
+ */
+GEN_EXPORT int64_t ucfdgen_face_pairs(int64_t N, int nface, const int32_t *nei_ele, int32_t *pair)
+{
+    int64_t bad = 0;
+    #pragma omp parallel for schedule(static) reduction(+:bad)
+    for (int64_t i = 0; i < N; ++i) {
+        for (int f = 0; f < nface; ++f) {
+            const int64_t nb = nei_ele[f*N + i];
+            int32_t pf = -1;
+            if (nb != i) {
+                for (int g = 0; g < nface; ++g)
+                    if (nei_ele[g*N + nb] == i) { pf = g; break; }
+                if (pf < 0) ++bad;
+            }
+            pair[f*N + i] = pf;
+        }
+    }
+    return bad;
+}

@@ -1,0 +1,91 @@
+"""Host-side colouring and colour-contiguous renumbering (wrappers of the
+library's host routines ``ucfd_greedy_coloring`` / ``ucfd_color_reorder`` /
+``ucfd_permute_*``, include/ucfd_ext.h).
+
+The colour-contiguous layout is how the MI355X path wants the mesh in HBM:
+after renumbering, colour c occupies cells [off[c], off[c+1]), icolor is the
+identity, every per-cell array is read coalesced by the sweep kernels and
+the neighbour gathers stay local when the locality key (a Morton code of
+the cell position) orders the cells inside each colour.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as L
+from .mesh import Flow, Mesh
+
+
+def greedy_coloring(nei_ele: np.ndarray):
+    """First-fit greedy colouring; returns (ncolors, lcolor, icolor, color_off)."""
+    nei = np.ascontiguousarray(nei_ele, np.int32)
+    F, N = nei.shape
+    lcolor = np.empty(N, np.int32)
+    icolor = np.empty(N, np.int32)
+    off = np.empty(max(2 * F + 4, 64), np.int32)
+    nc = L.load().ucfd_greedy_coloring(N, F, nei.ctypes.data, icolor.ctypes.data,
+                                       lcolor.ctypes.data, off.ctypes.data)
+    if nc <= 0 and N > 0:
+        raise ValueError("colouring failed")
+    return nc, lcolor, icolor, off[:nc + 1].copy()
+
+
+def color_reorder(lcolor: np.ndarray, key: np.ndarray | None = None):
+    """perm[old] = new, sorted by (colour, key, id); returns (perm, color_off)."""
+    lc = np.ascontiguousarray(lcolor, np.int32)
+    N = lc.shape[0]
+    nc = int(lc.max()) if N else 0
+    perm = np.empty(N, np.int32)
+    off = np.empty(nc + 1, np.int32)
+    kp = None
+    if key is not None:
+        key = np.ascontiguousarray(key, np.uint64)
+        kp = key.ctypes.data
+    L.load().ucfd_color_reorder(N, lc.ctypes.data, kp, perm.ctypes.data, off.ctypes.data)
+    return perm, off
+
+
+def permute_cells(a: np.ndarray, perm: np.ndarray) -> np.ndarray:
+    """out[..., perm[i]] = a[..., i]"""
+    out = np.empty_like(a)
+    out[..., perm] = a
+    return out
+
+
+def permute_nei(nei_ele: np.ndarray, perm: np.ndarray) -> np.ndarray:
+    nei = np.ascontiguousarray(nei_ele, np.int32)
+    F, N = nei.shape
+    out = np.empty_like(nei)
+    p = np.ascontiguousarray(perm, np.int32)
+    L.load().ucfd_permute_nei(N, F, p.ctypes.data, nei.ctypes.data, out.ctypes.data)
+    return out
+
+
+@dataclass
+class ColoredLayout:
+    mesh: Mesh            # renumbered: colour c = cells off[c]..off[c+1]-1
+    flow: Flow | None
+    lcolor: np.ndarray    # non-decreasing
+    icolor: np.ndarray    # identity
+    off: np.ndarray       # [ncolors+1]
+    perm: np.ndarray      # perm[original] = new
+
+    @property
+    def ncolors(self) -> int:
+        return len(self.off) - 1
+
+
+def colored_layout(mesh: Mesh, flow: Flow | None = None, use_key: bool = True) -> ColoredLayout:
+    """Colour the mesh and renumber it colour-contiguously (Morton order inside
+    each colour when use_key)."""
+    nc, lcolor, icolor, off = greedy_coloring(mesh.nei_ele)
+    perm, off2 = color_reorder(lcolor, mesh.key if use_key else None)
+    assert np.array_equal(off, off2)
+    p = lambda a: None if a is None else permute_cells(a, perm)  # noqa: E731
+    m2 = Mesh(permute_nei(mesh.nei_ele, perm), p(mesh.fnorm_vol), p(mesh.vec_fnorm),
+              p(mesh.orig_id), p(mesh.key), vol=p(mesh.vol), centroid=p(mesh.centroid),
+              face_area=p(mesh.face_area))
+    f2 = None if flow is None else Flow(p(flow.upts), p(flow.fspr), p(flow.dt), p(flow.rhsb))
+    return ColoredLayout(m2, f2, p(lcolor), np.arange(mesh.neles, dtype=np.int32), off, perm)

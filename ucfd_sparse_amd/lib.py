@@ -1,0 +1,128 @@
+"""ctypes binding of the MI355X C-ABI library ``liblusgs.so``.
+
+The argument types of the legacy LU-SGS entry points are exactly the ones the
+reference's Python examples declare (``examples/omp3d.py:27-35``,
+``examples/mpi3d.py:26-34``): ``c_int`` scalars, ``c_double`` factor and
+``c_void_p`` arrays.  Arrays may be numpy arrays (host memory, staged through
+the GPU by the library) or torch CUDA tensors / raw device addresses (used in
+place).  There is no CPU fallback: loading fails loudly if the library is not
+built, and every compute call fails with a nonzero status when no GPU is
+present.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblusgs.so")
+_LIB = None
+
+c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+
+# name -> argtypes (restype c_int unless listed in _RESTYPE)
+_SIGS = {
+    # legacy array API, coloured (examples/omp3d.py:27-35, omp3d.c:180-189)
+    "parallel_pre_lusgs": [c_int, c_int, c_double] + [c_void_p] * 4,
+    "parallel_ns_lower_sweep": [c_int] * 4 + [c_void_p] * 9,
+    "parallel_ns_upper_sweep": [c_int] * 4 + [c_void_p] * 9,
+    "ns_parallel_lower_sweep": [c_int] * 4 + [c_void_p] * 10,
+    "ns_parallel_upper_sweep": [c_int] * 4 + [c_void_p] * 10,
+    "lusgs_parallel_ns_update": [c_int, c_void_p, c_void_p],
+    "parallel_ns_lusgs_update": [c_int, c_void_p, c_void_p],
+    # legacy array API, serial (examples/mpi3d.py:26-34, mpi3d.c:166-173)
+    "serial_pre_lusgs": [c_int, c_int, c_double] + [c_void_p] * 4,
+    "serial_ns_lower_sweep": [c_int, c_int] + [c_void_p] * 7,
+    "serial_ns_upper_sweep": [c_int, c_int] + [c_void_p] * 7,
+    "ns_serial_lower_sweep": [c_int, c_int] + [c_void_p] * 8,
+    "ns_serial_upper_sweep": [c_int, c_int] + [c_void_p] * 8,
+    "serial_ns_lusgs_update": [c_int, c_void_p, c_void_p],
+    "lusgs_serial_ns_update": [c_int, c_void_p, c_void_p],
+    # extensions (include/ucfd_ext.h)
+    "ucfd_set_stream": [c_void_p],
+    "ucfd_get_stream": [],
+    "ucfd_synchronize": [],
+    "ucfd_device_available": [],
+    "ucfd_release_workspace": [],
+    "ucfd_set_config": [c_int, c_int, c_int],
+    "ucfd_get_config": [c_void_p, c_void_p, c_void_p],
+    "ucfd_greedy_coloring": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "ucfd_coloring_conflicts": [c_int, c_int, c_void_p, c_void_p],
+    "ucfd_color_reorder": [c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "ucfd_permute_rows": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "ucfd_permute_nei": [c_int, c_int, c_void_p, c_void_p, c_void_p],
+}
+_RESTYPE = {"ucfd_get_stream": c_void_p, "ucfd_coloring_conflicts": ctypes.c_longlong}
+
+
+def _bind(lib, name, argtypes):
+    fn = getattr(lib, name)
+    fn.argtypes = argtypes
+    fn.restype = _RESTYPE.get(name, c_int)
+
+
+def extra_signatures():
+    """Signatures registered by other modules of the package (block LU-SGS,
+    Krylov, mixed precision, adjoint); they add to this table on import."""
+    return _SIGS
+
+
+def load():
+    """Load liblusgs.so (once) and set every argtype; raises if missing."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: the HIP library is required (run "
+                               "__graft_entry__.build() or make -C ucfd_sparse_amd/csrc)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, at in _SIGS.items():
+            if hasattr(lib, name):
+                _bind(lib, name, at)
+        _LIB = lib
+    return _LIB
+
+
+def bind(name, argtypes, restype=c_int):
+    """Register and bind one more entry point (used by sibling modules)."""
+    _SIGS[name] = argtypes
+    if restype is not c_int:
+        _RESTYPE[name] = restype
+    if _LIB is not None:
+        _bind(_LIB, name, argtypes)
+
+
+def device_count() -> int:
+    return int(load().ucfd_device_available())
+
+
+def ptr(a):
+    """Raw address of a numpy array, torch tensor or integer address."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+class UCFDError(RuntimeError):
+    pass
+
+
+def check(rc, what="ucfd call"):
+    if rc != 0:
+        raise UCFDError(f"{what} failed with status {rc}")
+    return rc
+
+
+def call(name, *args):
+    """Call an entry point with numpy / torch / int arguments; raise on error."""
+    fn = getattr(load(), name)
+    conv = []
+    for a, t in zip(args, fn.argtypes):
+        conv.append(ptr(a) if t is c_void_p else a)
+    rc = fn(*conv)
+    if fn.restype is c_int:
+        check(rc, name)
+    return rc
