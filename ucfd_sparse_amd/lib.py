@@ -68,9 +68,20 @@ def extra_signatures():
 
 
 def load():
-    """Load liblusgs.so (once) and set every argtype; raises if missing."""
+    """Load liblusgs.so (once) and set every argtype; raises if missing.
+
+    torch (when installed) is imported first: it ships its own HIP runtime
+    (torch/lib/libamdhip64.so, soname libamdhip64.so.7).  Loaded first, it
+    satisfies liblusgs.so's libamdhip64.so.7 dependency, so one process holds
+    one HIP runtime and torch tensors and library calls share a device
+    context.  Loading liblusgs.so first would pull in a second runtime from
+    /opt/rocm and torch would then find no device."""
     global _LIB
     if _LIB is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: the HIP library is required (run "
                                "__graft_entry__.build() or make -C ucfd_sparse_amd/csrc)")
