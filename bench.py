@@ -5,7 +5,8 @@ Euler (nvars = 5), FP64, on a 96^3 hex grid split into 6 Kuhn tets per hex
 (5,308,416 cells, nface = 4), cell order randomly permuted from the seed,
 normals / areas / volumes from jittered vertex coordinates, fields as in
 ucfd_sparse_amd.mesh.flow_fields.  The mesh is greedy-coloured and stored
-colour-contiguously (Morton order inside a colour).  One step = one
+colour-contiguously (Hilbert order inside a colour) and the layout is
+declared to the library (ucfd_declare_colored_layout).  One step = one
 legacy-API pass: parallel_pre_lusgs, ns_parallel_lower_sweep for every
 colour, ns_parallel_upper_sweep for every colour in reverse, and
 lusgs_parallel_ns_update - all on device-resident arrays, enqueued on one
@@ -97,6 +98,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=float, default=0.0,
+                    help="only run a plain elementwise copy kernel (out = in + 0) over this "
+                         "many MiB, 5 times: the reference reading for the PMC byte counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -108,6 +112,16 @@ def main():
     import numpy as np
     import torch
 
+    if args.calibrate_copy_mb > 0:
+        n = int(args.calibrate_copy_mb * 2**20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.add(a, 0.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel_bytes_read": 8 * n, "copy_kernel_bytes_written": 8 * n}))
+        return
+
     from ucfd_sparse_amd import lib as L
     from ucfd_sparse_amd import mesh as M
     from ucfd_sparse_amd.layout import colored_layout
@@ -118,6 +132,7 @@ def main():
     t0 = time.time()
     mesh = M.tet_mesh(args.m, seed=args.seed)
     flow = M.flow_fields(mesh, seed=args.seed)
+    mesh.key = M.hilbert_keys(mesh, "tet", (args.m,))
     lay = colored_layout(mesh, flow)
     del mesh, flow
     N, F = lay.mesh.neles, lay.mesh.nface
@@ -141,6 +156,7 @@ def main():
     off = [int(x) for x in lay.off]
     lib = L.load()
     L.call("ucfd_set_config", 5, 5, 3)
+    L.call("ucfd_declare_colored_layout", N, ic, lc, None)
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.synchronize()
     L.call("ucfd_set_stream", stream.cuda_stream)
@@ -224,7 +240,7 @@ def main():
         "config": {
             "workload": f"colored LU-SGS FP64 legacy API pass (pre + lower + upper + update), "
                         f"{args.m}^3 hex x 6 Kuhn tets, permuted, greedy coloured, "
-                        f"colour-contiguous layout",
+                        f"colour-contiguous Hilbert-ordered layout",
             "neles": N, "nface": F, "ncolors": nc, "seed": args.seed,
             "parallelism": "single GPU",
         },

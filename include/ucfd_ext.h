@@ -61,6 +61,19 @@ int ucfd_permute_rows(int neles, int nrows, int esize, const int *perm,
 /* out[f*neles + perm[i]] = perm[nei[f*neles + i]] */
 int ucfd_permute_nei(int neles, int nface, const int *perm, const int *nei, int *out);
 
+/* ---- declared colour-contiguous layouts ------------------------------------
+ * Declares that icolor (the identity) and lcolor (non-decreasing) describe a
+ * colour-contiguous mesh; the library checks both arrays once (host or device
+ * memory) and remembers the pointer pair.  Coloured sweeps called with these
+ * exact pointers and an [n0, ne) covering one whole colour then decide
+ * "lower" / "upper" from the index range (nb < n0 / nb >= ne), which equals
+ * the lcolor test on such a layout and saves the lcolor[nb] gathers.  The
+ * arrays must not change while declared: release them first.  Writes the
+ * number of colours to *ncolors (nullable). */
+int ucfd_declare_colored_layout(int neles, const int *icolor, const int *lcolor,
+                                int *ncolors);
+int ucfd_release_colored_layout(const int *icolor, const int *lcolor);
+
 #ifdef __cplusplus
 }
 #endif

@@ -133,3 +133,20 @@ def test_benchmark_mesh_colorings_fixture():
         nc, lcolor, icolor, off = color(mesh.nei_ele)
         assert np.array_equal(np.bincount(lcolor)[1:], z[name + "_counts"])
         assert hashlib.sha256(lcolor.tobytes()).hexdigest() == str(z[name + "_digest"])
+
+
+def test_declared_layout_validation_on_host_arrays():
+    lib = L.load()
+    N = 50
+    lc = np.repeat(np.arange(1, 6, dtype=np.int32), 10)
+    ic = np.arange(N, dtype=np.int32)
+    nc = np.zeros(1, np.int32)
+    assert lib.ucfd_declare_colored_layout(N, ic.ctypes.data, lc.ctypes.data, nc.ctypes.data) == 0
+    assert nc[0] == 5
+    assert lib.ucfd_release_colored_layout(ic.ctypes.data, lc.ctypes.data) == 0
+    bad_ic = ic.copy()
+    bad_ic[[3, 4]] = bad_ic[[4, 3]]
+    assert lib.ucfd_declare_colored_layout(N, bad_ic.ctypes.data, lc.ctypes.data, None) != 0
+    bad_lc = lc.copy()
+    bad_lc[7] = 3
+    assert lib.ucfd_declare_colored_layout(N, ic.ctypes.data, bad_lc.ctypes.data, None) != 0

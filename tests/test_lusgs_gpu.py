@@ -63,9 +63,11 @@ def load_case(name):
 def colored_pass_lib(mesh, flow, lcolor, icolor, to_dev=None, two_array=False):
     """pre / lower per colour / upper per colour / update through the C-ABI."""
     N, F = mesh.neles, mesh.nface
-    off = O.color_offsets(lcolor)
+    off = O.color_offsets(lcolor.cpu().numpy() if hasattr(lcolor, "cpu") else lcolor)
     cv = (lambda a: a) if to_dev is None else to_dev
-    nei, ic, lc = cv(mesh.nei_ele), cv(icolor), cv(lcolor)
+    nei = cv(mesh.nei_ele)
+    ic = icolor if hasattr(icolor, "data_ptr") else cv(icolor)
+    lc = lcolor if hasattr(lcolor, "data_ptr") else cv(lcolor)
     fnv, vfn, fsp, dt = cv(mesh.fnorm_vol), cv(mesh.vec_fnorm), cv(flow.fspr), cv(flow.dt)
     upts = cv(flow.upts.copy())
     diag = cv(np.zeros(N))
@@ -213,6 +215,32 @@ def test_color_contiguous_layout_gives_identical_results():
         assert np.array_equal(x, y[..., perm])
 
 
+def test_declared_layout_fast_path_is_bitwise_identical():
+    """ucfd_declare_colored_layout switches whole-colour calls to the
+    index-range neighbour test; results must not change by a bit."""
+    for name in ("tet5p", "cfg0_hex32"):
+        z, mesh, flow = load_case(name)
+        lcolor = z["lcolor"].astype(np.int32)
+        icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+        m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
+        cv = torch_dev()
+        lc_d, ic_d = cv(lc2), cv(ic2)
+        generic = colored_pass_lib(m2, f2, lc_d, ic_d, cv)
+        nc = np.zeros(1, np.int32)
+        L.call("ucfd_declare_colored_layout", m2.neles, ic_d, lc_d, nc)
+        try:
+            assert nc[0] == lcolor.max()
+            fast = colored_pass_lib(m2, f2, lc_d, ic_d, cv)
+            fast2 = colored_pass_lib(m2, f2, lc_d, ic_d, cv, two_array=True)
+        finally:
+            L.call("ucfd_release_colored_layout", ic_d, lc_d)
+        for x, y, w in zip(generic, fast, fast2):
+            assert np.array_equal(x, y) and np.array_equal(x, w)
+        od, os_, oq, ou = O.colored_pass(m2.nei_ele, ic2, lc2, m2.fnorm_vol, m2.vec_fnorm,
+                                         f2.fspr, f2.dt, f2.upts, f2.rhsb)
+        assert_close(fast[2], oq, f"{name} declared-layout dQ")
+
+
 def test_bad_neighbour_index_is_reported_not_faulted():
     z, mesh, flow = load_case("tet3p")
     lcolor = z["lcolor"].astype(np.int32)
@@ -349,8 +377,14 @@ def test_config1_size_fp64_matches_oracle():
                                   lcolor.ctypes.data, None)
     m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
     del mesh, flow
-    say("config 1 size: GPU pass")
-    diag, dstar, dq, upts = colored_pass_lib(m2, f2, lc2, ic2, torch_dev())
+    say("config 1 size: GPU pass (declared colour-contiguous layout)")
+    cv = torch_dev()
+    lc_d, ic_d = cv(lc2), cv(ic2)
+    L.call("ucfd_declare_colored_layout", N, ic_d, lc_d, None)
+    try:
+        diag, dstar, dq, upts = colored_pass_lib(m2, f2, lc_d, ic_d, cv)
+    finally:
+        L.call("ucfd_release_colored_layout", ic_d, lc_d)
     torch.cuda.empty_cache()
     say("config 1 size: oracle pass")
     od, os_, oq, ou = O.colored_pass(m2.nei_ele, ic2, lc2, m2.fnorm_vol, m2.vec_fnorm, f2.fspr,

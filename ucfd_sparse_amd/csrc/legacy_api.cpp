@@ -15,14 +15,41 @@
 #include "ucfd_internal.h"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "coloredlusgs.h"
 #include "lusgs.h"
+#include "ucfd_ext.h"
 
 namespace {
 
 bool bad_dims(int neles, int nface) { return neles < 0 || nface < 1 || nface > 32; }
+
+// Colour-contiguous layouts declared by the caller (ucfd_declare_colored_layout):
+// icolor is the identity and lcolor is non-decreasing, so colour c occupies
+// the index range [off[c], off[c+1]).  A coloured call on such a layout whose
+// [n0, ne) is one whole colour takes the neighbour test "nb < n0" / "nb >= ne"
+// instead of reading lcolor[nb]; that is the same test, without the gather.
+struct DeclaredLayout {
+    const int *icolor;
+    const int *lcolor;
+    int neles;
+    std::vector<int> off;
+};
+std::mutex g_layout_mu;
+std::vector<DeclaredLayout> g_layouts;
+
+bool whole_colour_of_declared(const int *icolor, const int *lcolor, int neles, int n0, int ne)
+{
+    std::lock_guard<std::mutex> g(g_layout_mu);
+    for (const DeclaredLayout &l : g_layouts) {
+        if (l.icolor != icolor || l.lcolor != lcolor || l.neles != neles) continue;
+        auto it = std::lower_bound(l.off.begin(), l.off.end(), n0);
+        return it != l.off.end() && *it == n0 && it + 1 != l.off.end() && *(it + 1) == ne;
+    }
+    return false;
+}
 
 int do_pre(const char *fn, int neles, int nface, double factor, const double *fnorm_vol,
            const double *dt, double *diag, const double *fspr)
@@ -67,9 +94,11 @@ int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, co
     const int a_out = sg.add(out, V*N*8, UCFD_INOUT);
     const int a_dg = sg.add(diag, N*8, UCFD_IN);
     const int a_sp = sg.add(fspr, F*N*8, UCFD_IN);
+    const int mode = whole_colour_of_declared(icolor, lcolor, neles, n0, ne) ? UCFD_SWEEP_RANGE
+                                                                             : UCFD_SWEEP_COLORED;
     int rc = sg.enter();
     if (rc) return rc;
-    rc = ucfd_launch_sweep(ucfd_stream(), dir, UCFD_SWEEP_COLORED, n0, ne, neles, nface, c.ndims,
+    rc = ucfd_launch_sweep(ucfd_stream(), dir, mode, n0, ne, neles, nface, c.ndims,
                            c.nfvars, (const int *)sg.dev(a_nei), (const int *)sg.dev(a_ic),
                            (const int *)sg.dev(a_lc), (const double *)sg.dev(a_fnv),
                            (const double *)sg.dev(a_vfn), (const double *)sg.dev(a_u),
@@ -290,4 +319,55 @@ UCFD_API int serial_ns_lusgs_update(int neles, double *upts, double *dub)
 UCFD_API int lusgs_serial_ns_update(int neles, double *upts, double *rhsb)
 {
     return do_update("lusgs_serial_ns_update", neles, upts, rhsb);
+}
+
+// ------------------------------------------------------- declared layouts
+
+UCFD_API int ucfd_declare_colored_layout(int neles, const int *icolor, const int *lcolor,
+                                         int *ncolors_out)
+{
+    const char *fn = "ucfd_declare_colored_layout";
+    if (neles <= 0 || !icolor || !lcolor) return ucfd_report(UCFD_ERR_ARG, fn, "bad arguments");
+    std::vector<int> ic(neles), lc(neles);
+    const size_t bytes = (size_t)neles*4;
+    const bool dev_i = ucfd_is_device_ptr(icolor), dev_l = ucfd_is_device_ptr(lcolor);
+    if (dev_i || dev_l) {
+        int rc = ucfd_ensure_device(fn);
+        if (rc) return rc;
+        if (dev_i && (rc = ucfd_check_hip(hipMemcpy(ic.data(), icolor, bytes, hipMemcpyDeviceToHost),
+                                          fn, "hipMemcpy(icolor)")))
+            return rc;
+        if (dev_l && (rc = ucfd_check_hip(hipMemcpy(lc.data(), lcolor, bytes, hipMemcpyDeviceToHost),
+                                          fn, "hipMemcpy(lcolor)")))
+            return rc;
+    }
+    if (!dev_i) std::copy(icolor, icolor + neles, ic.begin());
+    if (!dev_l) std::copy(lcolor, lcolor + neles, lc.begin());
+    DeclaredLayout l{icolor, lcolor, neles, {0}};
+    for (int p = 0; p < neles; ++p) {
+        if (ic[p] != p)
+            return ucfd_report(UCFD_ERR_ARG, fn, "icolor is not the identity permutation");
+        if (p > 0 && lc[p] < lc[p - 1])
+            return ucfd_report(UCFD_ERR_ARG, fn, "lcolor is not non-decreasing");
+        if (p > 0 && lc[p] != lc[p - 1]) l.off.push_back(p);
+    }
+    l.off.push_back(neles);
+    std::lock_guard<std::mutex> g(g_layout_mu);
+    for (DeclaredLayout &o : g_layouts)
+        if (o.icolor == icolor && o.lcolor == lcolor) { o = l; goto done; }
+    g_layouts.push_back(l);
+done:
+    if (ncolors_out) *ncolors_out = (int)l.off.size() - 1;
+    return UCFD_OK;
+}
+
+UCFD_API int ucfd_release_colored_layout(const int *icolor, const int *lcolor)
+{
+    std::lock_guard<std::mutex> g(g_layout_mu);
+    for (size_t k = 0; k < g_layouts.size(); ++k)
+        if (g_layouts[k].icolor == icolor && g_layouts[k].lcolor == lcolor) {
+            g_layouts.erase(g_layouts.begin() + k);
+            return UCFD_OK;
+        }
+    return UCFD_OK;
 }

@@ -41,10 +41,41 @@ __global__ __launch_bounds__(kBlock) void k_pre(int N, int nface, double factor,
     diag[i] = d + sd;
 }
 
+// Workgroups are dealt round-robin over the 8 XCDs of the MI355X, each with
+// its own L2.  Remapping the hardware workgroup id so that each XCD owns one
+// contiguous eighth of the launched cell range keeps the neighbour rows that
+// adjacent cells share inside one L2 (spatially sorted cells share most of
+// their neighbours' cache lines).
+__device__ __forceinline__ int xcd_block()
+{
+    const int b = blockIdx.x, nblk = gridDim.x;
+    const int x = b % 8, k = b / 8, per = nblk / 8, rem = nblk % 8;
+    return x*per + (x < rem ? x : rem) + k;
+}
+
+template <typename T> __device__ __forceinline__ T ld_stream(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
 // DIR 0: lower, out_i = (in_i - 0.5 df)/diag_i
 // DIR 1: upper, out_i = in_i - (0.5 df)/diag_i
-// MODE 0 (colored): neighbour taken iff lcolor[nb] </> lcolor[i]
-// MODE 1 (ordered): neighbour taken iff nb </> i
+// MODE 0 (colored):  cell i = order[n0 + t]; neighbour taken iff
+//                    lcolor[nb] </> lcolor[i]
+// MODE 1 (ordered):  cell i = order[n0 + t]; neighbour taken iff nb </> i
+// MODE 2 (range):    colour-contiguous layout, cell i = n0 + t; neighbour
+//                    taken iff nb < n0 (lower) / nb >= ne (upper), which is
+//                    the colour test when [n0, ne) is one whole colour of a
+//                    layout with non-decreasing lcolor
+//
+// Memory schedule: all neighbour indices of the cell are loaded first; then,
+// G faces at a time, every load a taken face needs (the 2*NV gathered
+// neighbour values and the face's own geometry) is issued under that face's
+// predicate with no wait in between, and only then is the flux arithmetic
+// done, face by face in index order (the reference's accumulation order).
+// Rows streamed once per launch (indices, geometry, diag, the cell's own
+// input and output) use non-temporal accesses so they do not evict the
+// gathered neighbour lines from L2.
 template <int DIR, int MODE, int ND, int FC>
 __global__ __launch_bounds__(kBlock) void k_sweep(
     int n0, int cnt, int N, int nface, const int *__restrict__ nei,
@@ -54,23 +85,101 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
     const double *__restrict__ diag, const double *__restrict__ fspr, int *flag)
 {
     constexpr int NV = ND + 2;
-    const int t = blockIdx.x*kBlock + threadIdx.x;
+    constexpr int G = 2;
+    static_assert(FC % G == 0, "face count must be a multiple of the group size");
+    const int t = xcd_block()*kBlock + threadIdx.x;
     if (t >= cnt) return;
-    const int i = order[n0 + t];
+    const int i = MODE == 2 ? n0 + t : ld_stream(order + n0 + t);
     if ((unsigned)i >= (unsigned)N) {
         raise_flag(flag, UCFD_KFLAG_INDEX);
         return;
     }
-    const int F = FC ? FC : nface;
     int ci = 0;
     if (MODE == 0) ci = lcolor[i];
+
+    int nbv[FC];
+    bool tk[FC];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) nbv[f] = ld_stream(nei + (size_t)f*N + i);
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+        const int nb = nbv[f];
+        if ((unsigned)nb >= (unsigned)N) {
+            raise_flag(flag, UCFD_KFLAG_INDEX);
+            tk[f] = false;
+        } else if (MODE == 0) {
+            const int cn = lcolor[nb];
+            tk[f] = DIR == 0 ? (cn < ci) : (cn > ci);
+        } else if (MODE == 1) {
+            tk[f] = DIR == 0 ? (nb < i) : (nb > i);
+        } else {
+            tk[f] = DIR == 0 ? (nb < n0) : (nb >= n0 + cnt);
+        }
+    }
 
     double df[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) df[k] = 0.0;
-
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
+    for (int f0 = 0; f0 < FC; f0 += G) {
+        double u[G][NV], du[G][NV], nf[G][ND], fv[G], lam[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int f = f0 + g;
+            if (!tk[f]) continue;
+            const int nb = nbv[f];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                u[g][k] = upts[(size_t)k*N + nb];
+                du[g][k] = out[(size_t)k*N + nb];
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) nf[g][d] = ld_stream(vfn + (size_t)(f*ND + d)*N + i);
+            fv[g] = ld_stream(fnv + (size_t)f*N + i);
+            lam[g] = ld_stream(fspr + (size_t)f*N + i);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (!tk[f0 + g]) continue;
+            double dfj[NV];
+            ucfd::diff_flux<double, ND>(u[g], du[g], nf[g], dfj);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) df[k] += (dfj[k] - lam[g]*du[g][k])*fv[g];
+        }
+    }
+    const double dg = ld_stream(diag + i);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const size_t o = (size_t)k*N + i;
+        const double x = ld_stream(in + o);
+        if (DIR == 0) __builtin_nontemporal_store((x - 0.5*df[k])/dg, out + o);
+        else          __builtin_nontemporal_store(x - (0.5*df[k]/dg), out + o);
+    }
+}
+
+// Any other face count: one face at a time.
+template <int DIR, int MODE, int ND>
+__global__ __launch_bounds__(kBlock) void k_sweep_any(
+    int n0, int cnt, int N, int nface, const int *__restrict__ nei,
+    const int *__restrict__ order, const int *__restrict__ lcolor,
+    const double *__restrict__ fnv, const double *__restrict__ vfn,
+    const double *__restrict__ upts, const double *in, double *out,
+    const double *__restrict__ diag, const double *__restrict__ fspr, int *flag)
+{
+    constexpr int NV = ND + 2;
+    const int t = xcd_block()*kBlock + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = MODE == 2 ? n0 + t : order[n0 + t];
+    if ((unsigned)i >= (unsigned)N) {
+        raise_flag(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    int ci = 0;
+    if (MODE == 0) ci = lcolor[i];
+    double df[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) df[k] = 0.0;
+    for (int f = 0; f < nface; ++f) {
         const int nb = nei[(size_t)f*N + i];
         if ((unsigned)nb >= (unsigned)N) {
             raise_flag(flag, UCFD_KFLAG_INDEX);
@@ -80,8 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
         if (MODE == 0) {
             const int cn = lcolor[nb];
             take = DIR == 0 ? (cn < ci) : (cn > ci);
-        } else {
+        } else if (MODE == 1) {
             take = DIR == 0 ? (nb < i) : (nb > i);
+        } else {
+            take = DIR == 0 ? (nb < n0) : (nb >= n0 + cnt);
         }
         if (!take) continue;
         double u[NV], du[NV], nf[ND], dfj[NV];
@@ -132,8 +243,8 @@ int launch_sweep_f(hipStream_t s, int n0, int cnt, int N, int nface, const int *
                            order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
         break;
     default:
-        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 0>), grid, block, 0, s, n0, cnt, N, nface, nei,
-                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+        hipLaunchKernelGGL((k_sweep_any<DIR, MODE, ND>), grid, block, 0, s, n0, cnt, N, nface,
+                           nei, order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
         break;
     }
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
@@ -177,6 +288,12 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
                                     vec_fnorm, upts, in, out, diag, fspr, flag);
     if (dir == 1 && mode == UCFD_SWEEP_COLORED)
         return launch_sweep_d<1, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    if (dir == 0 && mode == UCFD_SWEEP_RANGE)
+        return launch_sweep_d<0, 2>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    if (dir == 1 && mode == UCFD_SWEEP_RANGE)
+        return launch_sweep_d<1, 2>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
                                     vec_fnorm, upts, in, out, diag, fspr, flag);
     if (dir == 0)
         return launch_sweep_d<0, 1>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,

@@ -144,6 +144,61 @@ GEN_EXPORT int ucfdgen_hex(int m, int n, int l, uint64_t seed, int permute, doub
     return 0;
 }
 
+/*
+ * 3-D Hilbert index of (x, y, z) with `bits` bits per axis, by Skilling's
+ * transpose algorithm (J. Skilling, "Programming the Hilbert curve", AIP
+ * Conf. Proc. 707, 2004): undo excess work, Gray-encode, then interleave
+ * the transposed bits from the most significant bit down.
+ */
+static uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits)
+{
+    uint32_t X[3] = {x, y, z};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) X[0] ^= P;
+            else { const uint32_t t = (X[0] ^ X[i]) & P; X[0] ^= t; X[i] ^= t; }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint64_t h = 0;
+    for (int q = bits - 1; q >= 0; --q)
+        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> q) & 1u);
+    return h;
+}
+
+static int bits_for(int m)
+{
+    int b = 1;
+    while ((1 << b) < m) ++b;
+    return b;
+}
+
+/*
+ * Hilbert locality keys from the original cell ids: hex cells (kind 0,
+ * id = x + m*y + m*n*z) or Kuhn tets (kind 1, id = 6*cube + s, cube on an
+ * m^3 grid; key = 8*hilbert(cube) + s).
+ */
+GEN_EXPORT void ucfdgen_hilbert_keys(int kind, int m, int n, int l, int64_t N,
+                                     const int32_t *orig_id, uint64_t *key)
+{
+    const int bits = bits_for(m > n ? (m > l ? m : l) : (n > l ? n : l));
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        int64_t o = orig_id[i], s = 0;
+        if (kind == 1) { s = o % 6; o /= 6; }
+        const uint32_t x = (uint32_t)(o % m), y = (uint32_t)((o / m) % n);
+        const uint32_t z = (uint32_t)(o / ((int64_t)m*n));
+        const uint64_t h = hilbert3(x, y, z, bits);
+        key[i] = kind == 1 ? 8*h + (uint64_t)s : h;
+    }
+}
+
 /* Kuhn (Freudenthal) split of a cube into 6 tets: tet s walks the cube's
  * main diagonal adding unit steps in the axis order KUHN[s]. */
 static const int KUHN[6][3] = {{0,1,2},{0,2,1},{1,0,2},{1,2,0},{2,0,1},{2,1,0}};

@@ -63,7 +63,7 @@ struct UcfdStage {
 };
 
 // ---- kernel launchers (lusgs_kernels.hip)
-enum { UCFD_SWEEP_COLORED = 0, UCFD_SWEEP_ORDERED = 1 };
+enum { UCFD_SWEEP_COLORED = 0, UCFD_SWEEP_ORDERED = 1, UCFD_SWEEP_RANGE = 2 };
 
 UCFD_HIDDEN int ucfd_launch_pre(hipStream_t s, int neles, int nface, double factor,
                                 const double *fnorm_vol, const double *dt, double *diag,
@@ -72,6 +72,8 @@ UCFD_HIDDEN int ucfd_launch_pre(hipStream_t s, int neles, int nface, double fact
 // dir 0 = lower, 1 = upper.  mode COLORED: cells order[n0:ne), neighbour
 // taken iff lcolor[nb] </> lcolor[i].  mode ORDERED: cells order[n0:ne),
 // neighbour taken iff nb </> i (sequential semantics, level scheduled).
+// mode RANGE: cells n0..ne-1 of a declared colour-contiguous layout,
+// neighbour taken iff nb < n0 / nb >= ne (order and lcolor unused).
 UCFD_HIDDEN int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int neles,
                                   int nface, int ndims, int nfvars, const int *nei,
                                   const int *order, const int *lcolor, const double *fnorm_vol,

@@ -51,6 +51,8 @@ _SIGS = {
     "ucfd_color_reorder": [c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_rows": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_nei": [c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "ucfd_declare_colored_layout": [c_int, c_void_p, c_void_p, c_void_p],
+    "ucfd_release_colored_layout": [c_void_p, c_void_p],
 }
 _RESTYPE = {"ucfd_get_stream": c_void_p, "ucfd_coloring_conflicts": ctypes.c_longlong}
 

@@ -46,6 +46,8 @@ def _lib():
         lib.ucfdgen_permutation.restype = None
         lib.ucfdgen_face_pairs.argtypes = [ctypes.c_int64, ctypes.c_int, vp, vp]
         lib.ucfdgen_face_pairs.restype = ctypes.c_int64
+        lib.ucfdgen_hilbert_keys.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, vp, vp]
+        lib.ucfdgen_hilbert_keys.restype = None
         lib.ucfdgen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         lib.ucfdgen_uniform.restype = ctypes.c_double
         lib.ucfdgen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
@@ -144,6 +146,17 @@ def flow_fields(mesh: Mesh, seed: int = 0, nvars: int | None = None) -> Flow:
     if rc:
         raise ValueError(f"ucfdgen_flow failed ({rc})")
     return Flow(upts, fspr, dt, rhsb)
+
+
+def hilbert_keys(mesh: Mesh, kind: str, dims) -> np.ndarray:
+    """Hilbert-curve locality key of every cell (from its original id):
+    kind "hex" with dims (m, n, l), or "tet" with dims (m,) for Kuhn tets
+    (key = 8*hilbert(cube) + tet index)."""
+    key = np.empty(mesh.neles, np.uint64)
+    d = tuple(dims) + (dims[0],) * (3 - len(dims))
+    _lib().ucfdgen_hilbert_keys(0 if kind == "hex" else 1, d[0], d[1], d[2], mesh.neles,
+                                _p(np.ascontiguousarray(mesh.orig_id, np.int32)), _p(key))
+    return key
 
 
 def face_pairs(nei_ele: np.ndarray) -> np.ndarray:
