@@ -89,6 +89,51 @@ def cpu_baseline(lay, seconds, nvars=5):
                       f"{threads} OpenMP threads, {t:.1f} s"}
 
 
+def precision_forms(lay, T, rtol, repeats, log):
+    """Time-to-rtol of the FP64 and mixed FP32/FP64 defect-correction forms on
+    the same device-resident system; error of the mixed result vs FP64."""
+    import torch
+
+    from ucfd_sparse_amd.solver import LUSGSSolver
+    m = lay.mesh
+    s = LUSGSSolver(m.neles, m.nface, lay.off, T["nei"], T["fnv"], T["vfn"], T["fsp"], T["dt"],
+                    T["upts"])
+    b = T["rhs"]
+    out = {}
+    res = {}
+    for mode in ("fp64", "mixed"):
+        dq = torch.zeros_like(b)
+        st = s.solve(b, dq, mode, rtol, 400)          # warm-up
+        times = []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = s.solve(b, dq, mode, rtol, 400)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        times.sort()
+        res[mode] = dq
+        out[mode] = {"ms": times[len(times) // 2], "passes": st.passes, "outer": st.outer,
+                     "rel_residual": st.rel_residual, "converged": bool(st.converged)}
+        log(f"{mode}: {out[mode]}")
+    s.close()
+    a, bm = res["fp64"], res["mixed"]
+    err = float((bm - a).abs().max() / a.abs().max())
+    N = m.neles
+    k64 = out["fp64"]["passes"]
+    return {
+        "rtol": rtol,
+        "fp64": out["fp64"],
+        "mixed": out["mixed"],
+        "speedup_mixed_vs_fp64": out["fp64"]["ms"] / out["mixed"]["ms"],
+        "fp64_equivalent_cell_iters_per_s": {"fp64": N * k64 / (out["fp64"]["ms"] * 1e-3),
+                                             "mixed": N * k64 / (out["mixed"]["ms"] * 1e-3)},
+        "mixed_max_rel_err_vs_fp64": err,
+        "note": "defect correction A(dq) = b with one coloured LU-SGS pass as preconditioner; "
+                "mixed = FP32 sweeps inside an FP64 outer residual loop, dq accumulated in FP64",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +143,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
+    ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--calibrate-copy-mb", type=float, default=0.0,
                     help="only run a plain elementwise copy kernel (out = in + 0) over this "
                          "many MiB, 5 times: the reference reading for the PMC byte counters")
@@ -219,6 +266,12 @@ def main():
     if tr and "hbm_bytes_per_launch" in tr:
         traffic = tr["hbm_bytes_per_launch"]
 
+    mixed = None
+    if not args.no_mixed:
+        log("precision forms (FP64 vs mixed) ...")
+        T = dict(nei=nei, fnv=fnv, vfn=vfn, fsp=fsp, dt=dt, upts=upts, rhs=rhs)
+        mixed = precision_forms(lay, T, args.rtol, 3, log)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -259,6 +312,7 @@ def main():
             "avg_launch_ms": lower_launch_ms,
         },
         "cpu_baseline": cpu,
+        "precision_forms": mixed,
     }
     print(json.dumps(rec), flush=True)
 

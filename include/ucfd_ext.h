@@ -74,6 +74,43 @@ int ucfd_declare_colored_layout(int neles, const int *icolor, const int *lcolor,
                                 int *ncolors);
 int ucfd_release_colored_layout(const int *icolor, const int *lcolor);
 
+/* ---- defect-correction solver (FP64 and mixed FP32/FP64 forms) -------------
+ * Solves A(dq) = b, the LU-SGS system of the legacy API,
+ *   A(dq)_i = D_i dq_i + 1/2 sum_{f, nb != i} fnorm_vol_f [F(Q_nb + dq_nb; n_f)
+ *             - F(Q_nb; n_f) - fspr_f dq_nb],  D_i = 1/dt_i + 1/2 sum_f fspr_f fnorm_vol_f,
+ * by defect correction with one coloured LU-SGS pass as the preconditioner.
+ * The mesh must be colour-contiguous: colour c = cells color_off[c] ..
+ * color_off[c+1]-1 (see ucfd_color_reorder).  Inputs may be host or device
+ * arrays in the legacy layouts (upts: the first 5 rows are used); the solver
+ * keeps device copies (FP64 and packed FP32 face data) until destroyed.
+ * UCFD_SOLVE_MIXED runs the sweeps in FP32 inside an FP64 outer residual
+ * loop (iterative refinement); dq is accumulated in FP64. */
+typedef struct ucfd_lusgs_solver ucfd_lusgs_solver;
+#define UCFD_SOLVE_FP64 0
+#define UCFD_SOLVE_MIXED 1
+#define UCFD_SOLVE_HIST 128
+typedef struct {
+    int converged;          /* 1 if ||b - A(dq)|| <= rtol ||b|| */
+    int outer;              /* FP64 residual evaluations (outer iterations) */
+    int passes;             /* LU-SGS passes (lower + upper sweep) */
+    int nhist;              /* entries of history[] */
+    double rnorm0;          /* ||b|| */
+    double rel_residual;    /* ||b - A(dq)|| / ||b|| of the returned dq */
+    double history[UCFD_SOLVE_HIST];   /* relative residuals as computed */
+} ucfd_solve_stats;
+int ucfd_lusgs_solver_create(ucfd_lusgs_solver **solver, int neles, int nface,
+                             int ncolors, const int *color_off, const int *nei_ele,
+                             const double *fnorm_vol, const double *vec_fnorm,
+                             const double *fspr, const double *dt, const double *upts);
+int ucfd_lusgs_solver_destroy(ucfd_lusgs_solver *solver);
+/* r = b - A(dq) in FP64 (r nullable); *rnorm = ||r||_2 (nullable) */
+int ucfd_lusgs_residual(ucfd_lusgs_solver *solver, const double *b, const double *dq,
+                        double *r, double *rnorm);
+/* dq (output, [5][neles]) with ||b - A(dq)|| <= rtol ||b||; maxiter bounds the
+ * number of LU-SGS passes; stats nullable */
+int ucfd_lusgs_solve(ucfd_lusgs_solver *solver, int mode, const double *b, double *dq,
+                     double rtol, int maxiter, ucfd_solve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,10 @@ def lib(fast: bool = False):
         for fn in ("orc_pre", "orc_lower", "orc_upper", "orc_update", "orc_flux",
                    "orc_colored_iteration"):
             getattr(L, fn).restype = None
+        L.orc_residual.argtypes = [ci, ci, ci] + [vp] * 9
+        L.orc_residual.restype = cd
+        L.orc_dc_solve.argtypes = [ci, ci, ci, ci] + [vp] * 11 + [cd, ci, vp]
+        L.orc_dc_solve.restype = ci
         _LIBS[name] = L
     return _LIBS[name]
 
@@ -130,3 +134,34 @@ def serial_pass(nei, fnorm_vol, vec_fnorm, fspr, dt, upts, rhsb):
     full[:D + 2] = dub
     update(u, full)
     return diag, dstar, dub, u
+
+
+def residual(nei, fnorm_vol, vec_fnorm, fspr, dt, q, b, dq):
+    """r = b - A(dq) (oracle/residual_oracle.c); returns (r, ||r||)."""
+    F, N = nei.shape
+    D = vec_fnorm.shape[1]
+    diag = pre(fnorm_vol, dt, fspr)
+    r = np.empty((D + 2, N))
+    c = lambda a: _c(a, np.float64)  # noqa: E731
+    nrm = lib().orc_residual(N, F, D, _c(nei, np.int32).ctypes.data, c(fnorm_vol).ctypes.data,
+                             c(vec_fnorm).ctypes.data, c(fspr).ctypes.data, diag.ctypes.data,
+                             c(q).ctypes.data, c(b).ctypes.data, c(dq).ctypes.data, r.ctypes.data)
+    return r, nrm
+
+
+def dc_solve(nei, icolor, lcolor, fnorm_vol, vec_fnorm, fspr, dt, q, b, rtol, maxpass):
+    """FP64 defect correction with coloured LU-SGS passes; returns
+    (dq, passes, history)."""
+    F, N = nei.shape
+    D = vec_fnorm.shape[1]
+    off = color_offsets(lcolor)
+    dq = np.empty((D + 2, N))
+    hist = np.zeros(maxpass + 1)
+    c = lambda a: _c(a, np.float64)  # noqa: E731
+    i32 = lambda a: _c(a, np.int32)  # noqa: E731
+    n = lib().orc_dc_solve(N, F, D, len(off) - 1, off.ctypes.data, i32(nei).ctypes.data,
+                           i32(icolor).ctypes.data, i32(lcolor).ctypes.data,
+                           c(fnorm_vol).ctypes.data, c(vec_fnorm).ctypes.data, c(fspr).ctypes.data,
+                           c(dt).ctypes.data, c(q[:D + 2]).ctypes.data, c(b[:D + 2]).ctypes.data,
+                           dq.ctypes.data, rtol, maxpass, hist.ctypes.data)
+    return dq, n, hist[:n + 1]

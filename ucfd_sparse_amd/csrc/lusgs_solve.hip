@@ -1,0 +1,637 @@
+// Defect-correction LU-SGS solver on a colour-contiguous mesh: the FP64 form
+// and the mixed FP32/FP64 form (extension; the reference has no solver
+// around its sweeps - the operator is the one LU-SGS factors approximately,
+// doc/3.1_lusgs.md:79-100).
+//
+// Operator.  For an increment e on the base state Q:
+//   A(e)_i = D_i e_i + 1/2 sum_{f, nb != i} [ dF_S(Q_nb; e_nb) - L_f e_nb ]
+// with D_i = 1/dt_i + 1/2 sum_f fspr_f fnorm_vol_f (the pre diagonal),
+// S_f = fnorm_vol_f * vec_fnorm_f and L_f = fnorm_vol_f * fspr_f.  The Euler
+// flux is linear in the normal, so fnorm_vol*[F(U+d; n) - F(U; n)] =
+// F(U+d; S) - F(U; S) =: dF_S(U; d).  dF_S is evaluated in the exact
+// difference form below (no subtraction of two O(1) fluxes), which keeps
+// ~7 significant digits of dF in FP32 when |d| << |U|; where the pressure
+// floor PMIN is active on either state the two fluxes are evaluated and
+// subtracted as in src/lusgs/flux.c:29-67.
+//
+// Both forms solve A(dq) = b by defect correction with one LU-SGS pass as
+// the preconditioner:
+//   e <- e + M^{-1}(b - A(e)),  M^{-1} = upper sweep o lower sweep at base Q.
+// The residual of iteration j is computed inside the lower-sweep launch of
+// each colour and e += delta inside the upper-sweep launch.
+// Mixed form (iterative refinement): the outer loop evaluates r = b - A(dq)
+// in FP64, rounds r and the base state Q + dq to FP32, runs the same inner
+// defect correction in FP32 on the correction equation A_dq(e) = r until its
+// residual has dropped by the inner tolerance, and adds e to dq in FP64.
+#include <algorithm>
+#include <mutex>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "flux.h"
+#include "ucfd_ext.h"
+#include "ucfd_internal.h"
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int NV = 5;
+
+template <typename T> struct alignas(4*sizeof(T)) Face4 {
+    T s0, s1, s2, l;    // S = fnorm_vol * n, L = fnorm_vol * fspr
+};
+
+template <typename T> __device__ __forceinline__ T ldg_nt(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+// face records are read as 16-byte vectors
+__device__ __forceinline__ Face4<float> ldg_nt(const Face4<float> *p)
+{
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load((const v4 *)p);
+    return Face4<float>{v.x, v.y, v.z, v.w};
+}
+
+__device__ __forceinline__ Face4<double> ldg_nt(const Face4<double> *p)
+{
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 a = __builtin_nontemporal_load((const v2 *)p);
+    const v2 b = __builtin_nontemporal_load((const v2 *)p + 1);
+    return Face4<double>{a.x, a.y, b.x, b.y};
+}
+
+__device__ __forceinline__ int xcd_remap()
+{
+    const int b = blockIdx.x, nblk = gridDim.x;
+    const int x = b % 8, k = b / 8, per = nblk / 8, rem = nblk % 8;
+    return x*per + (x < rem ? x : rem) + k;
+}
+
+// out = dF_S(U; d) - L d   (5 components, 3-D Euler)
+template <typename T>
+__device__ __forceinline__ void face_term(const T *U, const T *d, const Face4<T> &g, T *out)
+{
+    using C = ucfd::Const<T>;
+    const T gm1 = C::gamma - T(1);
+    const T rho = U[0], rhop = U[0] + d[0];
+    const T mS = U[1]*g.s0 + U[2]*g.s1 + U[3]*g.s2;
+    const T dmS = d[1]*g.s0 + d[2]*g.s1 + d[3]*g.s2;
+    const T th = mS/rho;
+    const T dth = (dmS - th*d[0])/rhop;
+    const T thp = th + dth;
+    const T mm = U[1]*U[1] + U[2]*U[2] + U[3]*U[3];
+    const T mdm = U[1]*d[1] + U[2]*d[2] + U[3]*d[3];
+    const T dd = d[1]*d[1] + d[2]*d[2] + d[3]*d[3];
+    const T k = mm/rho;
+    const T p = gm1*(U[4] - T(0.5)*k);
+    const T dp = gm1*(d[4] - T(0.5)*(T(2)*mdm + dd - k*d[0])/rhop);
+    if (p < C::pmin || p + dp < C::pmin) {
+        // floor active: subtract the two clamped fluxes (flux.c:50-56)
+        const T nf[3] = {g.s0, g.s1, g.s2};
+        T up[NV], f0[NV], f1[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) up[q] = U[q] + d[q];
+        ucfd::ns_flux<T, 3>(U, nf, f0);
+        ucfd::ns_flux<T, 3>(up, nf, f1);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = (f1[q] - f0[q]) - g.l*d[q];
+        return;
+    }
+    out[0] = dmS - g.l*d[0];
+    out[1] = d[1]*thp + U[1]*dth + g.s0*dp - g.l*d[1];
+    out[2] = d[2]*thp + U[2]*dth + g.s1*dp - g.l*d[2];
+    out[3] = d[3]*thp + U[3]*dth + g.s2*dp - g.l*d[3];
+    out[4] = (d[4] + dp)*thp + (U[4] + p)*dth - g.l*d[4];
+}
+
+// block sum of v (FP64) written to part[blockIdx]; deterministic order
+__device__ __forceinline__ void block_sum(double v, double *part, int slot)
+{
+    __shared__ double red[kB/64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x/64, l = threadIdx.x%64;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < kB/64; ++q) s += red[q];
+        part[slot] = s;
+    }
+}
+
+// ---- lower sweep of colour [n0, n0+cnt), with the defect-correction residual
+//   s_i = rhs_i - A(e)_i        (WITH_RES; otherwise s_i = rhs_i, e = 0)
+//   dstar_i = (s_i - 1/2 sum_{nb < n0} term(Qb_nb; dstar_nb)) / D_i
+// Writes sum_i |s_i|^2 of this launch's blocks to part[pofs + block].
+template <typename T, int FC, bool WITH_RES>
+__global__ __launch_bounds__(kB) void k_lower_res(
+    int n0, int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
+    const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ rhs,
+    const T *__restrict__ e, T *dstar, double *part, int pofs)
+{
+    const int b = xcd_remap();
+    const int t = b*kB + threadIdx.x;
+    double ss = 0.0;
+    if (t < cnt) {
+        const int i = n0 + t;
+        int nbv[FC];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
+        const T dg = ldg_nt(diag + i);
+        T s[NV], acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) { s[q] = ldg_nt(rhs + (size_t)q*N + i); acc[q] = T(0); }
+        if (WITH_RES) {
+            T res[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) res[q] = T(0);
+#pragma unroll
+            for (int f = 0; f < FC; ++f) {
+                const int nb = nbv[f];
+                if (nb == i) continue;
+                const Face4<T> g = ldg_nt(geo + (size_t)f*N + i);
+                T U[NV], d[NV], o[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q) { U[q] = qb[(size_t)q*N + nb]; d[q] = e[(size_t)q*N + nb]; }
+                face_term<T>(U, d, g, o);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) res[q] += o[q];
+            }
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                s[q] = s[q] - (dg*ldg_nt(e + (size_t)q*N + i) + T(0.5)*res[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) ss += (double)s[q]*(double)s[q];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const int nb = nbv[f];
+            if (nb >= n0) continue;
+            const Face4<T> g = ldg_nt(geo + (size_t)f*N + i);
+            T U[NV], d[NV], o[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) { U[q] = qb[(size_t)q*N + nb]; d[q] = dstar[(size_t)q*N + nb]; }
+            face_term<T>(U, d, g, o);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) acc[q] += o[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            __builtin_nontemporal_store((s[q] - T(0.5)*acc[q])/dg, dstar + (size_t)q*N + i);
+    }
+    block_sum(ss, part, pofs + b);
+}
+
+// ---- upper sweep of colour [n0, n0+cnt), then e_i += delta_i
+//   delta_i = dstar_i - 1/2 sum_{nb >= ne} term(Qb_nb; delta_nb) / D_i
+template <typename T, int FC>
+__global__ __launch_bounds__(kB) void k_upper_upd(
+    int n0, int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
+    const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ dstar,
+    T *delta, T *e)
+{
+    const int t = xcd_remap()*kB + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = n0 + t, ne = n0 + cnt;
+    int nbv[FC];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
+    T acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = T(0);
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+        const int nb = nbv[f];
+        if (nb < ne) continue;
+        const Face4<T> g = ldg_nt(geo + (size_t)f*N + i);
+        T U[NV], d[NV], o[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) { U[q] = qb[(size_t)q*N + nb]; d[q] = delta[(size_t)q*N + nb]; }
+        face_term<T>(U, d, g, o);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += o[q];
+    }
+    const T dg = ldg_nt(diag + i);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const size_t o = (size_t)q*N + i;
+        const T dl = ldg_nt(dstar + o) - T(0.5)*acc[q]/dg;
+        __builtin_nontemporal_store(dl, delta + o);
+        __builtin_nontemporal_store(ldg_nt(e + o) + dl, e + o);
+    }
+}
+
+// ---- FP64 residual r = b - A(dq) on base Q over all cells.  Optionally
+// writes fl32(r) and the FP32 base state fl32(Q + dq) for the inner loop.
+template <int FC, bool OUT32>
+__global__ __launch_bounds__(kB) void k_residual64(
+    int N, const int *__restrict__ nei, const Face4<double> *__restrict__ geo,
+    const double *__restrict__ diag, const double *__restrict__ q, const double *__restrict__ b,
+    const double *__restrict__ dq, double *__restrict__ r, float *__restrict__ r32,
+    float *__restrict__ qb32, double *part)
+{
+    const int blk = blockIdx.x;
+    const int i = blk*kB + threadIdx.x;
+    double ss = 0.0;
+    if (i < N) {
+        double res[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) res[k] = 0.0;
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const int nb = ldg_nt(nei + (size_t)f*N + i);
+            if (nb == i) continue;
+            const Face4<double> g = ldg_nt(geo + (size_t)f*N + i);
+            double U[NV], d[NV], o[NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) { U[k] = q[(size_t)k*N + nb]; d[k] = dq[(size_t)k*N + nb]; }
+            face_term<double>(U, d, g, o);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) res[k] += o[k];
+        }
+        const double dg = ldg_nt(diag + i);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const size_t o = (size_t)k*N + i;
+            const double dqi = dq[o];
+            const double rv = ldg_nt(b + o) - (dg*dqi + 0.5*res[k]);
+            ss += rv*rv;
+            if (r) r[o] = rv;
+            if (OUT32) {
+                r32[o] = (float)rv;
+                qb32[o] = (float)(q[o] + dqi);
+            }
+        }
+    }
+    block_sum(ss, part, blk);
+}
+
+__global__ void k_sum_parts(const double *part, int n, double *out)
+{
+    // one wave, fixed order: lane l sums l, l+64, ...; then a fixed tree
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += 64) s += part[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_add32(int64_t n, double *__restrict__ y, const float *__restrict__ x)
+{
+    const int64_t st = (int64_t)gridDim.x*kB;
+    for (int64_t j = (int64_t)blockIdx.x*kB + threadIdx.x; j < n; j += st) y[j] += (double)x[j];
+}
+
+template <typename T> __global__ void k_fill(int64_t n, T *y, T v)
+{
+    const int64_t st = (int64_t)gridDim.x*kB;
+    for (int64_t j = (int64_t)blockIdx.x*kB + threadIdx.x; j < n; j += st) y[j] = v;
+}
+
+// packs S = fnv*n, L = fnv*lam and D = 1/dt + 1/2 sum fspr*fnv (FP64 and FP32)
+__global__ void k_pack(int N, int F, const double *__restrict__ fnv, const double *__restrict__ vfn,
+                       const double *__restrict__ fspr, const double *__restrict__ dt,
+                       Face4<double> *g64, Face4<float> *g32, double *d64, float *d32)
+{
+    const int i = blockIdx.x*kB + threadIdx.x;
+    if (i >= N) return;
+    double sd = 0.0;
+    for (int f = 0; f < F; ++f) {
+        const size_t o = (size_t)f*N + i;
+        const double fv = fnv[o], lam = fspr[o];
+        Face4<double> g;
+        g.s0 = fv*vfn[(size_t)(3*f + 0)*N + i];
+        g.s1 = fv*vfn[(size_t)(3*f + 1)*N + i];
+        g.s2 = fv*vfn[(size_t)(3*f + 2)*N + i];
+        g.l = fv*lam;
+        g64[o] = g;
+        g32[o] = Face4<float>{(float)g.s0, (float)g.s1, (float)g.s2, (float)g.l};
+        sd += 0.5*lam*fv;
+    }
+    const double d = 1.0/dt[i] + sd;
+    d64[i] = d;
+    d32[i] = (float)d;
+}
+
+inline dim3 grid1(int64_t n)
+{
+    int64_t b = (n + kB - 1)/kB;
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 20)));
+}
+
+}  // namespace
+
+// ============================================================== host driver
+
+struct ucfd_lusgs_solver {
+    int N = 0, F = 0, nc = 0;
+    std::vector<int> off;
+    int *nei = nullptr;
+    Face4<double> *g64 = nullptr;
+    Face4<float> *g32 = nullptr;
+    double *d64 = nullptr, *q64 = nullptr, *r64 = nullptr, *dstar64 = nullptr,
+           *delta64 = nullptr;
+    float *d32 = nullptr, *qb32 = nullptr, *r32 = nullptr, *e32 = nullptr, *dstar32 = nullptr,
+          *delta32 = nullptr;
+    double *part = nullptr, *norm = nullptr, *hnorm = nullptr;
+    int nparts = 0;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T> int dalloc(ucfd_lusgs_solver *s, T **p, size_t n, const char *fn)
+{
+    int rc = ucfd_check_hip(hipMalloc((void **)p, sizeof(T)*std::max<size_t>(n, 1)), fn, "hipMalloc");
+    if (!rc) s->allocs.push_back((void *)*p);
+    return rc;
+}
+
+void free_all(ucfd_lusgs_solver *s)
+{
+    for (void *p : s->allocs) (void)hipFree(p);
+    if (s->hnorm) (void)hipHostFree(s->hnorm);
+    s->allocs.clear();
+}
+
+int copy_in(void *dst, const void *src, size_t bytes, hipStream_t st, const char *fn)
+{
+    return ucfd_check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st), fn,
+                          "hipMemcpyAsync(in)");
+}
+
+// lower (with residual) over all colours, norm of s to *norm
+template <typename T>
+int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb, const T *rhs,
+               const T *e, T *dstar, const Face4<T> *geo, const T *diag)
+{
+    int pofs = 0;
+    for (int c = 0; c < s->nc; ++c) {
+        const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
+        if (cnt <= 0) continue;
+        const dim3 g((cnt + kB - 1)/kB);
+#define L4(RES) hipLaunchKernelGGL((k_lower_res<T, 4, RES>), g, dim3(kB), 0, st, n0, cnt, s->N, \
+                                   s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
+#define L6(RES) hipLaunchKernelGGL((k_lower_res<T, 6, RES>), g, dim3(kB), 0, st, n0, cnt, s->N, \
+                                   s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
+        if (s->F == 4) { if (with_res) L4(true); else L4(false); }
+        else           { if (with_res) L6(true); else L6(false); }
+#undef L4
+#undef L6
+        pofs += (int)g.x;
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, s->part, pofs, s->norm);
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}
+
+template <typename T>
+int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar, T *delta, T *e,
+               const Face4<T> *geo, const T *diag)
+{
+    for (int c = s->nc - 1; c >= 0; --c) {
+        const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
+        if (cnt <= 0) continue;
+        const dim3 g((cnt + kB - 1)/kB);
+        if (s->F == 4)
+            hipLaunchKernelGGL((k_upper_upd<T, 4>), g, dim3(kB), 0, st, n0, cnt, s->N, s->nei, geo,
+                               diag, qb, dstar, delta, e);
+        else
+            hipLaunchKernelGGL((k_upper_upd<T, 6>), g, dim3(kB), 0, st, n0, cnt, s->N, s->nei, geo,
+                               diag, qb, dstar, delta, e);
+    }
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}
+
+int residual64(ucfd_lusgs_solver *s, hipStream_t st, const double *b, const double *dq, double *r,
+               bool out32)
+{
+    const dim3 g((s->N + kB - 1)/kB);
+#define R(FC, O) hipLaunchKernelGGL((k_residual64<FC, O>), g, dim3(kB), 0, st, s->N, s->nei, s->g64, \
+                                    s->d64, s->q64, b, dq, r, s->r32, s->qb32, s->part)
+    if (s->F == 4) { if (out32) R(4, true); else R(4, false); }
+    else           { if (out32) R(6, true); else R(6, false); }
+#undef R
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, s->part, (int)g.x, s->norm);
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}
+
+int read_norm(ucfd_lusgs_solver *s, hipStream_t st, double *out, const char *fn)
+{
+    int rc = ucfd_check_hip(hipMemcpyAsync(s->hnorm, s->norm, sizeof(double), hipMemcpyDeviceToHost,
+                                           st), fn, "hipMemcpyAsync(norm)");
+    if (!rc) rc = ucfd_check_hip(hipStreamSynchronize(st), fn, "hipStreamSynchronize");
+    *out = sqrt(*s->hnorm);
+    return rc;
+}
+
+}  // namespace
+
+UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nface,
+                                      int ncolors, const int *color_off, const int *nei_ele,
+                                      const double *fnorm_vol, const double *vec_fnorm,
+                                      const double *fspr, const double *dt, const double *upts)
+{
+    const char *fn = "ucfd_lusgs_solver_create";
+    if (!out || neles <= 0 || (nface != 4 && nface != 6) || ncolors <= 0 || !color_off ||
+        !nei_ele || !fnorm_vol || !vec_fnorm || !fspr || !dt || !upts)
+        return ucfd_report(UCFD_ERR_ARG, fn, "bad arguments (nface must be 4 or 6, 3-D Euler)");
+    if (ucfd_config().ndims != 3)
+        return ucfd_report(UCFD_ERR_ARG, fn, "the solver supports ndims = 3");
+    if (color_off[0] != 0 || color_off[ncolors] != neles)
+        return ucfd_report(UCFD_ERR_ARG, fn, "color_off must run from 0 to neles");
+    for (int c = 0; c < ncolors; ++c)
+        if (color_off[c + 1] < color_off[c])
+            return ucfd_report(UCFD_ERR_ARG, fn, "color_off must be non-decreasing");
+    int rc = ucfd_ensure_device(fn);
+    if (rc) return rc;
+    const size_t N = (size_t)neles, F = (size_t)nface;
+
+    // the neighbour table is validated on the host: in range, and no
+    // neighbour inside the cell's own colour range (a valid colouring)
+    std::vector<int> hn(F*N);
+    if ((rc = ucfd_check_hip(hipMemcpy(hn.data(), nei_ele, F*N*4, hipMemcpyDefault), fn,
+                             "hipMemcpy(nei_ele)")))
+        return rc;
+    for (int c = 0; c < ncolors; ++c)
+        for (int i = color_off[c]; i < color_off[c + 1]; ++i)
+            for (size_t f = 0; f < F; ++f) {
+                const int nb = hn[f*N + i];
+                if (nb < 0 || nb >= neles)
+                    return ucfd_report(UCFD_ERR_INDEX, fn, "nei_ele entry out of range");
+                if (nb != i && nb >= color_off[c] && nb < color_off[c + 1])
+                    return ucfd_report(UCFD_ERR_ARG, fn, "a face joins two cells of one colour");
+            }
+
+    ucfd_lusgs_solver *s = new ucfd_lusgs_solver();
+    s->N = neles;
+    s->F = nface;
+    s->nc = ncolors;
+    s->off.assign(color_off, color_off + ncolors + 1);
+    int maxparts = 0;
+    for (int c = 0; c < ncolors; ++c) maxparts += (s->off[c + 1] - s->off[c] + kB - 1)/kB;
+    s->nparts = std::max(maxparts, (neles + kB - 1)/kB) + 8;
+    const hipStream_t st = ucfd_stream();
+    double *tfnv = nullptr, *tvfn = nullptr, *tsp = nullptr, *tdt = nullptr;
+    if ((rc = dalloc(s, &s->nei, F*N, fn)) || (rc = dalloc(s, &s->g64, F*N, fn)) ||
+        (rc = dalloc(s, &s->g32, F*N, fn)) || (rc = dalloc(s, &s->d64, N, fn)) ||
+        (rc = dalloc(s, &s->d32, N, fn)) || (rc = dalloc(s, &s->q64, NV*N, fn)) ||
+        (rc = dalloc(s, &s->r64, NV*N, fn)) || (rc = dalloc(s, &s->dstar64, NV*N, fn)) ||
+        (rc = dalloc(s, &s->delta64, NV*N, fn)) || (rc = dalloc(s, &s->qb32, NV*N, fn)) ||
+        (rc = dalloc(s, &s->r32, NV*N, fn)) || (rc = dalloc(s, &s->e32, NV*N, fn)) ||
+        (rc = dalloc(s, &s->dstar32, NV*N, fn)) || (rc = dalloc(s, &s->delta32, NV*N, fn)) ||
+        (rc = dalloc(s, &s->part, (size_t)s->nparts, fn)) || (rc = dalloc(s, &s->norm, 8, fn)) ||
+        (rc = dalloc(s, &tfnv, F*N, fn)) || (rc = dalloc(s, &tvfn, 3*F*N, fn)) ||
+        (rc = dalloc(s, &tsp, F*N, fn)) || (rc = dalloc(s, &tdt, N, fn))) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    if ((rc = ucfd_check_hip(hipHostMalloc((void **)&s->hnorm, 64, hipHostMallocDefault), fn,
+                             "hipHostMalloc"))) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    rc = copy_in(s->nei, hn.data(), F*N*4, st, fn);
+    if (!rc) rc = copy_in(tfnv, fnorm_vol, F*N*8, st, fn);
+    if (!rc) rc = copy_in(tvfn, vec_fnorm, 3*F*N*8, st, fn);
+    if (!rc) rc = copy_in(tsp, fspr, F*N*8, st, fn);
+    if (!rc) rc = copy_in(tdt, dt, N*8, st, fn);
+    if (!rc) rc = copy_in(s->q64, upts, NV*N*8, st, fn);
+    if (!rc) {
+        hipLaunchKernelGGL(k_pack, dim3((neles + kB - 1)/kB), dim3(kB), 0, st, neles, nface, tfnv,
+                           tvfn, tsp, tdt, s->g64, s->g32, s->d64, s->d32);
+        rc = ucfd_check_hip(hipStreamSynchronize(st), fn, "pack");
+    }
+    // the FP64 staging copies of the raw geometry are no longer needed
+    for (void *p : {(void *)tfnv, (void *)tvfn, (void *)tsp, (void *)tdt}) {
+        (void)hipFree(p);
+        s->allocs.erase(std::remove(s->allocs.begin(), s->allocs.end(), p), s->allocs.end());
+    }
+    if (rc) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return UCFD_OK;
+}
+
+UCFD_API int ucfd_lusgs_solver_destroy(ucfd_lusgs_solver *s)
+{
+    if (!s) return UCFD_OK;
+    (void)hipStreamSynchronize(ucfd_stream());
+    free_all(s);
+    delete s;
+    return UCFD_OK;
+}
+
+// r = b - A(dq) in FP64 (b, dq, r device or host [5][N]); *rnorm = ||r||_2.
+UCFD_API int ucfd_lusgs_residual(ucfd_lusgs_solver *s, const double *b, const double *dq,
+                                 double *r, double *rnorm)
+{
+    const char *fn = "ucfd_lusgs_residual";
+    if (!s || !b || !dq) return ucfd_report(UCFD_ERR_ARG, fn, "bad arguments");
+    const size_t n = (size_t)NV*s->N;
+    UcfdStage sg(fn);
+    const int ab = sg.add(b, n*8, UCFD_IN), aq = sg.add(dq, n*8, UCFD_IN);
+    const int ar = r ? sg.add(r, n*8, UCFD_OUT) : -1;
+    int rc = sg.enter();
+    if (rc) return rc;
+    const hipStream_t st = ucfd_stream();
+    rc = residual64(s, st, (const double *)sg.dev(ab), (const double *)sg.dev(aq),
+                    ar >= 0 ? (double *)sg.dev(ar) : nullptr, false);
+    double nrm = 0.0;
+    if (!rc) rc = read_norm(s, st, &nrm, fn);
+    if (rnorm) *rnorm = nrm;
+    return sg.leave(rc);
+}
+
+// Solve A(dq) = b to ||b - A(dq)|| <= rtol*||b|| by defect correction.
+// mode UCFD_SOLVE_FP64: FP64 sweeps; UCFD_SOLVE_MIXED: FP32 inner sweeps with
+// FP64 outer residual and FP64 accumulation of dq.  dq is output (the start
+// value is zero).  maxiter bounds the total number of LU-SGS passes.
+UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, double *dq,
+                              double rtol, int maxiter, ucfd_solve_stats *stats)
+{
+    const char *fn = "ucfd_lusgs_solve";
+    if (!s || !b || !dq || maxiter <= 0 || (mode != UCFD_SOLVE_FP64 && mode != UCFD_SOLVE_MIXED))
+        return ucfd_report(UCFD_ERR_ARG, fn, "bad arguments");
+    const size_t n = (size_t)NV*s->N;
+    UcfdStage sg(fn);
+    const int ab = sg.add(b, n*8, UCFD_IN), aq = sg.add(dq, n*8, UCFD_OUT);
+    int rc = sg.enter();
+    if (rc) return rc;
+    const hipStream_t st = ucfd_stream();
+    const double *B = (const double *)sg.dev(ab);
+    double *DQ = (double *)sg.dev(aq);
+    ucfd_solve_stats info;
+    memset(&info, 0, sizeof info);
+    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, DQ, 0.0);
+    double bnorm = 0.0, rn = 0.0;
+
+    auto record = [&](double v) {
+        if (info.nhist < UCFD_SOLVE_HIST) info.history[info.nhist++] = v;
+    };
+
+    if (mode == UCFD_SOLVE_FP64) {
+        // e = dq; iteration j: s = b - A(e) (in the lower launches), e += M^-1 s
+        for (int it = 0; it < maxiter && !rc; ++it) {
+            rc = lower_pass<double>(s, st, it > 0, s->q64, B, DQ, s->dstar64, s->g64, s->d64);
+            if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, DQ, s->g64,
+                                             s->d64);
+            if (!rc) rc = read_norm(s, st, &rn, fn);
+            if (rc) break;
+            if (it == 0) bnorm = rn;
+            record(bnorm > 0 ? rn/bnorm : 0.0);
+            info.passes++;
+            if (bnorm == 0.0 || rn <= rtol*bnorm) break;
+        }
+        // the last check saw the iterate before its update; report the
+        // residual of the dq actually returned
+        if (!rc) rc = residual64(s, st, B, DQ, nullptr, false);
+        if (!rc) rc = read_norm(s, st, &rn, fn);
+        info.outer = info.passes + 1;
+    } else {
+        // outer FP64 residual; inner FP32 defect correction on the correction
+        while (!rc && info.passes < maxiter) {
+            rc = residual64(s, st, B, DQ, nullptr, true);
+            if (!rc) rc = read_norm(s, st, &rn, fn);
+            if (rc) break;
+            if (info.outer == 0) bnorm = rn;
+            info.outer++;
+            record(bnorm > 0 ? rn/bnorm : 0.0);
+            if (bnorm == 0.0 || rn <= rtol*bnorm) break;
+            // the inner loop only needs to reach the outer target, within the
+            // accuracy FP32 can deliver
+            const double inner_tol = std::max(1e-5, 0.5*rtol*bnorm/rn);
+            hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
+            double s0 = 0.0;
+            for (int j = 0; info.passes < maxiter && !rc; ++j) {
+                rc = lower_pass<float>(s, st, j > 0, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
+                                       s->d32);
+                if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32,
+                                                s->g32, s->d32);
+                double sn = 0.0;
+                if (!rc) rc = read_norm(s, st, &sn, fn);
+                info.passes++;
+                if (j == 0) s0 = sn;
+                if (s0 == 0.0 || sn <= inner_tol*s0) break;
+            }
+            hipLaunchKernelGGL(k_add32, grid1(n), dim3(kB), 0, st, (int64_t)n, DQ, s->e32);
+        }
+        if (!rc && info.passes >= maxiter) {
+            // report the residual of the returned dq
+            rc = residual64(s, st, B, DQ, nullptr, false);
+            if (!rc) rc = read_norm(s, st, &rn, fn);
+            record(bnorm > 0 ? rn/bnorm : 0.0);
+        }
+    }
+    info.rnorm0 = bnorm;
+    info.rel_residual = bnorm > 0 ? rn/bnorm : 0.0;
+    info.converged = bnorm == 0.0 || rn <= rtol*bnorm;
+    if (stats) *stats = info;
+    return sg.leave(rc);
+}

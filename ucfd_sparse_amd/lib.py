@@ -53,7 +53,23 @@ _SIGS = {
     "ucfd_permute_nei": [c_int, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_declare_colored_layout": [c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_release_colored_layout": [c_void_p, c_void_p],
+    "ucfd_lusgs_solver_create": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 7,
+    "ucfd_lusgs_solver_destroy": [c_void_p],
+    "ucfd_lusgs_residual": [c_void_p] * 5,
+    "ucfd_lusgs_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_void_p],
 }
+
+UCFD_SOLVE_FP64, UCFD_SOLVE_MIXED, UCFD_SOLVE_HIST = 0, 1, 128
+
+
+class SolveStats(ctypes.Structure):
+    """ucfd_solve_stats (include/ucfd_ext.h)."""
+    _fields_ = [("converged", c_int), ("outer", c_int), ("passes", c_int), ("nhist", c_int),
+                ("rnorm0", c_double), ("rel_residual", c_double),
+                ("history", c_double * UCFD_SOLVE_HIST)]
+
+    def hist(self):
+        return [self.history[k] for k in range(self.nhist)]
 _RESTYPE = {"ucfd_get_stream": c_void_p, "ucfd_coloring_conflicts": ctypes.c_longlong}
 
 
