@@ -269,13 +269,22 @@ __global__ __launch_bounds__(kB) void k_residual64(
     block_sum(ss, part, blk);
 }
 
-__global__ void k_sum_parts(const double *part, int n, double *out)
+// Sum of the block partials in a fixed order (thread t sums t, t+1024, ...,
+// then a fixed tree), so the norm and every convergence decision are
+// reproducible run to run.
+__global__ __launch_bounds__(1024) void k_sum_parts(const double *part, int n, double *out)
 {
-    // one wave, fixed order: lane l sums l, l+64, ...; then a fixed tree
+    __shared__ double red[16];
     double s = 0.0;
-    for (int j = threadIdx.x; j < n; j += 64) s += part[j];
+    for (int j = threadIdx.x; j < n; j += 1024) s += part[j];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x % 64 == 0) red[threadIdx.x/64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w) t += red[w];
+        *out = t;
+    }
 }
 
 __global__ void k_add32(int64_t n, double *__restrict__ y, const float *__restrict__ x)
@@ -382,7 +391,7 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
 #undef L6
         pofs += (int)g.x;
     }
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, s->part, pofs, s->norm);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
@@ -413,7 +422,7 @@ int residual64(ucfd_lusgs_solver *s, hipStream_t st, const double *b, const doub
     if (s->F == 4) { if (out32) R(4, true); else R(4, false); }
     else           { if (out32) R(6, true); else R(6, false); }
 #undef R
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, s->part, (int)g.x, s->norm);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, (int)g.x, s->norm);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
