@@ -262,7 +262,7 @@ def main():
     lower_bytes_launch = b_sweep * N / nc
     achieved = lower_bytes_launch / (lower_launch_ms * 1e-3) / 1e9
     traffic = None
-    tr = read_traffic(os.path.join(REPO, "profiles", "traffic_lower_sweep_cfg1.txt"))
+    tr = read_traffic(os.path.join(REPO, "profiles", "traffic_lower_sweep_tet96.txt"))
     if tr and "hbm_bytes_per_launch" in tr:
         traffic = tr["hbm_bytes_per_launch"]
 

@@ -165,3 +165,98 @@ def dc_solve(nei, icolor, lcolor, fnorm_vol, vec_fnorm, fspr, dt, q, b, rtol, ma
                            c(dt).ctypes.data, c(q[:D + 2]).ctypes.data, c(b[:D + 2]).ctypes.data,
                            dq.ctypes.data, rtol, maxpass, hist.ctypes.data)
     return dq, n, hist[:n + 1]
+
+
+# ----------------------------------------------------------------- Krylov
+def _klib():
+    L = lib()
+    if not hasattr(L, "_krylov_bound"):
+        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.orc_spmv_bsr.argtypes = [ci, ci, vp, vp, vp, cd, vp, cd, vp]
+        L.orc_ludcmp.argtypes = [ci, vp]
+        L.orc_lusub.argtypes = [ci, vp, vp]
+        L.orc_blusgs_prepare.argtypes = [ci, ci, vp, vp, vp]
+        L.orc_blusgs_apply.argtypes = [ci, ci, vp, vp, vp, vp, vp, ci, vp, vp]
+        L.orc_gmres_solve.argtypes = [ci, ci] + [vp] * 5 + [ci, vp, ci, ci, cd, ci, vp, vp, vp,
+                                                           vp, vp]
+        L.orc_bicgstab_solve.argtypes = [ci, ci] + [vp] * 5 + [ci, vp, ci, cd, ci, vp, vp, vp,
+                                                              vp, vp]
+        for f in ("orc_spmv_bsr", "orc_ludcmp", "orc_lusub", "orc_blusgs_prepare",
+                  "orc_blusgs_apply"):
+            getattr(L, f).restype = None
+        L.orc_gmres_solve.restype = ci
+        L.orc_bicgstab_solve.restype = ci
+        L._krylov_bound = True
+    return L
+
+
+def spmv_bsr(A, x, alpha=1.0, beta=0.0, y=None):
+    y = np.zeros(A.bn * A.blk) if y is None else np.ascontiguousarray(y, np.float64).copy()
+    _klib().orc_spmv_bsr(A.bn, A.blk, A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                         A.values.ctypes.data, alpha, _c(x, np.float64).ctypes.data, beta,
+                         y.ctypes.data)
+    return y
+
+
+def ludcmp(Ab):
+    a = np.ascontiguousarray(Ab, np.float64).copy()
+    _klib().orc_ludcmp(a.shape[0], a.ctypes.data)
+    return a
+
+
+def lusub(LU, b):
+    x = np.ascontiguousarray(b, np.float64).copy()
+    _klib().orc_lusub(LU.shape[0], _c(LU, np.float64).ctypes.data, x.ctypes.data)
+    return x
+
+
+def blusgs_prepare(A):
+    dv = np.empty((A.bn, A.blk, A.blk))
+    _klib().orc_blusgs_prepare(A.bn, A.blk, A.diagslots.ctypes.data, A.values.ctypes.data,
+                               dv.ctypes.data)
+    return dv
+
+
+def blusgs_apply(A, dv, b, icolors=None):
+    x = np.ascontiguousarray(b, np.float64).copy()
+    nc = 0 if icolors is None else len(icolors) - 1
+    ic = None if icolors is None else _c(icolors, np.int32)
+    _klib().orc_blusgs_apply(A.bn, A.blk, A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                             A.diagslots.ctypes.data, A.values.ctypes.data, dv.ctypes.data, nc,
+                             None if ic is None else ic.ctypes.data, x.ctypes.data)
+    return x
+
+
+def gmres(A, b, m, maxiter, tol, right=False, precon=True, icolors=None, x0=None):
+    """returns (x, restarts, stat, residual, history)"""
+    dv = blusgs_prepare(A) if precon else None
+    x = np.zeros(A.bn * A.blk) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    st = ctypes.c_int()
+    res = ctypes.c_double()
+    hist = np.zeros(maxiter + 1)
+    nc = 0 if icolors is None else len(icolors) - 1
+    ic = None if icolors is None else _c(icolors, np.int32)
+    it = _klib().orc_gmres_solve(A.bn, A.blk, A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                                 A.diagslots.ctypes.data, A.values.ctypes.data,
+                                 None if dv is None else dv.ctypes.data, nc,
+                                 None if ic is None else ic.ctypes.data, m, maxiter, tol,
+                                 int(right), x.ctypes.data, _c(b, np.float64).ctypes.data,
+                                 ctypes.byref(st), ctypes.byref(res), hist.ctypes.data)
+    return x, it, st.value, res.value, hist[:it + 1]
+
+
+def bicgstab(A, b, maxiter, tol, relative=False, precon=True, icolors=None):
+    dv = blusgs_prepare(A) if precon else None
+    x = np.zeros(A.bn * A.blk)
+    st = ctypes.c_int()
+    res = ctypes.c_double()
+    hist = np.zeros(maxiter + 1)
+    nc = 0 if icolors is None else len(icolors) - 1
+    ic = None if icolors is None else _c(icolors, np.int32)
+    it = _klib().orc_bicgstab_solve(A.bn, A.blk, A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                                    A.diagslots.ctypes.data, A.values.ctypes.data,
+                                    None if dv is None else dv.ctypes.data, nc,
+                                    None if ic is None else ic.ctypes.data, maxiter, tol,
+                                    int(relative), x.ctypes.data, _c(b, np.float64).ctypes.data,
+                                    ctypes.byref(st), ctypes.byref(res), hist.ctypes.data)
+    return x, it, st.value, res.value, hist[:it + 1]

@@ -60,7 +60,7 @@ def test_greedy_matches_independent_statement(kind, arg, seed):
     assert np.array_equal(np.diff(off), np.bincount(lcolor)[1:])
 
 
-@pytest.mark.parametrize("name", ["hex8p", "tet3p", "tet5p", "cfg0_hex32"])
+@pytest.mark.parametrize("name", ["hex8p", "tet3p", "tet5p", "hex32p"])
 def test_greedy_matches_fixture_coloring(name):
     z = np.load(os.path.join(GOLD, f"lusgs_{name}.npz"))
     kind, params, seed = str(z["kind"]), [int(v) for v in z["params"]], int(z["seed"])
@@ -70,7 +70,7 @@ def test_greedy_matches_fixture_coloring(name):
     assert np.array_equal(lcolor, z["lcolor"].astype(np.int32))
 
 
-def test_config0_coloring_has_six_colors():
+def test_hex32_permuted_coloring_has_six_colors():
     mesh = M.hex_mesh(32, 32, 32, seed=0)
     nc, lcolor, icolor, off = color(mesh.nei_ele)
     assert nc == 6
@@ -128,7 +128,7 @@ def test_benchmark_mesh_colorings_fixture():
         pytest.skip("colorings.npz not generated")
     import hashlib
     z = np.load(path)
-    for name, m, seed in (("cfg4_tet55", 55, 4), ("cfg3_tet70", 70, 3), ("cfg1_tet96", 96, 1)):
+    for name, m, seed in (("tet55p", 55, 4), ("tet70p", 70, 3), ("tet96p", 96, 1)):
         mesh = M.tet_mesh(m, seed=seed)
         nc, lcolor, icolor, off = color(mesh.nei_ele)
         assert np.array_equal(np.bincount(lcolor)[1:], z[name + "_counts"])

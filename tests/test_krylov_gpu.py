@@ -1,0 +1,278 @@
+"""Krylov path on the GPU, through the C ABI (include/ucfdmat.h,
+ucfdprecon.h, ucfdsolver.h), against the CPU oracle on the same seeded
+inputs.
+
+Bars: SpMV and the FP64 block LU-SGS apply are bit-exact (same operation
+order, no FMA contraction); GMRES / BiCGSTAB follow the oracle step for step
+with device dot products (fixed-order two-stage sums, so not bitwise equal to
+the sequential CPU sum): same iteration counts and status, residual
+histories within 1e-6 relative, and the true residual of the GPU solution
+within the requested tolerance.  The mixed apply (FP32 block products, FP64
+diagonal LU) is within 1e-5 relative of the FP64 apply.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from ucfd_sparse_amd import layout as LY
+from ucfd_sparse_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def say(msg):
+    print(f"[krylov-gpu] {msg}", flush=True)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def system(kind, size, seed, blk=5, colored=False):
+    if kind == "tet":
+        mesh = M.tet_mesh(size, seed=seed, permute=True)
+    else:
+        mesh = M.hex_mesh(*size, seed=seed, permute=True)
+    flow = M.flow_fields(mesh, seed=seed)
+    off = None
+    if colored:
+        lay = LY.colored_layout(mesh, flow)
+        mesh, flow, off = lay.mesh, lay.flow, lay.off
+    A = M.block_system(mesh, flow, blk=blk, seed=seed)
+    return A, off
+
+
+def dev_bsr(A):
+    class D:
+        pass
+    d = D()
+    d.bn, d.blk = A.bn, A.blk
+    d.rowptr, d.colidx, d.diagslots = dev(A.rowptr), dev(A.colidx), dev(A.diagslots)
+    d.values = dev(A.values)
+    return d
+
+
+@pytest.fixture(scope="module")
+def K():
+    from ucfd_sparse_amd import krylov
+    return krylov
+
+
+@pytest.fixture(scope="module")
+def tet3():
+    A, _ = system("tet", 3, 301)
+    return A, M.normal_fill(A.bn * A.blk, 301), M.normal_fill(A.bn * A.blk, 302)
+
+
+@pytest.mark.parametrize("blk", [1, 3, 5])
+def test_spmv_device_bitwise(K, blk):
+    say(f"BSR SpMV blk={blk}, device arrays")
+    A, _ = system("tet", 4, 11, blk=blk)
+    x = M.normal_fill(A.bn * blk, 12)
+    y0 = M.normal_fill(A.bn * blk, 13)
+    mat = K.SpMat.from_bsr(dev_bsr(A))
+    yd = dev(y0)
+    mat.mult(dev(x), yd, alpha=-1.0, beta=0.5)
+    assert np.array_equal(yd.cpu().numpy(), O.spmv_bsr(A, x, -1.0, 0.5, y=y0))
+
+
+def test_spmv_host_arrays_staged(K, tet3):
+    say("BSR SpMV, host arrays")
+    A, b, x = tet3
+    mat = K.SpMat.from_bsr(A)
+    y = np.zeros(A.bn * A.blk)
+    mat.mult(x, y, alpha=1.5)
+    assert np.array_equal(y, O.spmv_bsr(A, x, 1.5, 0.0))
+
+
+def test_csr_spmv_matches_blk1_oracle(K):
+    say("CSR SpMV")
+    A, _ = system("hex", (5, 4, 3), 21, blk=1)
+    x = M.normal_fill(A.bn, 22)
+    mat = K.SpMat(dev(A.rowptr), dev(A.colidx), dev(A.values.reshape(-1)), A.bn)
+    y = dev(np.zeros(A.bn))
+    mat.mult(dev(x), y, alpha=2.0)
+    assert np.array_equal(y.cpu().numpy(), O.spmv_bsr(A, x, 2.0, 0.0))
+
+
+def test_apply_sequential_level_scheduled_bitwise(K, tet3):
+    say("sequential BLU-SGS apply (dependency levels)")
+    A, b, _ = tet3
+    pc = K.Precon.from_bsr(dev_bsr(A))
+    bd = dev(b)
+    pc.apply(bd)
+    ref = O.blusgs_apply(A, O.blusgs_prepare(A), b)
+    assert np.array_equal(bd.cpu().numpy(), ref)
+
+
+def test_apply_colored_bitwise(K):
+    say("coloured BLU-SGS apply, colour-ordered matrix")
+    A, off = system("tet", 5, 41, colored=True)
+    b = M.normal_fill(A.bn * A.blk, 42)
+    pc = K.Precon.from_bsr(dev_bsr(A), icolors=off)
+    bd = dev(b)
+    pc.apply(bd)
+    dv = O.blusgs_prepare(A)
+    ref = O.blusgs_apply(A, dv, b, icolors=off)
+    assert np.array_equal(bd.cpu().numpy(), ref)
+    assert np.array_equal(ref, O.blusgs_apply(A, dv, b))
+
+
+def test_apply_colored_with_coupled_ranges_bitwise(K, tet3):
+    """Colour ranges that couple internally: the rows of each range run in
+    the reference's one-thread order (dependency levels inside the range)."""
+    say("coloured apply, internally coupled ranges")
+    A, b, _ = tet3
+    off = np.array([0, A.bn // 3, A.bn // 2, A.bn], np.int32)
+    pc = K.Precon.from_bsr(dev_bsr(A), icolors=off)
+    bd = dev(b)
+    pc.apply(bd)
+    ref = O.blusgs_apply(A, O.blusgs_prepare(A), b, icolors=off)
+    assert np.array_equal(bd.cpu().numpy(), ref)
+
+
+def test_apply_host_arrays(K, tet3):
+    say("sequential apply, host arrays")
+    A, b, _ = tet3
+    pc = K.Precon.from_bsr(A)
+    bh = b.copy()
+    pc.apply(bh)
+    assert np.array_equal(bh, O.blusgs_apply(A, O.blusgs_prepare(A), b))
+
+
+@pytest.mark.parametrize("colored", [False, True])
+def test_apply_mixed_precision_close(K, colored):
+    say(f"mixed FP32/FP64 apply, colored={colored}")
+    A, off = system("tet", 5, 51, colored=colored)
+    b = M.normal_fill(A.bn * A.blk, 52)
+    pc = K.Precon.from_bsr(dev_bsr(A), icolors=off if colored else None, fp32=True)
+    bd = dev(b)
+    pc.apply(bd)
+    got = bd.cpu().numpy()
+    ref = O.blusgs_apply(A, O.blusgs_prepare(A), b, icolors=off if colored else None)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    say(f"  rel err {err:.2e}")
+    assert np.all(np.isfinite(got))
+    assert 0 < err <= 1e-5
+
+
+def _close_hist(h_gpu, h_orc, n):
+    h_gpu, h_orc = np.asarray(h_gpu[:n]), np.asarray(h_orc[:n])
+    rel = np.abs(h_gpu - h_orc) / np.maximum(np.abs(h_orc), 1e-300)
+    return float(rel.max()) if n else 0.0
+
+
+@pytest.mark.parametrize("precon", [True, False])
+def test_gmres_left_matches_oracle(K, tet3, precon):
+    say(f"GMRES(5) left-preconditioned, precon={precon}")
+    A, b, _ = tet3
+    tol, m, maxiter = (1e-10, 5, 40) if precon else (1e-9, 4, 60)
+    xo, ito, sto, reso, histo = O.gmres(A, b, m=m, maxiter=maxiter, tol=tol, precon=precon)
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    pc = K.Precon.from_bsr(D) if precon else K.Precon.none()
+    s = K.Solver("gmres", A.bn * A.blk, maxiter, tol, m=m)
+    x = dev(np.zeros(A.bn * A.blk))
+    st, it, res = s.solve(pc, mat, x, dev(b))
+    say(f"  gpu it={it} st={st} res={res:.3e}; oracle it={ito} st={sto} res={reso:.3e}")
+    assert (it, st) == (ito, sto)
+    assert _close_hist(s.history(), histo, it + 1 if it < maxiter else it) <= 1e-6
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(b - A.to_scipy() @ xg) <= 10 * tol
+    assert np.allclose(xg, xo, rtol=1e-6, atol=1e-9 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("precon", [True, False])
+def test_bicgstab_matches_oracle(K, tet3, precon):
+    say(f"BiCGSTAB, precon={precon}")
+    A, b, _ = tet3
+    tol, maxiter = (1e-10, 60) if precon else (1e-9, 80)
+    xo, ito, sto, reso, histo = O.bicgstab(A, b, maxiter=maxiter, tol=tol, precon=precon)
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    pc = K.Precon.from_bsr(D) if precon else K.Precon.none()
+    s = K.Solver("bicgstab", A.bn * A.blk, maxiter, tol)
+    x = dev(np.zeros(A.bn * A.blk))
+    st, it, res = s.solve(pc, mat, x, dev(b))
+    say(f"  gpu it={it} st={st} res={res:.3e}; oracle it={ito} st={sto} res={reso:.3e}")
+    assert (it, st) == (ito, sto)
+    assert _close_hist(s.history(), histo, it + 1) <= 1e-6
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(b - A.to_scipy() @ xg) <= 1e-8
+    assert np.allclose(xg, xo, rtol=1e-6, atol=1e-9 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("kind", ["gmres", "bicgstab"])
+def test_right_preconditioned_mixed_reaches_rtol(K, kind):
+    """The Krylov workload's form: right-preconditioned by the coloured
+    block LU-SGS with the FP32 apply, FP64 Krylov, rtol 1e-10."""
+    say(f"{kind} right/relative, coloured FP32 apply, 12^3x6 tets")
+    A, off = system("tet", 12, 61, colored=True)
+    b = M.normal_fill(A.bn * A.blk, 62)
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    pc = K.Precon.from_bsr(D, icolors=off, fp32=True)
+    s = K.Solver(kind, A.bn * A.blk, 200, 1e-10, m=30, right=True)
+    x = dev(np.zeros(A.bn * A.blk))
+    st, it, res = s.solve(pc, mat, x, dev(b))
+    rel = np.linalg.norm(b - A.to_scipy() @ x.cpu().numpy()) / np.linalg.norm(b)
+    say(f"  it={it} st={st} true rel residual {rel:.2e}")
+    assert st == 0
+    assert rel <= 1e-9
+    if kind == "gmres":
+        xo, ito, sto, _, _ = O.gmres(A, b, m=30, maxiter=200, tol=1e-10, right=True,
+                                     icolors=off)
+        assert sto == 0
+        assert np.allclose(x.cpu().numpy(), xo, rtol=1e-7, atol=1e-8 * np.abs(xo).max())
+
+
+def test_solves_are_deterministic(K, tet3):
+    say("determinism of two GMRES solves")
+    A, b, _ = tet3
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    pc = K.Precon.none()
+    out = []
+    for _ in range(2):
+        s = K.Solver("gmres", A.bn * A.blk, 30, 1e-9, m=4)
+        x = dev(np.zeros(A.bn * A.blk))
+        s.solve(pc, mat, x, dev(b))
+        out.append(x.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_status_codes(K, tet3):
+    say("solver status codes")
+    A, b, _ = tet3
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    pc = K.Precon.none()
+    s = K.Solver("gmres", A.bn * A.blk, 1, 1e-14, m=2)
+    st, it, _ = s.solve(pc, mat, dev(np.zeros(A.bn * A.blk)), dev(b))
+    assert (st, it) == (K.REACH_ITERMAX, 1)
+    s2 = K.Solver("bicgstab", A.bn * A.blk, 5, 1e-10)
+    st, it, _ = s2.solve(pc, mat, dev(np.zeros(A.bn * A.blk)), dev(np.zeros(A.bn * A.blk)))
+    assert (st, it) == (K.RHOBREAKDOWN, 0)
+
+
+def test_errors_are_reported_not_faulted(K, tet3):
+    say("index and pivot errors")
+    from ucfd_sparse_amd import lib as L
+    A, b, x = tet3
+    bad = A.colidx.copy()
+    bad[7] = A.bn + 5
+    mat = K.SpMat(dev(A.rowptr), dev(bad), dev(A.values), A.bn, A.blk)
+    with pytest.raises(L.UCFDError):
+        mat.mult(dev(x), dev(np.zeros(A.bn * A.blk)))
+    with pytest.raises(L.UCFDError):
+        K.Precon(dev(A.rowptr), dev(bad), dev(A.diagslots), dev(A.values), A.bn, A.blk)
+    vals = A.values.copy()
+    vals[A.diagslots[3]] = 0.0
+    with pytest.raises(L.UCFDError):
+        K.Precon(dev(A.rowptr), dev(A.colidx), dev(A.diagslots), dev(vals), A.bn, A.blk)
+    # the library keeps working after the errors
+    ok = K.SpMat.from_bsr(dev_bsr(A))
+    y = dev(np.zeros(A.bn * A.blk))
+    ok.mult(dev(x), y)
+    assert np.array_equal(y.cpu().numpy(), O.spmv_bsr(A, x))

@@ -106,7 +106,7 @@ def torch_dev():
     return cv
 
 
-@pytest.mark.parametrize("name", ["hex8p", "hex6x5x4", "tet3p", "tet5p", "cfg0_hex32"])
+@pytest.mark.parametrize("name", ["hex8p", "hex6x5x4", "tet3p", "tet5p", "hex32p"])
 @pytest.mark.parametrize("where", ["host", "device"])
 def test_colored_matches_oracle_and_fixture(name, where):
     say(f"case {name} ({where} arrays)")
@@ -169,9 +169,9 @@ def test_serial_matches_oracle(name):
     assert_close(upts_h, ou, "serial two-array upts")
 
 
-def test_config0_ten_iterations_match_reference_fixture():
-    say("config 0: 32^3 hex, 10 iterations, rhsb restored each iteration")
-    z, mesh, flow = load_case("cfg0_hex32")
+def test_hex32_ten_iterations_match_reference_fixture():
+    say("32^3 permuted hex, 10 iterations, rhsb restored each iteration")
+    z, mesh, flow = load_case("hex32p")
     lcolor = z["lcolor"].astype(np.int32)
     icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
     cv = torch_dev()
@@ -218,7 +218,7 @@ def test_color_contiguous_layout_gives_identical_results():
 def test_declared_layout_fast_path_is_bitwise_identical():
     """ucfd_declare_colored_layout switches whole-colour calls to the
     index-range neighbour test; results must not change by a bit."""
-    for name in ("tet5p", "cfg0_hex32"):
+    for name in ("tet5p", "hex32p"):
         z, mesh, flow = load_case(name)
         lcolor = z["lcolor"].astype(np.int32)
         icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
@@ -363,11 +363,11 @@ def test_c_abi_program_matches_oracle():
     assert_close(take(np.float64, 5 * N).reshape(5, N), su, "C caller serial upts")
 
 
-def test_config1_size_fp64_matches_oracle():
+def test_tet96_5m_cells_fp64_matches_oracle():
     """Full benchmark size (96^3 x 6 tets = 5,308,416 cells), colour-contiguous
     layout, one iteration against the OpenMP oracle."""
     import torch
-    say("config 1 size: generating 5.3M-cell tet mesh")
+    say("96^3x6 tets (5.3M cells): generating 5.3M-cell tet mesh")
     mesh = M.tet_mesh(96, seed=1)
     flow = M.flow_fields(mesh, seed=1)
     N = mesh.neles
@@ -377,7 +377,7 @@ def test_config1_size_fp64_matches_oracle():
                                   lcolor.ctypes.data, None)
     m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
     del mesh, flow
-    say("config 1 size: GPU pass (declared colour-contiguous layout)")
+    say("96^3x6 tets (5.3M cells): GPU pass (declared colour-contiguous layout)")
     cv = torch_dev()
     lc_d, ic_d = cv(lc2), cv(ic2)
     L.call("ucfd_declare_colored_layout", N, ic_d, lc_d, None)
@@ -386,8 +386,8 @@ def test_config1_size_fp64_matches_oracle():
     finally:
         L.call("ucfd_release_colored_layout", ic_d, lc_d)
     torch.cuda.empty_cache()
-    say("config 1 size: oracle pass")
+    say("96^3x6 tets (5.3M cells): oracle pass")
     od, os_, oq, ou = O.colored_pass(m2.nei_ele, ic2, lc2, m2.fnorm_vol, m2.vec_fnorm, f2.fspr,
                                      f2.dt, f2.upts, f2.rhsb)
-    assert_close(dq, oq, "config 1 dQ")
-    assert_close(upts, ou, "config 1 upts")
+    assert_close(dq, oq, "tet96 dQ")
+    assert_close(upts, ou, "tet96 upts")

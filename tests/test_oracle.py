@@ -16,7 +16,7 @@ import pyoracle as O
 from ucfd_sparse_amd import mesh as M
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["hex8p", "hex6x5x4", "tet3p", "tet5p", "cfg0_hex32"]
+CASES = ["hex8p", "hex6x5x4", "tet3p", "tet5p", "hex32p"]
 
 
 def load_case(name):

@@ -402,3 +402,112 @@ GEN_EXPORT int64_t ucfdgen_face_pairs(int64_t N, int nface, const int32_t *nei_e
     }
     return bad;
 }
+
+/* N(0,1) / U[0,1) fills (counter based, index = position) */
+GEN_EXPORT void ucfdgen_normal_fill(uint64_t seed, uint64_t stream, int64_t n, double *out)
+{
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = normal01(seed, stream, (uint64_t)i);
+}
+
+/* number of BSR blocks of the face-neighbour system: one diagonal block per
+ * cell plus one per interior face slot */
+GEN_EXPORT int64_t ucfdgen_bsr_count(int64_t N, int nface, const int32_t *nei_ele)
+{
+    int64_t nnz = N;
+    for (int f = 0; f < nface; ++f)
+        for (int64_t i = 0; i < N; ++i) nnz += nei_ele[f*N + i] != i;
+    return nnz;
+}
+
+/*
+ * Block Jacobian-like system on a cell mesh (block size B = 5 for the Euler
+ * block, B = 1 for a one-equation turbulence block), in the mesh's own cell
+ * numbering, BSR with sorted columns:
+ *   off-diagonal (i, nb):  0.5 fv (s A_f - lam I),  A_f entries N(0, 0.2)
+ *       shared by both sides of the face, s = +1 on the side with the smaller
+ *       original id and -1 on the other, fv = fnorm_vol[f,i], lam = fspr[f,i]
+ *   diagonal i: (1/dt_i + sum_f fv (0.5 lam + nu_f / dist_f)) I + P_i,
+ *       nu_f ~ U[1e-5, 1e-3] per face, dist_f = centroid distance (1/fv on
+ *       boundary faces or without centroids), P_i entries U[-0.1, 0.1] times
+ *       the diagonal value.
+ * Draws are indexed by original ids (face pairs), so renumbering the mesh
+ * permutes the matrix and nothing else.  Outputs: rowptr[N+1], colidx[nnz],
+ * diagslots[N], values[nnz*B*B] (row-major blocks).
+ */
+GEN_EXPORT int ucfdgen_bsr_fill(int64_t N, int nface, int B, uint64_t seed, const int32_t *nei,
+                                const double *fnorm_vol, const double *fspr, const double *dt,
+                                const int32_t *orig_id, const double *centroid, int32_t *rowptr,
+                                int32_t *colidx, int32_t *diagslots, double *values)
+{
+    if (B < 1 || B > 8 || nface > 16) return 1;
+    const int B2 = B*B;
+    rowptr[0] = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        int cnt = 1;
+        for (int f = 0; f < nface; ++f) cnt += nei[f*N + i] != i;
+        rowptr[i + 1] = rowptr[i] + cnt;
+    }
+    const uint64_t STA = (uint64_t)ST_AFACE*16 + (uint64_t)B;
+    const uint64_t STN = (uint64_t)ST_NU*16 + (uint64_t)B;
+    const uint64_t STP = (uint64_t)ST_PERT*16 + (uint64_t)B;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        int32_t cols[17];
+        int32_t fof[17];
+        int n = 0;
+        cols[n] = (int32_t)i; fof[n] = -1; ++n;
+        for (int f = 0; f < nface; ++f) {
+            const int32_t nb = nei[f*N + i];
+            if (nb == i) continue;
+            cols[n] = nb; fof[n] = f; ++n;
+        }
+        /* insertion sort by column */
+        for (int a = 1; a < n; ++a) {
+            int32_t c = cols[a], g = fof[a];
+            int b = a - 1;
+            while (b >= 0 && cols[b] > c) { cols[b + 1] = cols[b]; fof[b + 1] = fof[b]; --b; }
+            cols[b + 1] = c; fof[b + 1] = g;
+        }
+        const int64_t base = rowptr[i];
+        const uint64_t oi = (uint64_t)(orig_id ? orig_id[i] : i);
+        double dval = 1.0/dt[i];
+        for (int a = 0; a < n; ++a) {
+            colidx[base + a] = cols[a];
+            double *blk = values + (base + a)*B2;
+            if (fof[a] < 0) { diagslots[i] = (int32_t)(base + a); continue; }
+            const int f = fof[a];
+            const int64_t nb = cols[a];
+            const uint64_t on = (uint64_t)(orig_id ? orig_id[nb] : nb);
+            const uint64_t lo = oi < on ? oi : on, hi = oi < on ? on : oi;
+            const uint64_t pair = lo*(uint64_t)N + hi;
+            const double s = oi < on ? 1.0 : -1.0;
+            const double fv = fnorm_vol[f*N + i], lam = fspr[f*N + i];
+            for (int r = 0; r < B; ++r)
+                for (int c = 0; c < B; ++c) {
+                    const double af = 0.2*normal01(seed, STA, pair*64 + (uint64_t)(r*B + c));
+                    blk[r*B + c] = 0.5*fv*(s*af - (r == c ? lam : 0.0));
+                }
+            const double nu = 1e-5 + (1e-3 - 1e-5)*u01(seed, STN, pair);
+            double dist = 1.0/fv;
+            if (centroid) {
+                double d2 = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    const double x = centroid[d*N + i] - centroid[d*N + nb];
+                    d2 += x*x;
+                }
+                dist = sqrt(d2);
+            }
+            dval += fv*(0.5*lam + nu/dist);
+        }
+        for (int f = 0; f < nface; ++f)
+            if (nei[f*N + i] == i) dval += fnorm_vol[f*N + i]*0.5*fspr[f*N + i];
+        double *dg = values + (int64_t)diagslots[i]*B2;
+        for (int r = 0; r < B; ++r)
+            for (int c = 0; c < B; ++c) {
+                const double pert = (2.0*u01(seed, STP, oi*64 + (uint64_t)(r*B + c)) - 1.0)*0.1;
+                dg[r*B + c] = (r == c ? dval : 0.0) + pert*dval;
+            }
+    }
+    return 0;
+}

@@ -130,7 +130,7 @@ bool ucfd_is_device_ptr(const void *p)
 }
 
 // Wait for the stream and turn a raised device error word into a status.
-static int sync_and_check(const char *fn)
+int ucfd_sync_and_check(const char *fn)
 {
     State &s = st();
     int rc;
@@ -243,7 +243,7 @@ int UcfdStage::leave(int status)
         if (!rc) rc = r;
     }
     if (any_staged || !s.user_stream) {
-        int r = sync_and_check(fn);
+        int r = ucfd_sync_and_check(fn);
         if (!rc) rc = r;
     } else {
         int r = ucfd_check_hip(hipGetLastError(), fn, "kernel launch");
@@ -271,7 +271,7 @@ UCFD_API int ucfd_synchronize(void)
     int rc = ucfd_ensure_device("ucfd_synchronize");
     if (rc) return rc;
     std::lock_guard<std::recursive_mutex> g(st().mu);
-    return sync_and_check("ucfd_synchronize");
+    return ucfd_sync_and_check("ucfd_synchronize");
 }
 
 UCFD_API int ucfd_set_config(int nvars, int nfvars, int ndims)

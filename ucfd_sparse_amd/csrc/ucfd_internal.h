@@ -40,6 +40,9 @@ UCFD_HIDDEN const UcfdConfig &ucfd_config();
 UCFD_HIDDEN int *ucfd_device_flag();          // device error word (per process)
 UCFD_HIDDEN int ucfd_ensure_device(const char *fn);
 UCFD_HIDDEN bool ucfd_is_device_ptr(const void *p);
+// Wait for the library stream; a raised device error word becomes a status
+// (and is cleared).  Call with the runtime lock held or outside any staging.
+UCFD_HIDDEN int ucfd_sync_and_check(const char *fn);
 
 // Staging of host arrays for one call.  Device (or managed) pointers are used
 // in place; host pointers are copied to a library workspace on entry and

@@ -57,6 +57,37 @@ _SIGS = {
     "ucfd_lusgs_solver_destroy": [c_void_p],
     "ucfd_lusgs_residual": [c_void_p] * 5,
     "ucfd_lusgs_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_void_p],
+    # Krylov API (include/ucfdmat.h, ucfdprecon.h, ucfdsolver.h); handles are
+    # passed by address (SpMat*, Precon*, Solver*) or by value (void*)
+    "UCFDMatInit": [c_void_p],
+    "UCFDMatDestroy": [c_void_p],
+    "UCFDMatMult": [c_double, c_void_p, c_void_p, c_double, c_void_p],
+    "UCFDMatCreateCSR": [c_void_p, c_int, c_void_p, c_void_p, c_void_p],
+    "UCFDMatCreateBSR": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "UCFDMatCreateCUDACSR": [c_void_p, c_int, c_void_p, c_void_p, c_void_p],
+    "UCFDMatCreateCUDABSR": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "UCFDPreconCreatefromArrays": [c_void_p] * 5,
+    "UCFDPreconCreateNone": [c_void_p],
+    "UCFDPreconPrepare": [c_void_p],
+    "UCFDPreconDestroy": [c_void_p],
+    "UCFDPreconSetBLUSGS": [c_void_p, c_int, c_int],
+    "UCFDPreconSetPBLUSGS": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "UCFDPreconSetCUDABLUSGS": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "UCFDPreconSetILU": [c_void_p, c_int],
+    "UCFDPreconSetBILU": [c_void_p, c_int, c_int],
+    "UCFDPreconSetPBILU": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "UCFDPreconApply": [c_void_p, c_void_p],
+    "UCFDPreconSetPrecision": [c_void_p, c_int],
+    "UCFDSolverInit": [c_void_p],
+    "UCFDSolverDestroy": [c_void_p],
+    "UCFDSolve": [c_void_p] * 5,
+    "UCFDSolverGetResult": [c_void_p] * 4,
+    "UCFDSolverTraceResidualHistory": [c_void_p],
+    "UCFDSolverGetResidualHistory": [c_void_p, c_void_p],
+    "UCFDSolverCreateGMRES": [c_void_p, c_int, c_int, c_int, c_double],
+    "UCFDSolverCreateCUDAGMRES": [c_void_p, c_int, c_int, c_int, c_double],
+    "UCFDSolverCreateBICGSTAB": [c_void_p, c_int, c_int, c_double],
+    "UCFDSolverSetRightPrecon": [c_void_p, ctypes.c_bool],
 }
 
 UCFD_SOLVE_FP64, UCFD_SOLVE_MIXED, UCFD_SOLVE_HIST = 0, 1, 128

@@ -48,6 +48,13 @@ def _lib():
         lib.ucfdgen_face_pairs.restype = ctypes.c_int64
         lib.ucfdgen_hilbert_keys.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, vp, vp]
         lib.ucfdgen_hilbert_keys.restype = None
+        lib.ucfdgen_normal_fill.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, vp]
+        lib.ucfdgen_normal_fill.restype = None
+        lib.ucfdgen_bsr_count.argtypes = [ctypes.c_int64, ctypes.c_int, vp]
+        lib.ucfdgen_bsr_count.restype = ctypes.c_int64
+        lib.ucfdgen_bsr_fill.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint64] + [vp] * 10
+        lib.ucfdgen_bsr_fill.restype = ctypes.c_int
         lib.ucfdgen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         lib.ucfdgen_uniform.restype = ctypes.c_double
         lib.ucfdgen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
@@ -185,3 +192,44 @@ def uniform(seed: int, stream: int, idx: np.ndarray) -> np.ndarray:
 def normal(seed: int, stream: int, idx: np.ndarray) -> np.ndarray:
     f = _lib().ucfdgen_normal
     return np.array([f(seed, stream, int(i)) for i in np.ravel(idx)])
+
+
+@dataclass
+class BSR:
+    """Block sparse row matrix, row-major blocks (the layout of
+    src/krylov/matrix/sparsemat.c:57-101)."""
+    bn: int
+    blk: int
+    rowptr: np.ndarray       # int32 [bn+1]
+    colidx: np.ndarray       # int32 [nnz]
+    diagslots: np.ndarray    # int32 [bn] index of the diagonal block of each row
+    values: np.ndarray       # f64 [nnz, blk, blk]
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.bsr_matrix((self.values, self.colidx, self.rowptr),
+                             shape=(self.bn * self.blk, self.bn * self.blk))
+
+
+def block_system(mesh: Mesh, flow: Flow, blk: int = 5, seed: int = 0) -> BSR:
+    """The block Jacobian of the block LU-SGS and Krylov workloads on a mesh
+    (definition in ucfdgen_bsr_fill, csrc/meshgen.c)."""
+    N, F = mesh.neles, mesh.nface
+    lib = _lib()
+    nnz = lib.ucfdgen_bsr_count(N, F, _p(mesh.nei_ele))
+    rowptr = np.empty(N + 1, np.int32)
+    colidx = np.empty(nnz, np.int32)
+    diag = np.empty(N, np.int32)
+    values = np.empty((nnz, blk, blk), np.float64)
+    rc = lib.ucfdgen_bsr_fill(N, F, blk, seed, _p(mesh.nei_ele), _p(mesh.fnorm_vol),
+                              _p(flow.fspr), _p(flow.dt), _p(mesh.orig_id),
+                              _p(mesh.centroid), _p(rowptr), _p(colidx), _p(diag), _p(values))
+    if rc:
+        raise ValueError("ucfdgen_bsr_fill failed")
+    return BSR(N, blk, rowptr, colidx, diag, values)
+
+
+def normal_fill(n: int, seed: int, stream: int = 77) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    _lib().ucfdgen_normal_fill(seed, stream, n, _p(out))
+    return out
