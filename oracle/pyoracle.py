@@ -260,3 +260,120 @@ def bicgstab(A, b, maxiter, tol, relative=False, precon=True, icolors=None):
                                     int(relative), x.ctypes.data, _c(b, np.float64).ctypes.data,
                                     ctypes.byref(st), ctypes.byref(res), hist.ctypes.data)
     return x, it, st.value, res.value, hist[:it + 1]
+
+
+# ------------------------------------------------------ FlowSys block LU-SGS
+def _fslib():
+    L = lib()
+    if not hasattr(L, "_fs_bound"):
+        vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+        L.orc_fs_pack.argtypes = [ci, ci, ci, cd, vp, vp, vp, vp, vp]
+        L.orc_fs_tpack.argtypes = [ci, ci, ci, cd, cd, ci, vp, vp, vp, vp, vp]
+        L.orc_fs_update.argtypes = [ci, ci, vp, vp, vp]
+        L.orc_fs_subres.argtypes = [ci, ci, vp, vp, vp, vp, vp]
+        L.orc_fs_prepare.argtypes = [ci, ci, cl, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_fs_sweep.argtypes = [ci, ci, ci, ci, ci, cl] + [vp] * 10
+        for f in ("orc_fs_pack", "orc_fs_tpack", "orc_fs_update", "orc_fs_subres",
+                  "orc_fs_prepare", "orc_fs_sweep"):
+            getattr(L, f).restype = None
+        L._fs_bound = True
+    return L
+
+
+class FlowSysOracle:
+    """The FlowSys block LU-SGS state (rank rhs / du / dup / diag / tdiag) and
+    its operations, restated in oracle/blusgs_oracle.c."""
+
+    def __init__(self, fs, jmat, tjmat, el, nvars, nfvars):
+        self.L = _fslib()
+        self.fs, self.jmat, self.tjmat, self.el = fs, _c(jmat, np.float64), tjmat, el
+        self.nvars, self.nfvars, self.nt = nvars, nfvars, nvars - nfvars
+        N = fs.nlocal
+        self.rhs = np.zeros(N * nvars)
+        self.du = np.zeros(N * nvars)
+        self.dup = np.zeros(N * nvars)
+        self.diag = np.zeros(N * nfvars * nfvars)
+        self.tdiag = np.zeros(N * self.nt * self.nt) if self.nt else None
+        self.cid = _c(el["cell_ids"], np.int32)
+        self.upts = np.array(el["uptsb"], np.float64)
+        self.resid = np.zeros_like(self.upts)
+        self.g = [_c(fs.rowptr, np.int32), _c(fs.colidx, np.int32), _c(fs.slots, np.int32),
+                  _c(fs.sides, np.int8), _c(fs.face_area, np.float64), _c(fs.rcp_vol, np.float64)]
+
+    def pack(self, a0):
+        e = self.el
+        self.L.orc_fs_pack(len(self.cid), self.nvars, self.nfvars, a0, self.cid.ctypes.data,
+                           _c(e["rhs"], np.float64).ctypes.data, _c(e["dt"], np.float64).ctypes.data,
+                           self.rhs.ctypes.data, self.diag.ctypes.data)
+
+    def tpack(self, factor, a0, kind="sa"):
+        e = self.el
+        self.L.orc_fs_tpack(len(self.cid), self.nvars, self.nfvars, a0, factor,
+                            1 if kind == "sa" else 2, self.cid.ctypes.data,
+                            self.upts.ctypes.data, _c(e["dsrc"], np.float64).ctypes.data,
+                            _c(e["dt"], np.float64).ctypes.data, self.tdiag.ctypes.data)
+
+    def prepare(self, turb=False):
+        rp, ci, sl, sd, fa, rv = self.g
+        nv = self.nt if turb else self.nfvars
+        jm = _c(self.tjmat if turb else self.jmat, np.float64)
+        dg = self.tdiag if turb else self.diag
+        self.L.orc_fs_prepare(self.fs.nlocal, nv, self.fs.nfaces, rp.ctypes.data, sl.ctypes.data,
+                              sd.ctypes.data, fa.ctypes.data, rv.ctypes.data, dg.ctypes.data,
+                              jm.ctypes.data)
+
+    def sweep(self, upper, turb=False):
+        rp, ci, sl, sd, fa, rv = self.g
+        nv = self.nt if turb else self.nfvars
+        var0 = self.nfvars if turb else 0
+        jm = _c(self.tjmat if turb else self.jmat, np.float64)
+        dg = self.tdiag if turb else self.diag
+        self.L.orc_fs_sweep(int(upper), var0, nv, self.fs.nlocal, self.nvars, self.fs.nfaces,
+                            rp.ctypes.data, ci.ctypes.data, sd.ctypes.data, sl.ctypes.data,
+                            fa.ctypes.data, rv.ctypes.data, self.rhs.ctypes.data, dg.ctypes.data,
+                            jm.ctypes.data, self.du.ctypes.data)
+
+    def reset(self):
+        self.du[:] = 0.0
+        self.dup[:] = 0.0
+
+    def subres(self):
+        self.L.orc_fs_subres(len(self.cid), self.nvars, self.cid.ctypes.data,
+                             _c(self.el["vol"], np.float64).ctypes.data, self.du.ctypes.data,
+                             self.dup.ctypes.data, self.resid.ctypes.data)
+
+    def update(self):
+        self.L.orc_fs_update(len(self.cid), self.nvars, self.cid.ctypes.data, self.du.ctypes.data,
+                             self.upts.ctypes.data)
+
+
+def flowsys_run(fs, tjmat, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kind="sa", iterations=2):
+    """The oracle's version of tests/golden/refflowsys.run (same stages)."""
+    O = FlowSysOracle(fs, fs.jmat, tjmat, el, nvars, nfvars)
+    out = {}
+    O.pack(a0)
+    if O.nt:
+        O.tpack(turb_factor, a0, kind)
+        out["tdiag_pack"] = O.tdiag.copy()
+    out["rank_rhs"] = O.rhs.copy()
+    out["diag_pack"] = O.diag.copy()
+    O.prepare()
+    out["diag"] = O.diag.copy()
+    if O.nt:
+        O.prepare(turb=True)
+        out["tdiag"] = O.tdiag.copy()
+    O.reset()
+    for it in range(iterations):
+        O.sweep(False)
+        out[f"du_lower{it}"] = O.du.copy()
+        O.sweep(True)
+        out[f"du_upper{it}"] = O.du.copy()
+        if O.nt:
+            O.sweep(False, turb=True)
+            O.sweep(True, turb=True)
+            out[f"du_rans{it}"] = O.du.copy()
+        O.subres()
+        out[f"resid{it}"] = O.resid.copy()
+    O.update()
+    out["upts"] = O.upts.copy()
+    return out

@@ -61,8 +61,9 @@ def test_legacy_signatures_match_reference_ctypes_bindings():
     assert lib.lusgs_serial_ns_update.argtypes == [ci, vp, vp]
 
 
-def test_c_caller_compiles_and_links():
-    src = os.path.join(REPO, "tests", "abi", "legacy_calls.c")
+@pytest.mark.parametrize("prog", ["legacy_calls", "krylov_calls"])
+def test_c_caller_compiles_and_links(prog):
+    src = os.path.join(REPO, "tests", "abi", prog + ".c")
     libdir = os.path.join(REPO, "ucfd_sparse_amd")
     with tempfile.TemporaryDirectory() as d:
         exe = os.path.join(d, "a.x")
@@ -70,6 +71,35 @@ def test_c_caller_compiles_and_links():
                             "-L", libdir, "-llusgs", "-lm", f"-Wl,-rpath,{libdir}"],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_headers_declare_the_krylov_api():
+    """Every public entry point of src/krylov/include/ucfd{mat,precon,solver}.h
+    that the CPU/CUDA builds offer, under the same names."""
+    names = {n for v in declared_functions().values() for n in v}
+    for n in ("UCFDMatInit", "UCFDMatDestroy", "UCFDMatMult", "UCFDMatCreateCSR",
+              "UCFDMatCreateBSR", "UCFDMatCreateCUDACSR", "UCFDMatCreateCUDABSR",
+              "UCFDPreconCreatefromArrays", "UCFDPreconCreateNone", "UCFDPreconPrepare",
+              "UCFDPreconDestroy", "UCFDPreconSetILU", "UCFDPreconSetBILU",
+              "UCFDPreconSetBLUSGS", "UCFDPreconSetPBILU", "UCFDPreconSetCUDABLUSGS",
+              "UCFDSolverInit", "UCFDSolverDestroy", "UCFDSolve", "UCFDSolverGetResult",
+              "UCFDSolverTraceResidualHistory", "UCFDSolverGetResidualHistory",
+              "UCFDSolverCreateGMRES", "UCFDSolverCreateBICGSTAB", "UCFDSolverCreateCUDAGMRES"):
+        assert n in names, n
+
+
+def test_headers_declare_the_flowsys_api():
+    """Every entry point of src/lusgs/include/ucfdlusgs.h:8-40."""
+    names = {n for v in declared_functions().values() for n in v}
+    for n in ("UCFDFlowSysCreate", "UCFDFlowSysSetElement", "UCFDFlowSysDestroy",
+              "UCFDFlowSysSetLUSGS", "UCFDLUSGS_Pack", "UCFDLUSGS_Update", "UCFDLUSGS_NSPrepare",
+              "UCFDLUSGS_RANSPrepare", "UCFDLUSGS_NSLowerSweep", "UCFDLUSGS_NSUpperSweep",
+              "UCFDLUSGS_RANSLowerSweep", "UCFDLUSGS_RANSUpperSweep", "UCFDFlowSysSetBLUSGS",
+              "UCFDBLUSGS_Pack", "UCFDBLUSGS_KWSST_Pack", "UCFDBLUSGS_SA_Pack",
+              "UCFDBLUSGS_Update", "UCFDBLUSGS_SubResidual", "UCFDBLUSGS_NSPrepare",
+              "UCFDBLUSGS_RANSPrepare", "UCFDBLUSGS_NSLowerSweep", "UCFDBLUSGS_RANSLowerSweep",
+              "UCFDBLUSGS_NSUpperSweep", "UCFDBLUSGS_RANSUpperSweep", "UCFDBLUSGS_Reset"):
+        assert n in names, n
 
 
 def test_device_count_call_is_safe_without_gpu():

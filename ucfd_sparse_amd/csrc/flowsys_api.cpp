@@ -1,0 +1,557 @@
+// FlowSys API with the block LU-SGS method on the GPU
+// (src/lusgs/include/ucfdlusgs.h, src/lusgs/flowsys.c, src/lusgs/blusgs.c).
+//
+// The system keeps the caller's structure, geometry and element arrays (as
+// the reference does, flowsys.c:18-54, 120-140); the working arrays rhs, du,
+// dup, diag and tdiag are library-owned device memory (the reference mallocs
+// them, flowsys.c:91-117).  At the first Prepare the face-slot structure is
+// validated, copied to the device, and the sweep schedule is built: the
+// reference visits cells one at a time and updates du in place, reading every
+// face neighbour (new values for cells visited before, old values for the
+// others); cells are grouped into dependency levels (schedule.h) so that each
+// level is one kernel launch and the result equals the sequential sweep bit
+// for bit.  On a colour-ordered numbering each colour is one level.  Prepare
+// also packs every position's neighbour block and fv next to each other in
+// position order: the sweeps then stream them instead of gathering 25
+// separate face-major arrays through the slot index.  A changed jmat
+// therefore needs Prepare again (the reference's diag already does).
+#include "flowsys_internal.h"
+#include "schedule.h"
+#include "ucfdlusgs.h"
+
+#include <new>
+#include <string.h>
+#include <vector>
+
+namespace {
+
+struct FsElem {
+    int neles = 0, nface = 0;
+    UCFDInt *cell_ids = nullptr;
+    UCFDReal *uptsb = nullptr, *rhs = nullptr, *dt = nullptr, *dsrc = nullptr;
+    UCFDReal *vol = nullptr, *resid_out = nullptr;
+};
+
+struct Packed {               // per-position blocks of one variable group
+    double *jp = nullptr, *fv = nullptr;
+    float *jp32 = nullptr, *fv32 = nullptr;
+    bool ready = false;
+};
+
+enum { FS_NONE = 0, FS_BLUSGS = 2 };
+
+}  // namespace
+
+struct _FlowSys {
+    int nlocal, nvars, nfvars, nturbvars, ndims, nfaces, nelem;
+    UCFDInt *rowptr, *colidx, *slots;
+    UCFDInt8 *sides;
+    UCFDReal *face_area, *face_normal, *rcp_vol;
+    std::vector<FsElem> eles;
+    int method = FS_NONE;
+    UCFDReal *jmat = nullptr, *tjmat = nullptr;
+    // device state
+    double *rhs = nullptr, *du = nullptr, *dup = nullptr, *diag = nullptr, *tdiag = nullptr;
+    float *du32 = nullptr;
+    int *rowptr_d = nullptr, *colidx_d = nullptr;
+    int64_t npos = 0;
+    bool fp32 = false;
+    bool structured = false;
+    Packed ns, rans;
+    UcfdSchedule lower, upper;
+};
+
+namespace {
+
+ucfd_status_t fail(const char *fn, const char *msg, int code = UCFD_ERR_ARG)
+{
+    ucfd_report(code, fn, msg);
+    return UCFD_FAILED;
+}
+
+ucfd_status_t status(int rc) { return rc == UCFD_OK ? UCFD_SUCCESS : UCFD_FAILED; }
+
+int hip(hipError_t e, const char *fn, const char *what) { return ucfd_check_hip(e, fn, what); }
+
+template <typename T> int dalloc(T **p, size_t n, const char *fn)
+{
+    if (*p) return UCFD_OK;
+    return hip(hipMalloc((void **)p, (n ? n : 1)*sizeof(T)), fn, "hipMalloc");
+}
+
+template <typename T> void dfree(T **p)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+}
+
+void free_packed(Packed &P)
+{
+    dfree(&P.jp);
+    dfree(&P.fv);
+    dfree(&P.jp32);
+    dfree(&P.fv32);
+    P.ready = false;
+}
+
+void free_device(FlowSys s)
+{
+    (void)hipStreamSynchronize(ucfd_stream());
+    dfree(&s->rhs);
+    dfree(&s->du);
+    dfree(&s->dup);
+    dfree(&s->diag);
+    dfree(&s->tdiag);
+    dfree(&s->du32);
+    dfree(&s->rowptr_d);
+    dfree(&s->colidx_d);
+    free_packed(s->ns);
+    free_packed(s->rans);
+    ucfd_free_schedule(s->lower);
+    ucfd_free_schedule(s->upper);
+    s->structured = false;
+}
+
+bool is_blusgs(FlowSys s) { return s && s->method == FS_BLUSGS; }
+
+// Validate the face-slot structure, copy it to the device and build the
+// forward (cells ascending) and backward (descending) schedules.
+int build_structure(FlowSys s, const char *fn)
+{
+    if (s->structured) return UCFD_OK;
+    const int n = s->nlocal;
+    std::vector<int> rp, ci, sl;
+    int rc;
+    if (!s->rowptr || !s->colidx || !s->slots || !s->sides || !s->face_area || !s->rcp_vol)
+        return ucfd_report(UCFD_ERR_ARG, fn, "NULL structure or geometry array");
+    if ((rc = ucfd_fetch_ints(s->rowptr, (size_t)n + 1, rp, fn))) return rc;
+    if (rp[0] != 0) return ucfd_report(UCFD_ERR_INDEX, fn, "rowptr[0] must be 0");
+    for (int i = 0; i < n; ++i)
+        if (rp[i + 1] < rp[i]) return ucfd_report(UCFD_ERR_INDEX, fn, "rowptr must be nondecreasing");
+    s->npos = rp[n];
+    if ((rc = ucfd_fetch_ints(s->colidx, (size_t)s->npos, ci, fn))) return rc;
+    if ((rc = ucfd_fetch_ints(s->slots, (size_t)s->npos, sl, fn))) return rc;
+    for (int64_t p = 0; p < s->npos; ++p) {
+        if (ci[p] < 0 || ci[p] >= n)
+            return ucfd_report(UCFD_ERR_INDEX, fn, "a face neighbour lies outside [0, nlocal)");
+        if (sl[p] < 0 || sl[p] >= s->nfaces)
+            return ucfd_report(UCFD_ERR_INDEX, fn, "a face slot lies outside [0, nfaces)");
+    }
+    if ((rc = dalloc(&s->rowptr_d, (size_t)n + 1, fn)) || (rc = dalloc(&s->colidx_d, (size_t)s->npos, fn)))
+        return rc;
+    if ((rc = hip(hipMemcpy(s->rowptr_d, rp.data(), rp.size()*sizeof(int), hipMemcpyHostToDevice),
+                  fn, "hipMemcpy(rowptr)")) ||
+        (rc = hip(hipMemcpy(s->colidx_d, ci.data(), ci.size()*sizeof(int), hipMemcpyHostToDevice),
+                  fn, "hipMemcpy(colidx)")))
+        return rc;
+    std::vector<int> seq(n), jb(n), je(n);
+    for (int i = 0; i < n; ++i) {
+        jb[i] = rp[i];
+        je[i] = rp[i + 1];
+    }
+    std::vector<std::vector<int>> levels;
+    bool unvisited;
+    for (int i = 0; i < n; ++i) seq[i] = i;
+    ucfd_build_levels(n, seq, jb, je, ci, levels, &unvisited);
+    if ((rc = ucfd_finish_schedule(levels, s->lower, fn))) return rc;
+    for (int i = 0; i < n; ++i) seq[i] = n - 1 - i;
+    ucfd_build_levels(n, seq, jb, je, ci, levels, &unvisited);
+    if ((rc = ucfd_finish_schedule(levels, s->upper, fn))) return rc;
+    s->structured = true;
+    return UCFD_OK;
+}
+
+FsElem *elem(FlowSys s, UCFDInt eidx, const char *fn)
+{
+    if (!s || eidx < 0 || eidx >= (int)s->eles.size()) {
+        ucfd_report(UCFD_ERR_ARG, fn, "element index out of range");
+        return nullptr;
+    }
+    FsElem *e = &s->eles[eidx];
+    if (!e->cell_ids) {
+        ucfd_report(UCFD_ERR_STATE, fn, "element arrays not set (UCFDFlowSysSetElement)");
+        return nullptr;
+    }
+    return e;
+}
+
+// NS (turb = false) or turbulence (turb = true) diagonal assembly + packing
+ucfd_status_t prepare(FlowSys s, bool turb, const char *fn)
+{
+    if (!is_blusgs(s)) return fail(fn, "block LU-SGS is not set (UCFDFlowSysSetBLUSGS)", UCFD_ERR_STATE);
+    const int nv = turb ? s->nturbvars : s->nfvars;
+    UCFDReal *jm = turb ? s->tjmat : s->jmat;
+    double *dg = turb ? s->tdiag : s->diag;
+    Packed &P = turb ? s->rans : s->ns;
+    if (nv < 1 || nv > 7) return fail(fn, "block size must lie in 1 .. 7");
+    if (!jm || !dg) return fail(fn, "Jacobian matrix is not set", UCFD_ERR_STATE);
+    int rc = build_structure(s, fn);
+    if (rc) return UCFD_FAILED;
+    const size_t d2 = (size_t)nv*nv;
+    if ((rc = dalloc(&P.jp, (size_t)s->npos*d2, fn)) || (rc = dalloc(&P.fv, (size_t)s->npos, fn)))
+        return UCFD_FAILED;
+    if (s->fp32 && ((rc = dalloc(&P.jp32, (size_t)s->npos*d2, fn)) ||
+                    (rc = dalloc(&P.fv32, (size_t)s->npos, fn))))
+        return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const int irp = sg.add(s->rowptr, sizeof(int)*((size_t)s->nlocal + 1), UCFD_IN);
+    const int isl = sg.add(s->slots, sizeof(int)*(size_t)s->npos, UCFD_IN);
+    const int isd = sg.add(s->sides, (size_t)s->npos, UCFD_IN);
+    const int ifa = sg.add(s->face_area, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
+    const int irv = sg.add(s->rcp_vol, sizeof(double)*(size_t)s->nlocal, UCFD_IN);
+    const int ijm = sg.add(jm, sizeof(double)*2*d2*(size_t)s->nfaces, UCFD_IN);
+    if ((rc = sg.enter())) return UCFD_FAILED;
+    rc = fs_launch_prepare(ucfd_stream(), nv, s->nlocal, s->nfaces, (const int *)sg.dev(irp),
+                           (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
+                           (const double *)sg.dev(ifa), (const double *)sg.dev(irv), dg,
+                           (const double *)sg.dev(ijm), P.jp, P.fv, s->fp32 ? P.jp32 : nullptr,
+                           s->fp32 ? P.fv32 : nullptr, ucfd_device_flag());
+    // the factorisation is complete and checked for zero pivots on return
+    if (!rc) rc = ucfd_sync_and_check(fn);
+    rc = sg.leave(rc);
+    P.ready = rc == UCFD_OK;
+    return status(rc);
+}
+
+ucfd_status_t sweep(FlowSys s, bool turb, bool upper, const char *fn)
+{
+    if (!is_blusgs(s)) return fail(fn, "block LU-SGS is not set (UCFDFlowSysSetBLUSGS)", UCFD_ERR_STATE);
+    const Packed &P = turb ? s->rans : s->ns;
+    if (!P.ready) return fail(fn, "Prepare must be called before the sweeps", UCFD_ERR_STATE);
+    const int nv = turb ? s->nturbvars : s->nfvars, var0 = turb ? s->nfvars : 0;
+    const double *dg = turb ? s->tdiag : s->diag;
+    const bool f32 = s->fp32 && P.jp32;
+    UcfdStage sg(fn);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    const UcfdSchedule &S = upper ? s->upper : s->lower;
+    for (const UcfdLevel &l : S.lv)
+        if ((rc = fs_launch_sweep(ucfd_stream(), nv, f32, l.base, l.cnt, l.ordered ? S.order : nullptr,
+                                  var0, s->nvars, s->rowptr_d, s->colidx_d, P.jp, P.fv, P.jp32,
+                                  P.fv32, s->rhs, dg, s->du, f32 ? s->du32 : nullptr)))
+            break;
+    return status(sg.leave(rc));
+}
+
+}  // namespace
+
+// ================================================================ common
+
+UCFD_API ucfd_status_t UCFDFlowSysCreate(FlowSys *sys, UCFDInt nlocal, UCFDInt nvars,
+                                         UCFDInt nfvars, UCFDInt ndims, UCFDInt nfaces,
+                                         UCFDInt nelem, UCFDInt *rowptr, UCFDInt *colidx,
+                                         UCFDInt *slots, UCFDInt8 *sides, UCFDReal *face_area,
+                                         UCFDReal *face_normal, UCFDReal *rcp_vol)
+{
+    const char *fn = "UCFDFlowSysCreate";
+    if (!sys) return fail(fn, "NULL system handle");
+    if (nlocal < 0 || nvars < 1 || nvars > 16 || nfvars < 1 || nfvars > nvars || nfaces < 0 ||
+        nelem < 0)
+        return fail(fn, "need nlocal >= 0, 1 <= nfvars <= nvars <= 16, nfaces >= 0, nelem >= 0");
+    FlowSys s = new (std::nothrow) _FlowSys();
+    if (!s) return fail(fn, "Flow system allocation failed", UCFD_ERR_ALLOC);
+    s->nlocal = nlocal;
+    s->nvars = nvars;
+    s->nfvars = nfvars;
+    s->nturbvars = nvars - nfvars;
+    s->ndims = ndims;
+    s->nfaces = nfaces;
+    s->nelem = nelem;
+    s->rowptr = rowptr;
+    s->colidx = colidx;
+    s->slots = slots;
+    s->sides = sides;
+    s->face_area = face_area;
+    s->face_normal = face_normal;
+    s->rcp_vol = rcp_vol;
+    s->eles.resize(nelem);
+    *sys = s;
+    return UCFD_SUCCESS;
+}
+
+UCFD_API ucfd_status_t UCFDFlowSysSetElement(FlowSys *sys, UCFDInt eidx, UCFDInt neles,
+                                             UCFDInt nface, UCFDInt *cell_ids, UCFDReal *uptsb,
+                                             UCFDReal *rhs, UCFDReal *dt, UCFDReal *dsrc,
+                                             UCFDReal *vol, UCFDReal *resid_out)
+{
+    const char *fn = "UCFDFlowSysSetElement";
+    if (!sys || !*sys) return fail(fn, "NULL system handle");
+    FlowSys s = *sys;
+    if (eidx < 0 || eidx >= (int)s->eles.size()) return fail(fn, "element index out of range");
+    if (neles < 0) return fail(fn, "need neles >= 0");
+    FsElem &e = s->eles[eidx];
+    e.neles = neles;
+    e.nface = nface;
+    e.cell_ids = cell_ids;
+    e.uptsb = uptsb;
+    e.rhs = rhs;
+    e.dt = dt;
+    e.dsrc = dsrc;
+    e.vol = vol;
+    e.resid_out = resid_out;
+    return UCFD_SUCCESS;
+}
+
+UCFD_API ucfd_status_t UCFDFlowSysDestroy(FlowSys *sys)
+{
+    if (!sys || !*sys) return UCFD_SUCCESS;
+    free_device(*sys);
+    delete *sys;
+    *sys = nullptr;
+    return UCFD_SUCCESS;
+}
+
+// ============================================================ block LU-SGS
+
+UCFD_API ucfd_status_t UCFDFlowSysSetBLUSGS(FlowSys *sys, UCFDReal *jmat, UCFDReal *tjmat)
+{
+    const char *fn = "UCFDFlowSysSetBLUSGS";
+    if (!sys || !*sys) return fail(fn, "NULL system handle");
+    FlowSys s = *sys;
+    int rc = ucfd_ensure_device(fn);
+    if (rc) return UCFD_FAILED;
+    free_device(s);
+    s->method = FS_BLUSGS;
+    s->jmat = jmat;
+    s->tjmat = tjmat;
+    const size_t nv = (size_t)s->nlocal*s->nvars;
+    if ((rc = dalloc(&s->rhs, nv, fn)) || (rc = dalloc(&s->du, nv, fn)) ||
+        (rc = dalloc(&s->dup, nv, fn)) ||
+        (rc = dalloc(&s->diag, (size_t)s->nlocal*s->nfvars*s->nfvars, fn)) ||
+        (s->nturbvars && (rc = dalloc(&s->tdiag, (size_t)s->nlocal*s->nturbvars*s->nturbvars, fn))))
+        return UCFD_FAILED;
+    // the reference's buffers come from malloc; start from zeros here
+    if ((rc = fs_launch_zero(ucfd_stream(), (int64_t)nv, s->du, s->dup, nullptr)) ||
+        (rc = fs_launch_zero(ucfd_stream(), (int64_t)nv, s->rhs, s->rhs, nullptr)) ||
+        (rc = ucfd_sync_and_check(fn)))
+        return UCFD_FAILED;
+    return UCFD_SUCCESS;
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_SetPrecision(FlowSys sys, UCFDInt fp32)
+{
+    const char *fn = "UCFDBLUSGS_SetPrecision";
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    const bool want = fp32 != 0;
+    if (want == sys->fp32) return UCFD_SUCCESS;
+    sys->fp32 = want;
+    sys->ns.ready = sys->rans.ready = false;     // Prepare again to pack FP32 blocks
+    if (!want) {
+        dfree(&sys->ns.jp32);
+        dfree(&sys->ns.fv32);
+        dfree(&sys->rans.jp32);
+        dfree(&sys->rans.fv32);
+        dfree(&sys->du32);
+        return UCFD_SUCCESS;
+    }
+    const size_t nv = (size_t)sys->nlocal*sys->nvars;
+    int rc = dalloc(&sys->du32, nv, fn);
+    if (rc) return UCFD_FAILED;
+    // the FP32 shadow of du starts from the current du
+    std::vector<double> h(nv);
+    if (nv && ((rc = hip(hipMemcpy(h.data(), sys->du, nv*sizeof(double), hipMemcpyDeviceToHost), fn,
+                         "hipMemcpy(du)"))))
+        return UCFD_FAILED;
+    std::vector<float> f(h.begin(), h.end());
+    if (nv && (rc = hip(hipMemcpy(sys->du32, f.data(), nv*sizeof(float), hipMemcpyHostToDevice), fn,
+                        "hipMemcpy(du32)")))
+        return UCFD_FAILED;
+    return UCFD_SUCCESS;
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal a0)
+{
+    const char *fn = "UCFDBLUSGS_Pack";
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    FsElem *e = elem(sys, eidx, fn);
+    if (!e) return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const size_t ne = (size_t)e->neles;
+    const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
+    const int ir = sg.add(e->rhs, sizeof(double)*ne*sys->nvars, UCFD_IN);
+    const int idt = sg.add(e->dt, sizeof(double)*ne, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_pack(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, sys->nfvars, a0,
+                        (const int *)sg.dev(ic), (const double *)sg.dev(ir),
+                        (const double *)sg.dev(idt), sys->rhs, sys->diag, ucfd_device_flag());
+    return status(sg.leave(rc));
+}
+
+static ucfd_status_t tpack(FlowSys sys, UCFDInt eidx, UCFDReal factor, UCFDReal a0, int kind,
+                           const char *fn)
+{
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    if (sys->nturbvars < (kind == 2 ? 2 : 1))
+        return fail(fn, "the system has too few turbulence variables", UCFD_ERR_STATE);
+    FsElem *e = elem(sys, eidx, fn);
+    if (!e) return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const size_t ne = (size_t)e->neles;
+    const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
+    const int iu = sg.add(e->uptsb, sizeof(double)*ne*sys->nvars, UCFD_IN);
+    const int isrc = sg.add(e->dsrc, sizeof(double)*ne*sys->nvars, UCFD_IN);
+    const int idt = sg.add(e->dt, sizeof(double)*ne, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_tpack(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, sys->nfvars, a0, factor,
+                         kind, (const int *)sg.dev(ic), (const double *)sg.dev(iu),
+                         (const double *)sg.dev(isrc), (const double *)sg.dev(idt), sys->tdiag,
+                         ucfd_device_flag());
+    return status(sg.leave(rc));
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_SA_Pack(FlowSys sys, UCFDInt eidx, UCFDReal turb_factor,
+                                          UCFDReal a0)
+{
+    return tpack(sys, eidx, turb_factor, a0, 1, "UCFDBLUSGS_SA_Pack");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_KWSST_Pack(FlowSys sys, UCFDInt eidx, UCFDReal turb_factor,
+                                             UCFDReal a0)
+{
+    return tpack(sys, eidx, turb_factor, a0, 2, "UCFDBLUSGS_KWSST_Pack");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_Update(FlowSys sys, UCFDInt eidx)
+{
+    const char *fn = "UCFDBLUSGS_Update";
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    FsElem *e = elem(sys, eidx, fn);
+    if (!e) return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const size_t ne = (size_t)e->neles;
+    const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
+    const int iu = sg.add(e->uptsb, sizeof(double)*ne*sys->nvars, UCFD_INOUT);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_update(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, (const int *)sg.dev(ic),
+                          sys->du, (double *)sg.dev(iu), ucfd_device_flag());
+    return status(sg.leave(rc));
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_SubResidual(FlowSys sys, UCFDInt eidx)
+{
+    const char *fn = "UCFDBLUSGS_SubResidual";
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    FsElem *e = elem(sys, eidx, fn);
+    if (!e) return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const size_t ne = (size_t)e->neles;
+    const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
+    const int iv = sg.add(e->vol, sizeof(double)*ne, UCFD_IN);
+    const int io = sg.add(e->resid_out, sizeof(double)*ne*sys->nvars, UCFD_OUT);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_subres(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, (const int *)sg.dev(ic),
+                          (const double *)sg.dev(iv), sys->du, sys->dup, (double *)sg.dev(io),
+                          ucfd_device_flag());
+    return status(sg.leave(rc));
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_NSPrepare(FlowSys sys)
+{
+    return prepare(sys, false, "UCFDBLUSGS_NSPrepare");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_RANSPrepare(FlowSys sys)
+{
+    return prepare(sys, true, "UCFDBLUSGS_RANSPrepare");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_NSLowerSweep(FlowSys sys)
+{
+    return sweep(sys, false, false, "UCFDBLUSGS_NSLowerSweep");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_RANSLowerSweep(FlowSys sys)
+{
+    return sweep(sys, true, false, "UCFDBLUSGS_RANSLowerSweep");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_NSUpperSweep(FlowSys sys)
+{
+    return sweep(sys, false, true, "UCFDBLUSGS_NSUpperSweep");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_RANSUpperSweep(FlowSys sys)
+{
+    return sweep(sys, true, true, "UCFDBLUSGS_RANSUpperSweep");
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_Reset(FlowSys sys)
+{
+    const char *fn = "UCFDBLUSGS_Reset";
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    UcfdStage sg(fn);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_zero(ucfd_stream(), (int64_t)sys->nlocal*sys->nvars, sys->du, sys->dup,
+                        sys->fp32 ? sys->du32 : nullptr);
+    return status(sg.leave(rc));
+}
+
+UCFD_API ucfd_status_t UCFDBLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal *out)
+{
+    const char *fn = "UCFDBLUSGS_Export";
+    if (!is_blusgs(sys)) return fail(fn, "block LU-SGS is not set", UCFD_ERR_STATE);
+    const size_t nv = (size_t)sys->nlocal*sys->nvars;
+    const double *src = nullptr;
+    size_t n = 0;
+    switch (which) {
+    case 0: src = sys->rhs; n = nv; break;
+    case 1: src = sys->du; n = nv; break;
+    case 2: src = sys->dup; n = nv; break;
+    case 3: src = sys->diag; n = (size_t)sys->nlocal*sys->nfvars*sys->nfvars; break;
+    case 4: src = sys->tdiag; n = (size_t)sys->nlocal*sys->nturbvars*sys->nturbvars; break;
+    default: return fail(fn, "which must be 0 (rhs), 1 (du), 2 (dup), 3 (diag) or 4 (tdiag)");
+    }
+    if (!src) return fail(fn, "the array is not allocated", UCFD_ERR_STATE);
+    UcfdStage sg(fn);
+    const int io = sg.add(out, sizeof(double)*n, UCFD_OUT);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = hip(hipMemcpyAsync(sg.dev(io), src, sizeof(double)*n, hipMemcpyDeviceToDevice,
+                            ucfd_stream()), fn, "hipMemcpyAsync(export)");
+    return status(sg.leave(rc));
+}
+
+// ======================================= scalar LU-SGS methods of the FlowSys
+
+#define UCFD_NOT_PROVIDED(name)                                                                  \
+    return fail(name, "the FlowSys scalar LU-SGS method is not part of this library; use the "  \
+                      "legacy array API (coloredlusgs.h / lusgs.h)", UCFD_ERR_STATE)
+
+UCFD_API ucfd_status_t UCFDFlowSysSetLUSGS(FlowSys *, UCFDReal *, UCFDReal *)
+{
+    UCFD_NOT_PROVIDED("UCFDFlowSysSetLUSGS");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_Pack(FlowSys, UCFDInt, UCFDReal, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_Pack");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_Update(FlowSys, UCFDInt) { UCFD_NOT_PROVIDED("UCFDLUSGS_Update"); }
+UCFD_API ucfd_status_t UCFDLUSGS_NSPrepare(FlowSys, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_NSPrepare");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_RANSPrepare(FlowSys, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_RANSPrepare");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_NSLowerSweep(FlowSys, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_NSLowerSweep");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_NSUpperSweep(FlowSys, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_NSUpperSweep");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_RANSLowerSweep(FlowSys, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_RANSLowerSweep");
+}
+UCFD_API ucfd_status_t UCFDLUSGS_RANSUpperSweep(FlowSys, UCFDReal)
+{
+    UCFD_NOT_PROVIDED("UCFDLUSGS_RANSUpperSweep");
+}

@@ -1,0 +1,42 @@
+// Internal interface between the FlowSys host code (flowsys_api.cpp) and its
+// kernels (flowsys_kernels.hip).  Not installed.
+#pragma once
+
+#include <stdint.h>
+
+#include "ucfd_internal.h"
+
+// Element-level kernels; cell_ids entries outside [0, nlocal) raise
+// UCFD_KFLAG_INDEX and are skipped.
+UCFD_HIDDEN int fs_launch_pack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
+                               double a0, const int *cell_ids, const double *rhs, const double *dt,
+                               double *rank_rhs, double *diag, int *flag);
+// kind 1 = Spalart-Allmaras, 2 = k-omega SST source Jacobian
+UCFD_HIDDEN int fs_launch_tpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
+                                double a0, double factor, int kind, const int *cell_ids,
+                                const double *uptsb, const double *dsrc, const double *dt,
+                                double *tdiag, int *flag);
+UCFD_HIDDEN int fs_launch_update(hipStream_t s, int neles, int nlocal, int nvars,
+                                 const int *cell_ids, const double *du, double *upts, int *flag);
+UCFD_HIDDEN int fs_launch_subres(hipStream_t s, int neles, int nlocal, int nvars,
+                                 const int *cell_ids, const double *vol, const double *du,
+                                 double *dup, double *res, int *flag);
+// diag += sum over positions of the signed face Jacobian times fv, then LU;
+// also writes every position's neighbour block jp[p][nv][nv] and fv (and
+// their FP32 copies when jp32 != NULL).  A zero pivot raises UCFD_KFLAG_PIVOT.
+UCFD_HIDDEN int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces,
+                                  const int *rowptr, const int *slots, const signed char *sides,
+                                  const double *face_area, const double *rcp_vol, double *diag,
+                                  const double *jmat, double *jp, double *fvp, float *jp32,
+                                  float *fv32, int *flag);
+// One level of a block Gauss-Seidel sweep: rows base..base+cnt-1 (or
+// order[base..base+cnt)), variables var0..var0+nv-1 of the [nlocal, nvars]
+// arrays.  f32: FP32 products on jp32 / fv32 / du32, FP64 diagonal solve,
+// du and du32 both written.
+UCFD_HIDDEN int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int cnt,
+                                const int *order, int var0, int nvars, const int *rowptr,
+                                const int *colidx, const double *jp, const double *fvp,
+                                const float *jp32, const float *fv32, const double *rhs,
+                                const double *diag, double *du, float *du32);
+// a = b = 0 (and c = 0 when not NULL), n entries
+UCFD_HIDDEN int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c);

@@ -15,6 +15,7 @@
 // two-stage device reductions in a fixed order, so results are reproducible
 // but not bitwise equal to a sequential CPU sum.
 #include "krylov_internal.h"
+#include "schedule.h"
 #include "ucfdmat.h"
 #include "ucfdprecon.h"
 #include "ucfdsolver.h"
@@ -37,20 +38,6 @@ struct _SpMat {
     int64_t nnz;    // stored blocks; -1 until read from rowptr
 };
 
-namespace {
-
-struct Level {
-    int base, cnt;
-    bool ordered;   // rows are order[base .. base+cnt) instead of base .. base+cnt-1
-};
-
-struct Schedule {
-    std::vector<Level> lv;
-    int *order = nullptr;   // device
-};
-
-}  // namespace
-
 struct _Precon {
     PreconType type_name;
     UCFDInt *rowptr, *colidx, *diagslots;
@@ -65,7 +52,7 @@ struct _Precon {
     float *vals32;           // device, nnz*blk*blk (fp32 only)
     float *x32;              // device, bn*blk (fp32 only)
     bool init32;             // the lower pass reads rows it has not yet produced
-    Schedule lower, upper;
+    UcfdSchedule lower, upper;
 };
 
 namespace {
@@ -104,24 +91,12 @@ ucfd_status_t status(int rc) { return rc == UCFD_OK ? UCFD_SUCCESS : UCFD_FAILED
 
 int hip(hipError_t e, const char *fn, const char *what) { return ucfd_check_hip(e, fn, what); }
 
-// Copy n ints of a host or device array to host memory.
-int fetch_ints(const int *p, size_t n, std::vector<int> &out, const char *fn)
-{
-    out.resize(n);
-    if (n == 0) return UCFD_OK;
-    if (ucfd_is_device_ptr(p))
-        return hip(hipMemcpy(out.data(), p, n*sizeof(int), hipMemcpyDeviceToHost), fn,
-                   "hipMemcpy(structure)");
-    memcpy(out.data(), p, n*sizeof(int));
-    return UCFD_OK;
-}
-
 // Validate a row pointer (0 at the start, nondecreasing) and return its last entry.
 int read_nnz(const int *rowptr, int rows, int64_t *nnz, const char *fn)
 {
     if (!rowptr) return ucfd_report(UCFD_ERR_ARG, fn, "NULL rowptr");
     std::vector<int> rp;
-    int rc = fetch_ints(rowptr, (size_t)rows + 1, rp, fn);
+    int rc = ucfd_fetch_ints(rowptr, (size_t)rows + 1, rp, fn);
     if (rc) return rc;
     if (rp[0] != 0) return ucfd_report(UCFD_ERR_INDEX, fn, "rowptr[0] must be 0");
     for (int i = 0; i < rows; ++i)
@@ -202,12 +177,12 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
     if (!pc_is_block(pc)) return UCFD_OK;
     int rc = UCFD_OK;
     if (!pc->fp32) {
-        for (const Level &l : pc->lower.lv)
+        for (const UcfdLevel &l : pc->lower.lv)
             if ((rc = kry_bsgs_sweep64(s, 0, pc->blk, l.base, l.cnt,
                                        l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
                                        d.diagslots, d.values, pc->diagvalues, b)))
                 return rc;
-        for (const Level &l : pc->upper.lv)
+        for (const UcfdLevel &l : pc->upper.lv)
             if ((rc = kry_bsgs_sweep64(s, 1, pc->blk, l.base, l.cnt,
                                        l.ordered ? pc->upper.order : nullptr, d.rowptr, d.colidx,
                                        d.diagslots, d.values, pc->diagvalues, b)))
@@ -215,12 +190,12 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
         return UCFD_OK;
     }
     if (pc->init32 && (rc = kry_cast32(s, (int64_t)pc->bn*pc->blk, b, pc->x32))) return rc;
-    for (const Level &l : pc->lower.lv)
+    for (const UcfdLevel &l : pc->lower.lv)
         if ((rc = kry_bsgs_sweep32(s, 0, pc->blk, l.base, l.cnt,
                                    l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
                                    d.diagslots, pc->vals32, pc->diagvalues, pc->x32, b)))
             return rc;
-    for (const Level &l : pc->upper.lv)
+    for (const UcfdLevel &l : pc->upper.lv)
         if ((rc = kry_bsgs_sweep32(s, 1, pc->blk, l.base, l.cnt,
                                    l.ordered ? pc->upper.order : nullptr, d.rowptr, d.colidx,
                                    d.diagslots, pc->vals32, pc->diagvalues, pc->x32, b)))
@@ -230,86 +205,21 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
 
 // ------------------------------------------------------------ schedules
 //
-// The reference visits rows one at a time (blusgs.c:43-99: ascending, then
-// descending; pblusgs.c:90-104 at one thread: colour ranges ascending with
-// rows ascending, then colour ranges descending with rows ascending) and
-// updates b in place.  A row reads b of the columns in its lower (or upper)
-// slot range: values already produced for rows visited earlier, original
-// values for rows visited later.  Rows are grouped into levels so that every
-// row runs after the rows whose new values it reads and before the rows whose
-// old values it reads; rows of one level never read each other.  The result
-// equals the sequential visit exactly.  For a colour-ordered matrix whose
-// colour ranges hold no internal couplings (the intended use of the coloured
-// form), every colour range is one level.
-
-struct Visit {
-    std::vector<int> seq;   // rows in visit order
-};
-
-void build_levels(const std::vector<int> &rp, const std::vector<int> &ci,
-                  const std::vector<int> &ds, bool upper, const std::vector<int> &seq, int bn,
-                  std::vector<std::vector<int>> &levels, bool *reads_unvisited)
-{
-    std::vector<int> pos(bn, -1), lev(bn, 0), need(bn, 0);
-    for (size_t k = 0; k < seq.size(); ++k) pos[seq[k]] = (int)k;
-    *reads_unvisited = false;
-    int maxlev = 0;
-    for (size_t k = 0; k < seq.size(); ++k) {
-        const int i = seq[k];
-        const int j0 = upper ? ds[i] + 1 : rp[i], j1 = upper ? rp[i + 1] : ds[i];
-        int l = need[i];
-        for (int j = j0; j < j1; ++j) {
-            const int c = ci[j];
-            if (c == i) {
-                *reads_unvisited = true;
-                continue;
-            }
-            if (pos[c] >= 0 && pos[c] < (int)k) l = std::max(l, lev[c]);
-            else *reads_unvisited = true;
-        }
-        lev[i] = l + 1;
-        maxlev = std::max(maxlev, lev[i]);
-        for (int j = j0; j < j1; ++j) {
-            const int c = ci[j];
-            if (c != i && pos[c] > (int)k) need[c] = std::max(need[c], lev[i]);
-        }
-    }
-    levels.assign(maxlev, {});
-    for (int i : seq) levels[lev[i] - 1].push_back(i);
-    for (auto &v : levels) std::sort(v.begin(), v.end());
-}
-
-int finish_schedule(const std::vector<std::vector<int>> &levels, Schedule &S, const char *fn)
-{
-    std::vector<int> ord;
-    S.lv.clear();
-    for (const auto &v : levels) {
-        if (v.empty()) continue;
-        const bool contig = v.back() - v.front() + 1 == (int)v.size();
-        if (contig) {
-            S.lv.push_back(Level{v.front(), (int)v.size(), false});
-        } else {
-            S.lv.push_back(Level{(int)ord.size(), (int)v.size(), true});
-            ord.insert(ord.end(), v.begin(), v.end());
-        }
-    }
-    if (S.order) (void)hipFree(S.order);
-    S.order = nullptr;
-    if (ord.empty()) return UCFD_OK;
-    int rc = hip(hipMalloc((void **)&S.order, ord.size()*sizeof(int)), fn, "hipMalloc(schedule)");
-    if (rc) return rc;
-    return hip(hipMemcpy(S.order, ord.data(), ord.size()*sizeof(int), hipMemcpyHostToDevice), fn,
-               "hipMemcpy(schedule)");
-}
+// Visit orders of the reference (see schedule.h for the levels):
+// blusgs.c:43-99 rows ascending, then descending; pblusgs.c:90-104 at one
+// thread: colour ranges ascending with rows ascending, then colour ranges
+// descending with rows ascending.  A colour-ordered matrix whose ranges hold
+// no internal couplings (the intended use of the coloured form) gets one
+// launch per colour range without building levels.
 
 int make_schedules(Precon pc, const char *fn)
 {
     const int bn = pc->bn;
     std::vector<int> rp, ci, ds;
     int rc;
-    if ((rc = fetch_ints(pc->rowptr, (size_t)bn + 1, rp, fn))) return rc;
-    if ((rc = fetch_ints(pc->colidx, (size_t)pc->nnz, ci, fn))) return rc;
-    if ((rc = fetch_ints(pc->diagslots, (size_t)bn, ds, fn))) return rc;
+    if ((rc = ucfd_fetch_ints(pc->rowptr, (size_t)bn + 1, rp, fn))) return rc;
+    if ((rc = ucfd_fetch_ints(pc->colidx, (size_t)pc->nnz, ci, fn))) return rc;
+    if ((rc = ucfd_fetch_ints(pc->diagslots, (size_t)bn, ds, fn))) return rc;
     for (int64_t j = 0; j < pc->nnz; ++j)
         if (ci[j] < 0 || ci[j] >= bn)
             return ucfd_report(UCFD_ERR_INDEX, fn, "colidx holds an entry outside [0, bn)");
@@ -322,7 +232,7 @@ int make_schedules(Precon pc, const char *fn)
     const bool colored = strcmp(pc->type_name, PBLUSGS) == 0;
     if (colored) {
         if (pc->ncolors < 0 || !pc->icolors) return ucfd_report(UCFD_ERR_ARG, fn, "bad colour offsets");
-        if ((rc = fetch_ints(pc->icolors, (size_t)pc->ncolors + 1, off, fn))) return rc;
+        if ((rc = ucfd_fetch_ints(pc->icolors, (size_t)pc->ncolors + 1, off, fn))) return rc;
         for (int c = 0; c < pc->ncolors; ++c)
             if (off[c] < 0 || off[c] > off[c + 1] || off[c + 1] > bn)
                 return ucfd_report(UCFD_ERR_INDEX, fn, "icolors must be nondecreasing offsets in [0, bn]");
@@ -349,29 +259,31 @@ int make_schedules(Precon pc, const char *fn)
                 }
     }
     if (fast) {
-        pc->lower.lv.clear();
-        pc->upper.lv.clear();
+        ucfd_free_schedule(pc->lower);
+        ucfd_free_schedule(pc->upper);
         for (int c = 0; c < pc->ncolors; ++c)
-            if (off[c + 1] > off[c]) pc->lower.lv.push_back(Level{off[c], off[c + 1] - off[c], false});
+            if (off[c + 1] > off[c]) pc->lower.lv.push_back(UcfdLevel{off[c], off[c + 1] - off[c], false});
         for (int c = pc->ncolors - 1; c >= 0; --c)
-            if (off[c + 1] > off[c]) pc->upper.lv.push_back(Level{off[c], off[c + 1] - off[c], false});
+            if (off[c + 1] > off[c]) pc->upper.lv.push_back(UcfdLevel{off[c], off[c + 1] - off[c], false});
         pc->init32 = unvis_l;
         return UCFD_OK;
     }
     std::vector<std::vector<int>> levels;
-    build_levels(rp, ci, ds, false, lseq, bn, levels, &unvis_l);
-    if ((rc = finish_schedule(levels, pc->lower, fn))) return rc;
-    build_levels(rp, ci, ds, true, useq, bn, levels, &unvis_u);
-    if ((rc = finish_schedule(levels, pc->upper, fn))) return rc;
+    std::vector<int> jb(bn), je(bn);
+    for (int i = 0; i < bn; ++i) {
+        jb[i] = rp[i];
+        je[i] = ds[i];
+    }
+    ucfd_build_levels(bn, lseq, jb, je, ci, levels, &unvis_l);
+    if ((rc = ucfd_finish_schedule(levels, pc->lower, fn))) return rc;
+    for (int i = 0; i < bn; ++i) {
+        jb[i] = ds[i] + 1;
+        je[i] = rp[i + 1];
+    }
+    ucfd_build_levels(bn, useq, jb, je, ci, levels, &unvis_u);
+    if ((rc = ucfd_finish_schedule(levels, pc->upper, fn))) return rc;
     pc->init32 = unvis_l;
     return UCFD_OK;
-}
-
-void free_schedule(Schedule &S)
-{
-    if (S.order) (void)hipFree(S.order);
-    S.order = nullptr;
-    S.lv.clear();
 }
 
 int pc_alloc(Precon pc, const char *fn)
@@ -395,8 +307,8 @@ void pc_free_device(Precon pc)
     pc->diagvalues = nullptr;
     pc->vals32 = nullptr;
     pc->x32 = nullptr;
-    free_schedule(pc->lower);
-    free_schedule(pc->upper);
+    ucfd_free_schedule(pc->lower);
+    ucfd_free_schedule(pc->upper);
     pc->prepared = false;
 }
 

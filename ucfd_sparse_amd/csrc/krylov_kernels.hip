@@ -10,6 +10,7 @@
 // every dot product is reproducible run to run.
 #include <hip/hip_runtime.h>
 
+#include "block_lu.h"
 #include "krylov_internal.h"
 
 namespace {
@@ -90,63 +91,6 @@ __global__ __launch_bounds__(KB) void k_spmv_csr(int n, const int *__restrict__ 
     if (bad) atomicOr(flag, UCFD_KFLAG_INDEX);
 }
 
-// ---------------------------------------------------- LU of 5x5 blocks etc.
-template <int B>
-__device__ __forceinline__ bool ludcmp(double *A)
-{
-    if (B == 1) {
-        if (A[0] == 0.0) return false;
-        A[0] = 1.0/A[0];
-        return true;
-    }
-    if (A[0] == 0.0) return false;
-#pragma unroll
-    for (int row = 1; row < B; ++row) {
-        const int nrow = B*row;
-        A[nrow] /= A[0];
-#pragma unroll
-        for (int col = 1; col < B; ++col) {
-            if (row > col) {
-                double val = 0.0;
-#pragma unroll
-                for (int k = 0; k < col; ++k) val += A[nrow + k]*A[col + B*k];
-                A[nrow + col] = (A[nrow + col] - val)/A[(B + 1)*col];
-            } else {
-                double val = 0.0;
-#pragma unroll
-                for (int k = 0; k < row; ++k) val += A[nrow + k]*A[B*k + col];
-                A[nrow + col] -= val;
-            }
-        }
-        if (A[(B + 1)*row] == 0.0) return false;
-    }
-    return true;
-}
-
-template <int B>
-__device__ __forceinline__ void lusub(const double *LU, double *b)
-{
-    if (B == 1) {
-        b[0] *= LU[0];
-        return;
-    }
-#pragma unroll
-    for (int row = 1; row < B; ++row) {
-        double val = 0.0;
-#pragma unroll
-        for (int col = 0; col < row; ++col) val += LU[row*B + col]*b[col];
-        b[row] -= val;
-    }
-    b[B - 1] /= LU[B*B - 1];
-#pragma unroll
-    for (int row = B - 2; row > -1; --row) {
-        double val = 0.0;
-#pragma unroll
-        for (int col = row + 1; col < B; ++col) val += LU[row*B + col]*b[col];
-        b[row] = (b[row] - val)/LU[row*B + row];
-    }
-}
-
 template <int B>
 __global__ __launch_bounds__(KB) void k_prepare(int bn, const int *__restrict__ diagslots,
                                                 const double *__restrict__ values,
@@ -158,7 +102,7 @@ __global__ __launch_bounds__(KB) void k_prepare(int bn, const int *__restrict__ 
     const double *src = values + (int64_t)diagslots[i]*B*B;
 #pragma unroll
     for (int q = 0; q < B*B; ++q) a[q] = src[q];
-    if (!ludcmp<B>(a)) atomicOr(flag, UCFD_KFLAG_PIVOT);
+    if (!ucfd::ludcmp<B>(a)) atomicOr(flag, UCFD_KFLAG_PIVOT);
 #pragma unroll
     for (int q = 0; q < B*B; ++q) dv[(int64_t)i*B*B + q] = a[q];
 }
@@ -210,7 +154,7 @@ __global__ __launch_bounds__(KB) void k_sweep64(int base, int cnt, const int *__
     double lu[B*B];
 #pragma unroll
     for (int q = 0; q < B*B; ++q) lu[q] = __builtin_nontemporal_load(dv + (int64_t)i*B*B + q);
-    lusub<B>(lu, arr);
+    ucfd::lusub<B>(lu, arr);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t o = (int64_t)i*B + k;
@@ -257,13 +201,13 @@ __global__ __launch_bounds__(KB) void k_sweep32(int base, int cnt, const int *__
     if (!UPPER) {
 #pragma unroll
         for (int k = 0; k < B; ++k) arr[k] = b[(int64_t)i*B + k] - (double)acc[k];
-        lusub<B>(lu, arr);
+        ucfd::lusub<B>(lu, arr);
 #pragma unroll
         for (int k = 0; k < B; ++k) x32[(int64_t)i*B + k] = (float)arr[k];
     } else {
 #pragma unroll
         for (int k = 0; k < B; ++k) arr[k] = (double)acc[k];
-        lusub<B>(lu, arr);
+        ucfd::lusub<B>(lu, arr);
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             const int64_t o = (int64_t)i*B + k;

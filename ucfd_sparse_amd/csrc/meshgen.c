@@ -511,3 +511,109 @@ GEN_EXPORT int ucfdgen_bsr_fill(int64_t N, int nface, int B, uint64_t seed, cons
     }
     return 0;
 }
+
+/*
+ * Face-slot CSR system for the FlowSys block LU-SGS API
+ * (src/lusgs/flowsys.h:34-59): one slot per interior face, numbered in the
+ * order first met scanning cells, then faces, of the given numbering; each
+ * cell lists its interior faces (face order) as positions with neighbour,
+ * slot and side (+1 on the side with the smaller original id, -1 on the
+ * other).  Boundary faces carry no position.  Per slot: face_area (the
+ * owner side's), face_normal [ndims, nslots] (owner's outward normal) and
+ *   jmat[0][r][c][slot] = A+ = 0.5 (A_f + lam I) + (nu/dist) I
+ *   jmat[1][r][c][slot] = A- = 0.5 (A_f - lam I) - (nu/dist) I
+ * with A_f, nu, dist drawn and defined as in ucfdgen_bsr_fill (lam = the
+ * owner's fspr), so the block system matches that generator's off-diagonal
+ * blocks.  Pass 1 (values == 0): counts only, *nslots and *npos.
+ */
+GEN_EXPORT int ucfdgen_flowsys_fill(int64_t N, int nface, int ndims, int nv, uint64_t seed,
+                                    const int32_t *nei, const double *face_area_cf,
+                                    const double *vec_fnorm, const double *fspr,
+                                    const int32_t *orig_id, const double *centroid,
+                                    const double *fnorm_vol, int64_t *nslots, int64_t *npos,
+                                    int32_t *rowptr, int32_t *colidx, int32_t *slots,
+                                    int8_t *sides, double *face_area, double *face_normal,
+                                    double *jmat, int values)
+{
+    if (nv < 1 || nv > 8) return 1;
+    int64_t ns = 0, np = 0;
+    for (int64_t i = 0; i < N; ++i)
+        for (int f = 0; f < nface; ++f) {
+            const int64_t nb = nei[f*N + i];
+            if (nb == i) continue;
+            ++np;
+            if (i < nb) ++ns;
+        }
+    *nslots = ns;
+    *npos = np;
+    if (!values) return 0;
+    /* slot of the face (i, f) with i < nb: running count; the partner cell
+     * finds it through its back face */
+    int32_t *slot_of = (int32_t *)malloc(sizeof(int32_t)*(size_t)N*nface);
+    if (!slot_of) return 2;
+    int64_t s = 0;
+    for (int64_t i = 0; i < N; ++i)
+        for (int f = 0; f < nface; ++f) {
+            const int64_t nb = nei[f*N + i];
+            slot_of[f*N + i] = (nb != i && i < nb) ? (int32_t)s++ : -1;
+        }
+    int64_t p = 0;
+    rowptr[0] = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        for (int f = 0; f < nface; ++f) {
+            const int64_t nb = nei[f*N + i];
+            if (nb == i) continue;
+            int32_t sl = slot_of[f*N + i];
+            if (sl < 0)
+                for (int g = 0; g < nface; ++g)
+                    if (nei[g*N + nb] == i) { sl = slot_of[g*N + nb]; break; }
+            if (sl < 0) { free(slot_of); return 3; }
+            const uint64_t oi = (uint64_t)(orig_id ? orig_id[i] : i);
+            const uint64_t on = (uint64_t)(orig_id ? orig_id[nb] : nb);
+            colidx[p] = (int32_t)nb;
+            slots[p] = sl;
+            sides[p] = oi < on ? 1 : -1;
+            ++p;
+        }
+        rowptr[i + 1] = (int32_t)p;
+    }
+    const int nv2 = nv*nv;
+    const uint64_t STA = (uint64_t)ST_AFACE*16 + (uint64_t)nv;
+    const uint64_t STN = (uint64_t)ST_NU*16 + (uint64_t)nv;
+    for (int64_t i = 0; i < N; ++i)
+        for (int f = 0; f < nface; ++f) {
+            const int64_t nb = nei[f*N + i];
+            if (nb == i) continue;
+            const uint64_t oi = (uint64_t)(orig_id ? orig_id[i] : i);
+            const uint64_t on = (uint64_t)(orig_id ? orig_id[nb] : nb);
+            if (oi > on) continue;                 /* the owner writes the slot */
+            int32_t sl = slot_of[f*N + i];
+            if (sl < 0)
+                for (int g = 0; g < nface; ++g)
+                    if (nei[g*N + nb] == i) { sl = slot_of[g*N + nb]; break; }
+            const uint64_t pair = oi*(uint64_t)N + on;
+            face_area[sl] = face_area_cf ? face_area_cf[f*N + i] : 1.0;
+            for (int d = 0; d < ndims; ++d)
+                face_normal[(int64_t)d*ns + sl] = vec_fnorm[((int64_t)f*ndims + d)*N + i];
+            const double lam = fspr[f*N + i];
+            const double nu = 1e-5 + (1e-3 - 1e-5)*u01(seed, STN, pair);
+            double dist = 1.0/fnorm_vol[f*N + i];
+            if (centroid) {
+                double d2 = 0.0;
+                for (int d = 0; d < ndims; ++d) {
+                    const double x = centroid[d*N + i] - centroid[d*N + nb];
+                    d2 += x*x;
+                }
+                dist = sqrt(d2);
+            }
+            for (int r = 0; r < nv; ++r)
+                for (int c = 0; c < nv; ++c) {
+                    const double af = 0.2*normal01(seed, STA, pair*64 + (uint64_t)(r*nv + c));
+                    const double dg = r == c ? 1.0 : 0.0;
+                    jmat[((int64_t)(0*nv2) + r*nv + c)*ns + sl] = 0.5*(af + lam*dg) + nu/dist*dg;
+                    jmat[((int64_t)(1*nv2) + r*nv + c)*ns + sl] = 0.5*(af - lam*dg) - nu/dist*dg;
+                }
+        }
+    free(slot_of);
+    return 0;
+}

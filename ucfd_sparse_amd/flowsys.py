@@ -1,0 +1,129 @@
+"""Python handle of the FlowSys block LU-SGS API of liblusgs.so
+(include/ucfdlusgs.h; reference src/lusgs/flowsys.c, blusgs.c).
+
+The handle keeps references to every array it was given, because the
+library keeps the pointers (flowsys.c:18-54, 120-140).  Arrays may be numpy
+(host; copied to the GPU for each call) or torch CUDA tensors (device; used
+in place).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import lib as L
+
+RHS, DU, DUP, DIAG, TDIAG = 0, 1, 2, 3, 4
+
+
+def _conv(a, dtype):
+    if hasattr(a, "data_ptr"):
+        return a
+    return np.ascontiguousarray(a, dtype)
+
+
+class FlowSys:
+    def __init__(self, fs, nvars, nfvars, ndims=3, nelem=1, jmat=None, tjmat=None, fp32=False):
+        """fs: ucfd_sparse_amd.mesh.FlowSysMesh (or any object with the same
+        fields, numpy or torch)."""
+        self._lib = L.load()
+        self.keep = {}
+        k = self.keep
+        k["rowptr"], k["colidx"] = _conv(fs.rowptr, np.int32), _conv(fs.colidx, np.int32)
+        k["slots"], k["sides"] = _conv(fs.slots, np.int32), _conv(fs.sides, np.int8)
+        k["face_area"] = _conv(fs.face_area, np.float64)
+        k["face_normal"] = _conv(fs.face_normal, np.float64)
+        k["rcp_vol"] = _conv(fs.rcp_vol, np.float64)
+        self.nlocal, self.nvars, self.nfvars = int(fs.nlocal), int(nvars), int(nfvars)
+        h = ctypes.c_void_p()
+        L.check(self._lib.UCFDFlowSysCreate(
+            ctypes.byref(h), self.nlocal, self.nvars, self.nfvars, int(ndims), int(fs.nfaces),
+            int(nelem), L.ptr(k["rowptr"]), L.ptr(k["colidx"]), L.ptr(k["slots"]),
+            L.ptr(k["sides"]), L.ptr(k["face_area"]), L.ptr(k["face_normal"]),
+            L.ptr(k["rcp_vol"])), "UCFDFlowSysCreate")
+        self.handle = h
+        k["jmat"] = _conv(fs.jmat if jmat is None else jmat, np.float64)
+        k["tjmat"] = None if tjmat is None else _conv(tjmat, np.float64)
+        L.check(self._lib.UCFDFlowSysSetBLUSGS(ctypes.byref(h), L.ptr(k["jmat"]),
+                                               L.ptr(k["tjmat"])), "UCFDFlowSysSetBLUSGS")
+        if fp32:
+            L.check(self._lib.UCFDBLUSGS_SetPrecision(h, 1), "UCFDBLUSGS_SetPrecision")
+
+    def set_element(self, eidx, el, nface=4):
+        k = self.keep
+        names = ("cell_ids", "uptsb", "rhs", "dt", "dsrc", "vol", "resid_out")
+        for n in names:
+            if n == "resid_out" and n not in el:
+                k[(eidx, n)] = np.zeros_like(np.asarray(el["uptsb"])) \
+                    if not hasattr(el["uptsb"], "data_ptr") else el["uptsb"].new_zeros(
+                        el["uptsb"].shape)
+                continue
+            k[(eidx, n)] = _conv(el[n], np.int32 if n == "cell_ids" else np.float64)
+        ne = len(k[(eidx, "cell_ids")])
+        L.check(self._lib.UCFDFlowSysSetElement(
+            ctypes.byref(self.handle), int(eidx), ne, int(nface),
+            *[L.ptr(k[(eidx, n)]) for n in names]), "UCFDFlowSysSetElement")
+
+    def element_array(self, eidx, name):
+        return self.keep[(eidx, name)]
+
+    def call(self, name, *args):
+        L.check(getattr(self._lib, name)(self.handle, *args), name)
+
+    def export(self, which, out=None):
+        n = {RHS: self.nlocal * self.nvars, DU: self.nlocal * self.nvars,
+             DUP: self.nlocal * self.nvars, DIAG: self.nlocal * self.nfvars ** 2,
+             TDIAG: self.nlocal * (self.nvars - self.nfvars) ** 2}[which]
+        out = np.zeros(n) if out is None else out
+        L.check(self._lib.UCFDBLUSGS_Export(self.handle, int(which), L.ptr(out)),
+                "UCFDBLUSGS_Export")
+        return out
+
+    def close(self):
+        if self.handle:
+            self._lib.UCFDFlowSysDestroy(ctypes.byref(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run(sysobj, a0=0.3, turb_factor=1.5, kind="sa", iterations=2):
+    """The stage sequence of tests/golden/refflowsys.run on a FlowSys handle
+    with element 0 set; returns host copies of every stage."""
+    s = sysobj
+    nt = s.nvars - s.nfvars
+    out = {}
+    s.call("UCFDBLUSGS_Pack", 0, float(a0))
+    if nt:
+        s.call("UCFDBLUSGS_SA_Pack" if kind == "sa" else "UCFDBLUSGS_KWSST_Pack", 0,
+               float(turb_factor), float(a0))
+        out["tdiag_pack"] = s.export(TDIAG)
+    out["rank_rhs"] = s.export(RHS)
+    out["diag_pack"] = s.export(DIAG)
+    s.call("UCFDBLUSGS_NSPrepare")
+    out["diag"] = s.export(DIAG)
+    if nt:
+        s.call("UCFDBLUSGS_RANSPrepare")
+        out["tdiag"] = s.export(TDIAG)
+    s.call("UCFDBLUSGS_Reset")
+    for it in range(iterations):
+        s.call("UCFDBLUSGS_NSLowerSweep")
+        out[f"du_lower{it}"] = s.export(DU)
+        s.call("UCFDBLUSGS_NSUpperSweep")
+        out[f"du_upper{it}"] = s.export(DU)
+        if nt:
+            s.call("UCFDBLUSGS_RANSLowerSweep")
+            s.call("UCFDBLUSGS_RANSUpperSweep")
+            out[f"du_rans{it}"] = s.export(DU)
+        s.call("UCFDBLUSGS_SubResidual", 0)
+        r = s.element_array(0, "resid_out")
+        out[f"resid{it}"] = (r.cpu().numpy() if hasattr(r, "data_ptr") else r).reshape(-1).copy()
+    s.call("UCFDBLUSGS_Update", 0)
+    u = s.element_array(0, "uptsb")
+    out["upts"] = (u.cpu().numpy() if hasattr(u, "data_ptr") else u).copy()
+    return out

@@ -88,6 +88,26 @@ _SIGS = {
     "UCFDSolverCreateCUDAGMRES": [c_void_p, c_int, c_int, c_int, c_double],
     "UCFDSolverCreateBICGSTAB": [c_void_p, c_int, c_int, c_double],
     "UCFDSolverSetRightPrecon": [c_void_p, ctypes.c_bool],
+    # FlowSys API (include/ucfdlusgs.h)
+    "UCFDFlowSysCreate": [c_void_p] + [c_int] * 6 + [c_void_p] * 7,
+    "UCFDFlowSysSetElement": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 7,
+    "UCFDFlowSysDestroy": [c_void_p],
+    "UCFDFlowSysSetLUSGS": [c_void_p, c_void_p, c_void_p],
+    "UCFDFlowSysSetBLUSGS": [c_void_p, c_void_p, c_void_p],
+    "UCFDBLUSGS_Pack": [c_void_p, c_int, c_double],
+    "UCFDBLUSGS_KWSST_Pack": [c_void_p, c_int, c_double, c_double],
+    "UCFDBLUSGS_SA_Pack": [c_void_p, c_int, c_double, c_double],
+    "UCFDBLUSGS_Update": [c_void_p, c_int],
+    "UCFDBLUSGS_SubResidual": [c_void_p, c_int],
+    "UCFDBLUSGS_NSPrepare": [c_void_p],
+    "UCFDBLUSGS_RANSPrepare": [c_void_p],
+    "UCFDBLUSGS_NSLowerSweep": [c_void_p],
+    "UCFDBLUSGS_RANSLowerSweep": [c_void_p],
+    "UCFDBLUSGS_NSUpperSweep": [c_void_p],
+    "UCFDBLUSGS_RANSUpperSweep": [c_void_p],
+    "UCFDBLUSGS_Reset": [c_void_p],
+    "UCFDBLUSGS_SetPrecision": [c_void_p, c_int],
+    "UCFDBLUSGS_Export": [c_void_p, c_int, c_void_p],
 }
 
 UCFD_SOLVE_FP64, UCFD_SOLVE_MIXED, UCFD_SOLVE_HIST = 0, 1, 128

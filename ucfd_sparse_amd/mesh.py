@@ -55,6 +55,10 @@ def _lib():
         lib.ucfdgen_bsr_fill.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_uint64] + [vp] * 10
         lib.ucfdgen_bsr_fill.restype = ctypes.c_int
+        lib.ucfdgen_flowsys_fill.argtypes = ([ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_uint64] + [vp] * 16
+                                             + [ctypes.c_int])
+        lib.ucfdgen_flowsys_fill.restype = ctypes.c_int
         lib.ucfdgen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         lib.ucfdgen_uniform.restype = ctypes.c_double
         lib.ucfdgen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
@@ -233,3 +237,72 @@ def normal_fill(n: int, seed: int, stream: int = 77) -> np.ndarray:
     out = np.empty(n, np.float64)
     _lib().ucfdgen_normal_fill(seed, stream, n, _p(out))
     return out
+
+
+@dataclass
+class FlowSysMesh:
+    """Face-slot CSR arrays of the FlowSys API (src/lusgs/flowsys.h:34-59)."""
+    nlocal: int
+    nfaces: int              # number of face slots
+    rowptr: np.ndarray       # int32 [nlocal+1]
+    colidx: np.ndarray       # int32 [npos] neighbour cell of each position
+    slots: np.ndarray        # int32 [npos] face slot of each position
+    sides: np.ndarray        # int8 [npos] +1 owner side, -1 other side
+    face_area: np.ndarray    # f64 [nfaces]
+    face_normal: np.ndarray  # f64 [ndims, nfaces]
+    rcp_vol: np.ndarray      # f64 [nlocal]
+    jmat: np.ndarray         # f64 [2, nv, nv, nfaces]
+
+
+def flowsys_system(mesh: Mesh, flow: Flow, nv: int = 5, seed: int = 0) -> FlowSysMesh:
+    """FlowSys arrays of a mesh (definition in ucfdgen_flowsys_fill,
+    csrc/meshgen.c): interior faces as positions, jmat = A+/A- of the
+    block Jacobian used by block_system.  Meshes without volumes (the hex
+    generator) use rcp_vol = 1 and the owner's fnorm_vol as face area."""
+    N, F = mesh.neles, mesh.nface
+    D = mesh.vec_fnorm.shape[1]
+    lib = _lib()
+    ns, npos = ctypes.c_int64(), ctypes.c_int64()
+    area = mesh.face_area if mesh.vol is not None else mesh.fnorm_vol
+    rcp = 1.0 / mesh.vol if mesh.vol is not None else np.ones(N)
+    args = [N, F, D, nv, seed, _p(mesh.nei_ele), _p(area), _p(mesh.vec_fnorm),
+            _p(flow.fspr), _p(mesh.orig_id), _p(mesh.centroid), _p(mesh.fnorm_vol)]
+    rc = lib.ucfdgen_flowsys_fill(*args, ctypes.byref(ns), ctypes.byref(npos), *([None] * 7), 0)
+    if rc:
+        raise ValueError("ucfdgen_flowsys_fill failed")
+    ns, npos = ns.value, npos.value
+    rowptr = np.empty(N + 1, np.int32)
+    colidx = np.empty(npos, np.int32)
+    slots = np.empty(npos, np.int32)
+    sides = np.empty(npos, np.int8)
+    face_area = np.empty(ns)
+    face_normal = np.empty((D, ns))
+    jmat = np.empty((2, nv, nv, ns))
+    rc = lib.ucfdgen_flowsys_fill(*args, ctypes.byref(ctypes.c_int64()),
+                                  ctypes.byref(ctypes.c_int64()), _p(rowptr), _p(colidx),
+                                  _p(slots), _p(sides), _p(face_area), _p(face_normal), _p(jmat), 1)
+    if rc:
+        raise ValueError("ucfdgen_flowsys_fill failed")
+    return FlowSysMesh(N, ns, rowptr, colidx, slots, sides, face_area, face_normal, rcp, jmat)
+
+
+def flowsys_elements(mesh: Mesh, flow: Flow, nvars: int = 6, seed: int = 0) -> dict:
+    """One element type covering every cell of the mesh for the FlowSys API:
+    cell_ids = a seeded permutation (element-local index -> rank cell), and
+    element arrays in [nvars, neles] layout: uptsb (the flow state, then
+    turbulence variables U[0.1, 1]), rhs ~ N(0, 1e-3), dt (the flow's),
+    dsrc ~ U[0, 1] (source-Jacobian inputs), vol (the cells' volumes)."""
+    N = mesh.neles
+    rng = np.random.default_rng(seed + 4242)
+    cell_ids = rng.permutation(N).astype(np.int32)
+    nf = flow.upts.shape[0]
+    up = np.empty((nvars, N))
+    up[:min(nf, nvars)] = flow.upts[:min(nf, nvars), cell_ids]
+    if nvars > nf:
+        up[nf:] = rng.uniform(0.1, 1.0, size=(nvars - nf, N))
+    return {"cell_ids": cell_ids, "uptsb": up,
+            "rhs": rng.normal(0.0, 1e-3, size=(nvars, N)),
+            "dt": np.ascontiguousarray(flow.dt[cell_ids]),
+            "dsrc": rng.uniform(0.0, 1.0, size=(nvars, N)),
+            "vol": (np.ascontiguousarray(mesh.vol[cell_ids]) if mesh.vol is not None
+                    else np.ones(N))}
