@@ -269,6 +269,126 @@ def krylov_workload(args, log):
     print(json.dumps(rec), flush=True)
 
 
+def adjoint_workload(args, log):
+    """Differentiation form: the FP64 coloured LU-SGS pass (pre + lower +
+    upper) on a 55^3 x 6 tet mesh (998,250 cells) behind
+    torch.autograd.Function with the hand-written adjoint; J = sum(w dQ),
+    w ~ N(0,1).  Times forward + backward with dQ* stored and recomputed,
+    reports the peak memory of each, and checks the gradients with respect to
+    rhsb, dt and fspr against central differences (h = 1e-6) of the forward
+    on 64 seeded directions per input."""
+    import numpy as np
+    import torch
+
+    from ucfd_sparse_amd import mesh as M
+    from ucfd_sparse_amd.autograd import ColoredSystem, lusgs_pass
+    from ucfd_sparse_amd.layout import colored_layout
+
+    m, seed = args.adjoint_m, 4
+    t0 = time.time()
+    mesh = M.tet_mesh(m, seed=seed)
+    flow = M.flow_fields(mesh, seed=seed)
+    mesh.key = M.hilbert_keys(mesh, "tet", (m,))
+    lay = colored_layout(mesh, flow)
+    del mesh, flow
+    N = lay.mesh.neles
+    log(f"adjoint mesh {N} cells, {lay.ncolors} colours, setup {time.time() - t0:.1f} s")
+    S = ColoredSystem(lay)
+    dev = torch.device("cuda:0")
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    f = lay.flow
+    rhs0, dt0, fs0 = up(f.rhsb[:5]), up(f.dt), up(f.fspr)
+    w = up(M.normal_fill(5 * N, 4000).reshape(5, N))
+    runs = {}
+    grads = {}
+    for recompute in (False, True):
+        def step():
+            r = rhs0.clone().requires_grad_()
+            d = dt0.clone().requires_grad_()
+            q = fs0.clone().requires_grad_()
+            J = (w * lusgs_pass(S, r, d, q, recompute=recompute)).sum()
+            J.backward()
+            return r.grad, d.grad, q.grad
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        reps = args.adjoint_reps
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            g = step()
+        e1.record()
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated() - base
+        runs["recompute" if recompute else "stored"] = {
+            "ms_forward_backward": e0.elapsed_time(e1) / reps, "peak_extra_bytes": peak}
+        grads[recompute] = g
+        log(f"recompute={recompute}: {e0.elapsed_time(e1) / reps:.3f} ms fwd+bwd, "
+            f"peak extra {peak / 2**20:.1f} MiB")
+    same = all(torch.equal(a, b) for a, b in zip(grads[False], grads[True]))
+    # forward alone and adjoint alone (stored dQ*)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    dq, dstar, _ = S.forward_arrays(rhs0, dt0, fs0)
+    e1.record()
+    S.adjoint(dt0, fs0, dstar, dq, w)
+    e2.record()
+    torch.cuda.synchronize()
+    fwd_ms, adj_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    # central differences: 64 directions per input
+    h = 1e-6
+    scale = {"rhsb": 1.0, "dt": 1.0, "fspr": 1e3}
+    rng = np.random.default_rng(40)
+    base_in = {"rhsb": rhs0, "dt": dt0, "fspr": fs0}
+    g_in = dict(zip(("rhsb", "dt", "fspr"), grads[False]))
+    fd_err = {}
+    for name in ("rhsb", "dt", "fspr"):
+        worst = 0.0
+        for _ in range(args.adjoint_dirs):
+            d = torch.from_numpy(rng.normal(size=tuple(base_in[name].shape)) * scale[name]).to(dev)
+            xp = dict(base_in, **{name: base_in[name] + h * d})
+            xm = dict(base_in, **{name: base_in[name] - h * d})
+            dqp, _, _ = S.forward_arrays(xp["rhsb"], xp["dt"], xp["fspr"])
+            dqm, _, _ = S.forward_arrays(xm["rhsb"], xm["dt"], xm["fspr"])
+            fd = float((w * (dqp - dqm)).sum()) / (2 * h)
+            ad = float((g_in[name] * d).sum())
+            worst = max(worst, abs(fd - ad) / abs(ad))
+        fd_err[name] = worst
+        log(f"FD check {name}: worst relative disagreement {worst:.2e} over "
+            f"{args.adjoint_dirs} directions")
+    S.close()
+    st = runs["stored"]["ms_forward_backward"]
+    rec = {
+        "metric": "differentiation form: forward + adjoint time of one coloured LU-SGS pass",
+        "value": st,
+        "unit": "ms per forward+backward",
+        "n_gpus": 1,
+        "steps": args.adjoint_reps,
+        "warmup": 2,
+        "ms_per_step": st,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Kuhn-tet mesh and Euler fields, ucfd_sparse_amd.mesh)",
+        "config": {"workload": f"FP64 coloured LU-SGS pass behind torch.autograd.Function, "
+                               f"{m}^3 hex x 6 tets, J = sum(w dQ)",
+                   "neles": N, "ncolors": lay.ncolors},
+        "forward_ms": fwd_ms,
+        "adjoint_ms": adj_ms,
+        "modes": runs,
+        "recompute_gradients_identical": same,
+        "fd_check": {"h": h, "directions_per_input": args.adjoint_dirs,
+                     "direction_scale": scale, "worst_relative_error": fd_err,
+                     "pass_1e-6": max(fd_err.values()) <= 1e-6},
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,9 +400,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--workload", choices=("lusgs", "krylov"), default="lusgs",
+    ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint"), default="lusgs",
                     help="lusgs: the headline colored LU-SGS line (default); krylov: the "
-                         "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system")
+                         "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system; adjoint: "
+                         "the differentiation form on a 55^3 x 6 tet mesh")
+    ap.add_argument("--adjoint-m", type=int, default=55)
+    ap.add_argument("--adjoint-reps", type=int, default=10)
+    ap.add_argument("--adjoint-dirs", type=int, default=64)
     ap.add_argument("--krylov-m", type=int, default=70)
     ap.add_argument("--krylov-rhs", type=int, default=4)
     ap.add_argument("--calibrate-copy-mb", type=float, default=0.0,
@@ -318,6 +442,9 @@ def main():
 
     if args.workload == "krylov":
         krylov_workload(args, log)
+        return
+    if args.workload == "adjoint":
+        adjoint_workload(args, log)
         return
 
     t0 = time.time()

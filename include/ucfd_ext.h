@@ -111,6 +111,28 @@ int ucfd_lusgs_residual(ucfd_lusgs_solver *solver, const double *b, const double
 int ucfd_lusgs_solve(ucfd_lusgs_solver *solver, int mode, const double *b, double *dq,
                      double rtol, int maxiter, ucfd_solve_stats *stats);
 
+/* ---- Differentiation form (no counterpart in the reference).
+ * Adjoint of one FP64 coloured pass dQ = upper(lower(pre(dt, fspr), rhsb))
+ * on a colour-contiguous numbering (colour c = cells color_off[c] ..
+ * color_off[c+1]-1, the layout of ucfd_declare_colored_layout): the upper
+ * sweep transposed (colours ascending), then the lower sweep transposed
+ * (colours descending), each cell gathering its neighbours' adjoints
+ * through the face-pair map in face order (deterministic).  The handle
+ * keeps the mesh pointers (nei_ele [nface][neles], fnorm_vol, vec_fnorm,
+ * upts [NVARS][neles]); create validates the colour ranges and builds the
+ * face-pair map. */
+int ucfd_adjoint_create(void **handle, int neles, int nface, int ncolors, const int *color_off,
+                        const int *nei_ele, const double *fnorm_vol, const double *vec_fnorm,
+                        const double *upts);
+int ucfd_adjoint_destroy(void *handle);
+/* Given the forward's dstar (lower-sweep output) and dq (upper-sweep output),
+ * both [NFVARS][neles], and w = dJ/d dq: rhsbar = dJ/d rhsb (rows < NFVARS),
+ * dtbar = dJ/d dt [neles], fsprbar = dJ/d fspr [nface][neles].  factor as in
+ * the pre call (diag = 1/(dt factor) + ...). */
+int ucfd_lusgs_adjoint(void *handle, double factor, const double *dt, const double *fspr,
+                       const double *dstar, const double *dq, const double *w, double *rhsbar,
+                       double *dtbar, double *fsprbar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -377,3 +377,61 @@ def flowsys_run(fs, tjmat, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kind="sa"
     O.update()
     out["upts"] = O.upts.copy()
     return out
+
+
+# ---------------------------------------------------------------- adjoint
+def _adjlib():
+    L = lib()
+    if not hasattr(L, "_adj_bound"):
+        vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+        L.orc_flux_jt.argtypes = [ci, vp, vp, vp, vp]
+        L.orc_flux_jt.restype = None
+        L.orc_lusgs_adjoint.argtypes = [cl, ci, ci, ci] + [vp] * 7 + [cd] + [vp] * 9
+        L.orc_lusgs_adjoint.restype = None
+        L._adj_bound = True
+    return L
+
+
+def flux_jt(u, n, v):
+    """A(u; n)^T v (oracle/adjoint_oracle.c)."""
+    out = np.zeros(len(u))
+    _adjlib().orc_flux_jt(len(n), _c(u, np.float64).ctypes.data, _c(n, np.float64).ctypes.data,
+                          _c(v, np.float64).ctypes.data, out.ctypes.data)
+    return out
+
+
+def face_pairs(nei):
+    """pair[f, i] = face of nb = nei[f, i] pointing back to i (-1 on self faces)."""
+    F, N = nei.shape
+    pair = np.full((F, N), -1, np.int32)
+    for f in range(F):
+        nb = nei[f]
+        inner = nb != np.arange(N)
+        for g in range(F):
+            hit = inner & (pair[f] < 0) & (nei[g, nb] == np.arange(N))
+            pair[f, hit] = g
+    return pair
+
+
+def lusgs_adjoint(nei, off, fnorm_vol, vec_fnorm, fspr, dt, upts, dstar, dq, w, factor=1.0):
+    """(rhsbar, dtbar, fsprbar) of J = sum(w * dq) for one coloured pass on a
+    colour-contiguous numbering (colour c = cells off[c]..off[c+1]-1)."""
+    F, N = nei.shape
+    D = vec_fnorm.shape[1]
+    nv = D + 2
+    diag = pre(fnorm_vol, dt, fspr, factor)
+    pair = face_pairs(nei)
+    work = np.zeros((nv, N))
+    rb = np.zeros((nv, N))
+    dtb = np.zeros(N)
+    fb = np.zeros((F, N))
+    o = _c(off, np.int32)
+    _adjlib().orc_lusgs_adjoint(
+        N, F, D, len(o) - 1, o.ctypes.data, _c(nei, np.int32).ctypes.data, pair.ctypes.data,
+        _c(fnorm_vol, np.float64).ctypes.data, _c(vec_fnorm, np.float64).ctypes.data,
+        _c(fspr, np.float64).ctypes.data, _c(dt, np.float64).ctypes.data, factor,
+        diag.ctypes.data, _c(upts[:nv], np.float64).ctypes.data,
+        _c(dstar[:nv], np.float64).ctypes.data, _c(dq[:nv], np.float64).ctypes.data,
+        _c(w[:nv], np.float64).ctypes.data, work.ctypes.data, rb.ctypes.data, dtb.ctypes.data,
+        fb.ctypes.data)
+    return rb, dtb, fb

@@ -50,6 +50,15 @@ def test_library_exports_every_declared_symbol():
     assert not missing, f"declared but not exported: {missing}"
 
 
+def test_every_declared_symbol_has_ctypes_argtypes():
+    """The Python binding never calls an entry point with default ctypes
+    conversions (which would truncate 64-bit pointers)."""
+    lib = L.load()
+    unbound = [n for ns in declared_functions().values() for n in ns
+               if getattr(lib, n).argtypes is None]
+    assert not unbound, unbound
+
+
 def test_legacy_signatures_match_reference_ctypes_bindings():
     """argtypes of examples/omp3d.py:27-35 and examples/mpi3d.py:26-34."""
     import ctypes

@@ -56,6 +56,65 @@ __host__ __device__ __forceinline__ void ns_flux(const T *uin, const T *nf, T *f
     f[NV - 1] = ht*contrav;
 }
 
+// out = A(u; n)^T v with A = dF/du the Jacobian of ns_flux (conservative
+// variables).  Unclamped (p >= PMIN), with theta = (m.n)/rho, u = m/rho,
+// phi = (gamma-1)|u|^2/2, H = (E+p)/rho:
+//   out_rho = -theta (u.v_m) + phi (n.v_m) + theta (phi - H) v_E
+//   out_m   = n v_rho + theta v_m + n (u.v_m) - (gamma-1) u (n.v_m)
+//             + (H n - (gamma-1) theta u) v_E
+//   out_E   = (gamma-1) (n.v_m) + gamma theta v_E
+// Clamped (p = PMIN, E replaced by PMIN/(gamma-1) + |m|^2/(2 rho)): the
+// pressure no longer depends on u and the flux on E:
+//   out_rho = -theta (u.v_m) - theta (|u|^2/2 + H) v_E
+//   out_m   = n v_rho + theta v_m + n (u.v_m) + (theta u + H n) v_E
+//   out_E   = 0
+template <typename T, int ND>
+__host__ __device__ __forceinline__ void ns_flux_jt(const T *uin, const T *nf, const T *v, T *out)
+{
+    constexpr int NV = ND + 2;
+    constexpr T g = Const<T>::gamma, gm1 = Const<T>::gamma - T(1);
+    const T rho = uin[0], inv = T(1)/rho;
+    T temp = T(0), contrav = T(0);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        contrav += uin[i + 1]*nf[i];
+        temp += uin[i + 1]*uin[i + 1];
+    }
+    const T theta = contrav/rho;
+    T et = uin[NV - 1];
+    T p = gm1*(et - T(0.5)*temp/rho);
+    const bool clamped = p < Const<T>::pmin;
+    if (clamped) {
+        p = Const<T>::pmin;
+        et = p/gm1 + T(0.5)*temp/rho;
+    }
+    const T H = (et + p)*inv;
+    T vel[ND], um = T(0), nm = T(0), q2 = T(0);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        vel[d] = uin[d + 1]*inv;
+        um += vel[d]*v[d + 1];
+        nm += nf[d]*v[d + 1];
+        q2 += vel[d]*vel[d];
+    }
+    const T v0 = v[0], vE = v[NV - 1];
+    if (!clamped) {
+        const T phi = gm1*T(0.5)*q2;
+        out[0] = -theta*um + phi*nm + theta*(phi - H)*vE;
+#pragma unroll
+        for (int k = 0; k < ND; ++k)
+            out[k + 1] = nf[k]*v0 + theta*v[k + 1] + nf[k]*um - gm1*vel[k]*nm
+                         + (H*nf[k] - gm1*theta*vel[k])*vE;
+        out[NV - 1] = gm1*nm + g*theta*vE;
+    } else {
+        out[0] = -theta*um - theta*(T(0.5)*q2 + H)*vE;
+#pragma unroll
+        for (int k = 0; k < ND; ++k)
+            out[k + 1] = nf[k]*v0 + theta*v[k + 1] + nf[k]*um + (theta*vel[k] + H*nf[k])*vE;
+        out[NV - 1] = T(0);
+    }
+}
+
 // dfj = F(u + du) - F(u)  (src/lusgs/lusgs.c:143-155)
 template <typename T, int ND>
 __host__ __device__ __forceinline__ void diff_flux(const T *u, const T *du, const T *nf, T *dfj)

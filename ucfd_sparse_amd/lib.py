@@ -57,6 +57,9 @@ _SIGS = {
     "ucfd_lusgs_solver_destroy": [c_void_p],
     "ucfd_lusgs_residual": [c_void_p] * 5,
     "ucfd_lusgs_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_void_p],
+    "ucfd_adjoint_create": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 5,
+    "ucfd_adjoint_destroy": [c_void_p],
+    "ucfd_lusgs_adjoint": [c_void_p, c_double] + [c_void_p] * 8,
     # Krylov API (include/ucfdmat.h, ucfdprecon.h, ucfdsolver.h); handles are
     # passed by address (SpMat*, Precon*, Solver*) or by value (void*)
     "UCFDMatInit": [c_void_p],
@@ -93,6 +96,14 @@ _SIGS = {
     "UCFDFlowSysSetElement": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 7,
     "UCFDFlowSysDestroy": [c_void_p],
     "UCFDFlowSysSetLUSGS": [c_void_p, c_void_p, c_void_p],
+    "UCFDLUSGS_Pack": [c_void_p, c_int, c_double, c_double],
+    "UCFDLUSGS_Update": [c_void_p, c_int],
+    "UCFDLUSGS_NSPrepare": [c_void_p, c_double],
+    "UCFDLUSGS_RANSPrepare": [c_void_p, c_double],
+    "UCFDLUSGS_NSLowerSweep": [c_void_p, c_double],
+    "UCFDLUSGS_NSUpperSweep": [c_void_p, c_double],
+    "UCFDLUSGS_RANSLowerSweep": [c_void_p, c_double],
+    "UCFDLUSGS_RANSUpperSweep": [c_void_p, c_double],
     "UCFDFlowSysSetBLUSGS": [c_void_p, c_void_p, c_void_p],
     "UCFDBLUSGS_Pack": [c_void_p, c_int, c_double],
     "UCFDBLUSGS_KWSST_Pack": [c_void_p, c_int, c_double, c_double],
