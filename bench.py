@@ -222,6 +222,10 @@ def krylov_workload(args, log):
     for k, v in kern.items():
         log(f"{k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s algorithmic")
 
+    # the solves and the residual checks below run on the default stream
+    # (library and torch alike)
+    L.call("ucfd_set_stream", None)
+    torch.cuda.synchronize()
     solves = {}
     for kind in ("gmres", "bicgstab"):
         for pname in ("fp32", "fp64"):
@@ -244,8 +248,8 @@ def krylov_workload(args, log):
             solves[key] = {"ms_mean": sum(r["ms"] for r in rec) / len(rec), "per_rhs": rec}
             rels = ", ".join(f"{r['true_rel_residual']:.1e}" for r in rec)
             log(f"{key}: {solves[key]['ms_mean']:.1f} ms mean, iterations "
-                f"{[r['iterations'] for r in rec]}, true rel residuals {rels}")
-    L.call("ucfd_set_stream", None)
+                f"{[r['iterations'] for r in rec]}, status {[r['status'] for r in rec]}, "
+                f"true rel residuals {rels}")
     g32 = solves["gmres_fp32"]["ms_mean"]
     rec = {
         "metric": "Krylov time to rtol (GMRES(30), right-preconditioned coloured block LU-SGS)",
@@ -304,18 +308,27 @@ def adjoint_workload(args, log):
     runs = {}
     grads = {}
     for recompute in (False, True):
-        def step():
+        def step(mem=None):
             r = rhs0.clone().requires_grad_()
             d = dt0.clone().requires_grad_()
             q = fs0.clone().requires_grad_()
+            if mem is not None:
+                torch.cuda.synchronize()
+                mem["before"] = torch.cuda.memory_allocated()
             J = (w * lusgs_pass(S, r, d, q, recompute=recompute)).sum()
+            if mem is not None:
+                torch.cuda.synchronize()
+                mem["saved"] = torch.cuda.memory_allocated() - mem["before"]
             J.backward()
             return r.grad, d.grad, q.grad
         for _ in range(2):
             step()
         torch.cuda.synchronize()
-        base = torch.cuda.memory_allocated()
+        mem = {}
         torch.cuda.reset_peak_memory_stats()
+        step(mem)
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated() - mem["before"]
         reps = args.adjoint_reps
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -323,12 +336,13 @@ def adjoint_workload(args, log):
             g = step()
         e1.record()
         torch.cuda.synchronize()
-        peak = torch.cuda.max_memory_allocated() - base
         runs["recompute" if recompute else "stored"] = {
-            "ms_forward_backward": e0.elapsed_time(e1) / reps, "peak_extra_bytes": peak}
+            "ms_forward_backward": e0.elapsed_time(e1) / reps,
+            "bytes_held_forward_to_backward": mem["saved"],
+            "peak_bytes_forward_backward": peak}
         grads[recompute] = g
-        log(f"recompute={recompute}: {e0.elapsed_time(e1) / reps:.3f} ms fwd+bwd, "
-            f"peak extra {peak / 2**20:.1f} MiB")
+        log(f"recompute={recompute}: {e0.elapsed_time(e1) / reps:.3f} ms fwd+bwd, held after "
+            f"forward {mem['saved'] / 2**20:.1f} MiB, peak {peak / 2**20:.1f} MiB")
     same = all(torch.equal(a, b) for a, b in zip(grads[False], grads[True]))
     # forward alone and adjoint alone (stored dQ*)
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -339,27 +353,48 @@ def adjoint_workload(args, log):
     e2.record()
     torch.cuda.synchronize()
     fwd_ms, adj_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
-    # central differences: 64 directions per input
+    # central differences, h = 1e-6, on seeded directions per input (unit
+    # scale for rhsb, relative x100 for dt and fspr):
+    #  * Gaussian directions: the scale-free error |fd - ad| / (|g| |d|) and
+    #    the raw |fd - ad| / |<g, d>|.  The raw error is the difference
+    #    quotient's per-entry accuracy amplified by sum|g_i d_i| / |<g, d>|,
+    #    about sqrt(N) for random-sign directions.
+    #  * gradient-signed directions d_i = sign(g_i) |z_i|: <g, d> has no
+    #    cancellation, so the raw error is meaningful; a wrong sign or
+    #    magnitude in any entry of g still shows.
     h = 1e-6
-    scale = {"rhsb": 1.0, "dt": 1.0, "fspr": 1e3}
     rng = np.random.default_rng(40)
     base_in = {"rhsb": rhs0, "dt": dt0, "fspr": fs0}
     g_in = dict(zip(("rhsb", "dt", "fspr"), grads[False]))
+
+    def scaled(name, z):
+        return z if name == "rhsb" else z * base_in[name] * 100.0
+
+    def fd_of(name, d):
+        xp = dict(base_in, **{name: base_in[name] + h * d})
+        xm = dict(base_in, **{name: base_in[name] - h * d})
+        dqp, _, _ = S.forward_arrays(xp["rhsb"], xp["dt"], xp["fspr"])
+        dqm, _, _ = S.forward_arrays(xm["rhsb"], xm["dt"], xm["fspr"])
+        return float((w * (dqp - dqm)).sum()) / (2 * h)
     fd_err = {}
     for name in ("rhsb", "dt", "fspr"):
-        worst = 0.0
+        shape = tuple(base_in[name].shape)
+        g = g_in[name]
+        gnorm = float(g.norm())
+        worst_n = worst_r = worst_s = 0.0
         for _ in range(args.adjoint_dirs):
-            d = torch.from_numpy(rng.normal(size=tuple(base_in[name].shape)) * scale[name]).to(dev)
-            xp = dict(base_in, **{name: base_in[name] + h * d})
-            xm = dict(base_in, **{name: base_in[name] - h * d})
-            dqp, _, _ = S.forward_arrays(xp["rhsb"], xp["dt"], xp["fspr"])
-            dqm, _, _ = S.forward_arrays(xm["rhsb"], xm["dt"], xm["fspr"])
-            fd = float((w * (dqp - dqm)).sum()) / (2 * h)
-            ad = float((g_in[name] * d).sum())
-            worst = max(worst, abs(fd - ad) / abs(ad))
-        fd_err[name] = worst
-        log(f"FD check {name}: worst relative disagreement {worst:.2e} over "
-            f"{args.adjoint_dirs} directions")
+            z = torch.from_numpy(rng.normal(size=shape)).to(dev)
+            d = scaled(name, z)
+            fd, ad = fd_of(name, d), float((g * d).sum())
+            worst_n = max(worst_n, abs(fd - ad) / (gnorm * float(d.norm())))
+            worst_r = max(worst_r, abs(fd - ad) / abs(ad))
+            d = scaled(name, torch.sign(g) * z.abs())
+            fd, ad = fd_of(name, d), float((g * d).sum())
+            worst_s = max(worst_s, abs(fd - ad) / abs(ad))
+        fd_err[name] = {"gaussian_scale_free": worst_n, "gaussian_raw": worst_r,
+                        "gradient_signed_raw": worst_s}
+        log(f"FD check {name}: gaussian |fd-ad|/(|g||d|) {worst_n:.2e} (raw {worst_r:.2e}), "
+            f"gradient-signed raw {worst_s:.2e}, {args.adjoint_dirs} directions each")
     S.close()
     st = runs["stored"]["ms_forward_backward"]
     rec = {
@@ -383,8 +418,12 @@ def adjoint_workload(args, log):
         "modes": runs,
         "recompute_gradients_identical": same,
         "fd_check": {"h": h, "directions_per_input": args.adjoint_dirs,
-                     "direction_scale": scale, "worst_relative_error": fd_err,
-                     "pass_1e-6": max(fd_err.values()) <= 1e-6},
+                     "direction_scale": {"rhsb": "N(0,1)", "dt": "N(0,1) x 100 dt",
+                                         "fspr": "N(0,1) x 100 fspr"},
+                     "worst": fd_err,
+                     "pass_1e-6": all(v["gaussian_scale_free"] <= 1e-6 and
+                                      v["gradient_signed_raw"] <= 1e-6
+                                      for v in fd_err.values())},
     }
     print(json.dumps(rec), flush=True)
 
@@ -400,6 +439,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--coloring", choices=("greedy", "balanced"), default="greedy",
+                    help="colouring of the headline mesh (ucfd_greedy_coloring or "
+                         "ucfd_balanced_coloring with nface + 1 colours)")
     ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint"), default="lusgs",
                     help="lusgs: the headline colored LU-SGS line (default); krylov: the "
                          "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system; adjoint: "
@@ -451,7 +493,7 @@ def main():
     mesh = M.tet_mesh(args.m, seed=args.seed)
     flow = M.flow_fields(mesh, seed=args.seed)
     mesh.key = M.hilbert_keys(mesh, "tet", (args.m,))
-    lay = colored_layout(mesh, flow)
+    lay = colored_layout(mesh, flow, coloring=args.coloring)
     del mesh, flow
     N, F = lay.mesh.neles, lay.mesh.nface
     nc = lay.ncolors
@@ -563,8 +605,9 @@ def main():
         "data": "synthetic (seeded Kuhn-tet mesh and Euler fields, ucfd_sparse_amd.mesh)",
         "config": {
             "workload": f"colored LU-SGS FP64 legacy API pass (pre + lower + upper + update), "
-                        f"{args.m}^3 hex x 6 Kuhn tets, permuted, greedy coloured, "
+                        f"{args.m}^3 hex x 6 Kuhn tets, permuted, {args.coloring} coloured, "
                         f"colour-contiguous Hilbert-ordered layout",
+            "coloring": args.coloring, "colour_sizes": [int(x) for x in np.diff(lay.off)],
             "neles": N, "nface": F, "ncolors": nc, "seed": args.seed,
             "parallelism": "single GPU",
         },

@@ -43,6 +43,12 @@ int ucfd_get_config(int *nvars, int *nfvars, int *ndims);
  * examples/utils/pbutils.c:98-121. */
 int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
                          int *lcolor, int *color_off);
+/* Balanced colouring (no counterpart in the reference): cells in ascending
+ * id take the least-used colour among 1..C not used by a coloured neighbour,
+ * C = max(ncolors, max degree + 1).  Same outputs as ucfd_greedy_coloring
+ * (color_off holds C+1 entries); returns C. */
+int ucfd_balanced_coloring(int neles, int nface, const int *nei_ele, int ncolors, int *icolor,
+                           int *lcolor, int *color_off);
 /* face pairs with equal colours (0 = valid colouring) */
 long long ucfd_coloring_conflicts(int neles, int nface, const int *nei_ele,
                                   const int *lcolor);
@@ -128,7 +134,8 @@ int ucfd_adjoint_destroy(void *handle);
 /* Given the forward's dstar (lower-sweep output) and dq (upper-sweep output),
  * both [NFVARS][neles], and w = dJ/d dq: rhsbar = dJ/d rhsb (rows < NFVARS),
  * dtbar = dJ/d dt [neles], fsprbar = dJ/d fspr [nface][neles].  factor as in
- * the pre call (diag = 1/(dt factor) + ...). */
+ * the pre call (diag = 1/(dt factor) + ...).  rhsbar may be the dstar array
+ * (it is then overwritten), which saves one [NFVARS][neles] buffer. */
 int ucfd_lusgs_adjoint(void *handle, double factor, const double *dt, const double *fspr,
                        const double *dstar, const double *dq, const double *w, double *rhsbar,
                        double *dtbar, double *fsprbar);

@@ -150,3 +150,21 @@ def test_declared_layout_validation_on_host_arrays():
     bad_lc = lc.copy()
     bad_lc[7] = 3
     assert lib.ucfd_declare_colored_layout(N, ic.ctypes.data, bad_lc.ctypes.data, None) != 0
+
+
+def test_balanced_coloring_is_valid_balanced_and_deterministic():
+    from ucfd_sparse_amd import lib as L
+    from ucfd_sparse_amd.layout import balanced_coloring
+    mesh = M.tet_mesh(12, seed=5, permute=True)
+    nei = np.ascontiguousarray(mesh.nei_ele)
+    for target in (0, 5, 7):
+        nc, lcolor, icolor, off = balanced_coloring(nei, target)
+        assert nc == max(target, 5)                      # tets: 4 distinct neighbours
+        assert L.load().ucfd_coloring_conflicts(mesh.neles, 4, nei.ctypes.data,
+                                                lcolor.ctypes.data) == 0
+        sizes = np.diff(off)
+        assert sizes.max() - sizes.min() <= max(2, sizes.max() // 100)
+        assert np.array_equal(np.sort(icolor), np.arange(mesh.neles))
+        assert np.all(np.diff(lcolor[icolor]) >= 0)
+        nc2, lcolor2, _, _ = balanced_coloring(nei, target)
+        assert np.array_equal(lcolor, lcolor2)

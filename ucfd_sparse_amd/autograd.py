@@ -6,9 +6,12 @@ include/ucfd_ext.h).
 The forward runs the legacy API (parallel_pre_lusgs, ns_parallel_lower_sweep
 per colour, ns_parallel_upper_sweep per colour in reverse) on device tensors
 of a colour-contiguous layout, on torch's current stream.  The lower sweep's
-output dQ* is either kept for the backward or, with recompute=True,
-recomputed there from the saved inputs (pre + lower sweeps again), which
-saves 5 x neles doubles of activation memory at the cost of half a forward.
+output dQ* is either kept for the backward (two-array sweeps: rhsb -> dQ*,
+dQ* -> dQ) or, with recompute=True, never kept: the forward runs the
+in-place sweeps on one buffer (rhsb copy -> dQ* -> dQ) and the backward
+recomputes dQ* (pre + lower sweeps again) and lets the adjoint overwrite it
+with the rhsb gradient.  That saves 5 x neles doubles both between forward
+and backward and at the peak, for half a forward of extra work.
 """
 from __future__ import annotations
 
@@ -90,6 +93,28 @@ class ColoredSystem:
                 p["vfn"], p["up"], rhsb.data_ptr(), dstar.data_ptr(), diag.data_ptr(),
                 fspr.data_ptr()), "lower")
 
+    def forward_inplace(self, rhsb, dt, fspr, factor=1.0):
+        """pre + lower + upper with the in-place C sweeps on one buffer;
+        returns dQ (dQ* is overwritten by the upper sweep)."""
+        lib, N, F = self.lib, self.N, self.F
+        diag = torch.empty(N, dtype=torch.float64, device=rhsb.device)
+        dub = rhsb.clone()
+        p = dict(nei=self.nei.data_ptr(), ic=self.icolor.data_ptr(), lc=self.lcolor.data_ptr(),
+                 fnv=self.fnv.data_ptr(), vfn=self.vfn.data_ptr(), up=self.upts.data_ptr())
+        with _Stream(lib):
+            L.check(lib.parallel_pre_lusgs(N, F, float(factor), p["fnv"], dt.data_ptr(),
+                                           diag.data_ptr(), fspr.data_ptr()), "pre")
+            nc = len(self.off) - 1
+            for c in range(nc):
+                L.check(lib.parallel_ns_lower_sweep(
+                    self.off[c], self.off[c + 1], N, F, p["nei"], p["ic"], p["lc"], p["fnv"],
+                    p["vfn"], p["up"], dub.data_ptr(), diag.data_ptr(), fspr.data_ptr()), "lower")
+            for c in reversed(range(nc)):
+                L.check(lib.parallel_ns_upper_sweep(
+                    self.off[c], self.off[c + 1], N, F, p["nei"], p["ic"], p["lc"], p["fnv"],
+                    p["vfn"], p["up"], dub.data_ptr(), diag.data_ptr(), fspr.data_ptr()), "upper")
+        return dub
+
     def recompute_dstar(self, rhsb, dt, fspr, factor=1.0):
         lib = self.lib
         diag = torch.empty(self.N, dtype=torch.float64, device=rhsb.device)
@@ -102,9 +127,10 @@ class ColoredSystem:
             self._lower(p, rhsb, dstar, diag, fspr)
         return dstar
 
-    def adjoint(self, dt, fspr, dstar, dq, w, factor=1.0):
-        """(rhsbar, dtbar, fsprbar) of J = sum(w * dq)."""
-        rb = torch.zeros_like(dq)
+    def adjoint(self, dt, fspr, dstar, dq, w, factor=1.0, overwrite_dstar=False):
+        """(rhsbar, dtbar, fsprbar) of J = sum(w * dq); with overwrite_dstar
+        the rhsb gradient is written into dstar's buffer and returned."""
+        rb = dstar if overwrite_dstar else torch.zeros_like(dq)
         dtb = torch.empty_like(dt)
         fb = torch.empty_like(fspr)
         w = w.contiguous()
@@ -135,12 +161,12 @@ class LUSGSPass(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rhsb, dt, fspr, system, recompute=False, factor=1.0):
         rhsb, dt, fspr = rhsb.contiguous(), dt.contiguous(), fspr.contiguous()
-        dq, dstar, _ = system.forward_arrays(rhsb, dt, fspr, factor)
         ctx.system, ctx.recompute, ctx.factor = system, recompute, factor
         if recompute:
-            del dstar
+            dq = system.forward_inplace(rhsb, dt, fspr, factor)
             ctx.save_for_backward(rhsb, dt, fspr, dq)
         else:
+            dq, dstar, _ = system.forward_arrays(rhsb, dt, fspr, factor)
             ctx.save_for_backward(rhsb, dt, fspr, dq, dstar)
         return dq
 
@@ -150,9 +176,10 @@ class LUSGSPass(torch.autograd.Function):
         if ctx.recompute:
             rhsb, dt, fspr, dq = ctx.saved_tensors
             dstar = S.recompute_dstar(rhsb, dt, fspr, ctx.factor)
+            rb, dtb, fb = S.adjoint(dt, fspr, dstar, dq, w, ctx.factor, overwrite_dstar=True)
         else:
             rhsb, dt, fspr, dq, dstar = ctx.saved_tensors
-        rb, dtb, fb = S.adjoint(dt, fspr, dstar, dq, w, ctx.factor)
+            rb, dtb, fb = S.adjoint(dt, fspr, dstar, dq, w, ctx.factor)
         return rb, dtb, fb, None, None, None
 
 

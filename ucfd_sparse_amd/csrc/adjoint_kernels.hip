@@ -6,6 +6,10 @@
 // so there are no atomics and results are deterministic.  The lower-sweep
 // launch also finishes the cell's gradients with respect to rhsb, dt (via
 // the pre diagonal) and fspr.  Operation order = oracle/adjoint_oracle.c.
+//
+// rhsbar may alias dstar: the lower-sweep adjoint reads a cell's own dstar
+// before writing its rhsbar, and reads dstar only of cells in lower colours,
+// which it processes later (colours descending).
 #include <hip/hip_runtime.h>
 
 #include "adjoint_internal.h"
@@ -109,9 +113,9 @@ __global__ __launch_bounds__(AB) void k_adj_lower(int n0, int ne, int64_t N,
                                                   const double *__restrict__ dt, double factor,
                                                   const double *__restrict__ diag,
                                                   const double *__restrict__ upts,
-                                                  const double *__restrict__ dstar,
+                                                  const double *dstar,
                                                   const double *__restrict__ dq, double *work,
-                                                  double *__restrict__ rhsbar,
+                                                  double *rhsbar,
                                                   double *__restrict__ dtbar,
                                                   double *__restrict__ fsprbar)
 {

@@ -86,6 +86,56 @@ UCFD_API int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int 
     return ncol;
 }
 
+// Balanced colouring: cells in ascending id take, among the colours 1..C not
+// used by an already coloured face neighbour, the one holding the fewest
+// cells so far (ties: the smallest colour).  C = max(ncolors, maximum degree
+// + 1), so a free colour always exists.  Colours come out of nearly equal
+// size: no small trailing colour, and every colour dense enough that the
+// sweeps' neighbour gathers share cache lines.  Returns C.
+UCFD_API int ucfd_balanced_coloring(int neles, int nface, const int *nei_ele, int ncolors,
+                                    int *icolor, int *lcolor, int *color_off)
+{
+    if (neles < 0 || nface < 1 || !nei_ele || !lcolor || ncolors < 0) return -1;
+    if (neles == 0) {
+        if (color_off) color_off[0] = 0;
+        return 0;
+    }
+    const int N = neles;
+    std::vector<int> ptr, adj;
+    sym_graph(N, nface, nei_ele, ptr, adj);
+    int maxdeg = 0;     // distinct neighbours (each face pair is listed from both sides)
+    for (int i = 0; i < N; ++i) {
+        std::sort(adj.begin() + ptr[i], adj.begin() + ptr[i + 1]);
+        const int d = (int)(std::unique(adj.begin() + ptr[i], adj.begin() + ptr[i + 1]) -
+                            (adj.begin() + ptr[i]));
+        maxdeg = std::max(maxdeg, d);
+    }
+    const int C = std::max(ncolors, maxdeg + 1);
+    std::vector<long long> count(C + 1, 0);
+    std::vector<int> mark(C + 1, -1);
+    for (int i = 0; i < N; ++i) {
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+            const int nb = adj[e];
+            if (nb < i) mark[lcolor[nb]] = i;
+        }
+        int best = 0;
+        for (int c = 1; c <= C; ++c)
+            if (mark[c] != i && (best == 0 || count[c] < count[best])) best = c;
+        lcolor[i] = best;
+        count[best]++;
+    }
+    std::vector<int> off(C + 2, 0);
+    for (int i = 0; i < N; ++i) off[lcolor[i] + 1]++;
+    for (int c = 1; c <= C; ++c) off[c + 1] += off[c];
+    if (icolor) {
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (int i = 0; i < N; ++i) icolor[pos[lcolor[i]]++] = i;
+    }
+    if (color_off)
+        for (int c = 0; c <= C; ++c) color_off[c] = off[c + 1];
+    return C;
+}
+
 // Number of face pairs (i, nb != i) with lcolor[i] == lcolor[nb]; 0 means a
 // valid colouring.  Out-of-range neighbours count as conflicts.
 UCFD_API long long ucfd_coloring_conflicts(int neles, int nface, const int *nei_ele,

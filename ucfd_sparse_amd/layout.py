@@ -32,6 +32,21 @@ def greedy_coloring(nei_ele: np.ndarray):
     return nc, lcolor, icolor, off[:nc + 1].copy()
 
 
+def balanced_coloring(nei_ele: np.ndarray, ncolors: int = 0):
+    """Balanced colouring (ucfd_balanced_coloring); returns (ncolors, lcolor,
+    icolor, color_off)."""
+    nei = np.ascontiguousarray(nei_ele, np.int32)
+    F, N = nei.shape
+    lcolor = np.empty(N, np.int32)
+    icolor = np.empty(N, np.int32)
+    off = np.empty(max(2 * F + 4, ncolors + 2, 64), np.int32)
+    nc = L.load().ucfd_balanced_coloring(N, F, nei.ctypes.data, int(ncolors),
+                                         icolor.ctypes.data, lcolor.ctypes.data, off.ctypes.data)
+    if nc <= 0 and N > 0:
+        raise ValueError("colouring failed")
+    return nc, lcolor, icolor, off[:nc + 1].copy()
+
+
 def color_reorder(lcolor: np.ndarray, key: np.ndarray | None = None):
     """perm[old] = new, sorted by (colour, key, id); returns (perm, color_off)."""
     lc = np.ascontiguousarray(lcolor, np.int32)
@@ -77,10 +92,14 @@ class ColoredLayout:
         return len(self.off) - 1
 
 
-def colored_layout(mesh: Mesh, flow: Flow | None = None, use_key: bool = True) -> ColoredLayout:
-    """Colour the mesh and renumber it colour-contiguously (Morton order inside
-    each colour when use_key)."""
-    nc, lcolor, icolor, off = greedy_coloring(mesh.nei_ele)
+def colored_layout(mesh: Mesh, flow: Flow | None = None, use_key: bool = True,
+                   coloring: str = "greedy", ncolors: int = 0) -> ColoredLayout:
+    """Colour the mesh (first-fit greedy, or balanced) and renumber it
+    colour-contiguously (locality-key order inside each colour when use_key)."""
+    if coloring == "balanced":
+        nc, lcolor, icolor, off = balanced_coloring(mesh.nei_ele, ncolors)
+    else:
+        nc, lcolor, icolor, off = greedy_coloring(mesh.nei_ele)
     perm, off2 = color_reorder(lcolor, mesh.key if use_key else None)
     assert np.array_equal(off, off2)
     p = lambda a: None if a is None else permute_cells(a, perm)  # noqa: E731

@@ -48,6 +48,7 @@ _SIGS = {
     "ucfd_get_config": [c_void_p, c_void_p, c_void_p],
     "ucfd_greedy_coloring": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "ucfd_coloring_conflicts": [c_int, c_int, c_void_p, c_void_p],
+    "ucfd_balanced_coloring": [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_color_reorder": [c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_rows": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_nei": [c_int, c_int, c_void_p, c_void_p, c_void_p],
