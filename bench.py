@@ -6,7 +6,11 @@ Euler (nvars = 5), FP64, on a 96^3 hex grid split into 6 Kuhn tets per hex
 normals / areas / volumes from jittered vertex coordinates, fields as in
 ucfd_sparse_amd.mesh.flow_fields.  The mesh is greedy-coloured and stored
 colour-contiguously (Hilbert order inside a colour) and the layout is
-declared to the library (ucfd_declare_colored_layout).  One step = one
+declared to the library (ucfd_declare_colored_layout).  The colouring is
+first-fit greedy visiting the cells along the Hilbert curve, with the highest
+colour then removed by Kempe-chain swaps (4 colours): most cells fall in two
+dense colours whose neighbour gathers stay cache-local, and no tiny colour
+costs a latency-bound launch.  One step = one
 legacy-API pass: parallel_pre_lusgs, ns_parallel_lower_sweep for every
 colour, ns_parallel_upper_sweep for every colour in reverse, and
 lusgs_parallel_ns_update - all on device-resident arrays, enqueued on one
@@ -439,9 +443,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--coloring", choices=("greedy", "balanced"), default="greedy",
-                    help="colouring of the headline mesh (ucfd_greedy_coloring or "
-                         "ucfd_balanced_coloring with nface + 1 colours)")
+    ap.add_argument("--coloring", choices=("greedy", "greedy-key", "greedy-key-reduced",
+                                           "balanced"),
+                    default="greedy-key-reduced",
+                    help="colouring of the headline mesh: first-fit in cell id "
+                         "(ucfd_greedy_coloring), first-fit along the Hilbert curve "
+                         "(ucfd_greedy_coloring_ordered) or balanced")
     ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint"), default="lusgs",
                     help="lusgs: the headline colored LU-SGS line (default); krylov: the "
                          "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system; adjoint: "

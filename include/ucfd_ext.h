@@ -43,6 +43,19 @@ int ucfd_get_config(int *nvars, int *nfvars, int *ndims);
  * examples/utils/pbutils.c:98-121. */
 int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
                          int *lcolor, int *color_off);
+/* First-fit greedy colouring visiting the cells in order[0..neles) (a
+ * permutation; no counterpart in the reference), e.g. along a space-filling
+ * curve of the cell centres.  Same outputs as ucfd_greedy_coloring; returns
+ * ncolors, or < 0 on bad input (order not a permutation). */
+int ucfd_greedy_coloring_ordered(int neles, int nface, const int *nei_ele, const int *order,
+                                 int *icolor, int *lcolor, int *color_off);
+/* Removes the highest colour where possible (no counterpart in the
+ * reference): each of its cells takes a colour no neighbour uses, or one
+ * freed by swapping a two-colour Kempe chain of at most max_chain cells.
+ * lcolor (a valid colouring) is updated in place; icolor and color_off
+ * (nullable) are rebuilt.  Returns the new number of colours. */
+int ucfd_reduce_colors(int neles, int nface, const int *nei_ele, int max_chain, int *icolor,
+                       int *lcolor, int *color_off);
 /* Balanced colouring (no counterpart in the reference): cells in ascending
  * id take the least-used colour among 1..C not used by a coloured neighbour,
  * C = max(ncolors, max degree + 1).  Same outputs as ucfd_greedy_coloring

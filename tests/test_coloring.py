@@ -168,3 +168,44 @@ def test_balanced_coloring_is_valid_balanced_and_deterministic():
         assert np.all(np.diff(lcolor[icolor]) >= 0)
         nc2, lcolor2, _, _ = balanced_coloring(nei, target)
         assert np.array_equal(lcolor, lcolor2)
+
+
+def test_ordered_greedy_coloring():
+    from ucfd_sparse_amd.layout import greedy_coloring, greedy_coloring_ordered
+    mesh = M.tet_mesh(10, seed=9, permute=True)
+    nei = np.ascontiguousarray(mesh.nei_ele)
+    nc, lc, ic, off = greedy_coloring(nei)
+    nc2, lc2, ic2, off2 = greedy_coloring_ordered(nei, np.arange(mesh.neles, dtype=np.int32))
+    assert nc == nc2 and np.array_equal(lc, lc2) and np.array_equal(ic, ic2)
+    order = np.argsort(M.hilbert_keys(mesh, "tet", (10,)), kind="stable").astype(np.int32)
+    nc3, lc3, ic3, off3 = greedy_coloring_ordered(nei, order)
+    assert nc3 <= 5
+    assert L.load().ucfd_coloring_conflicts(mesh.neles, 4, nei.ctypes.data, lc3.ctypes.data) == 0
+    # visiting along the curve packs most cells into the first two colours
+    assert (off3[2] - off3[0]) > 0.75 * mesh.neles
+    bad = order.copy()
+    bad[1] = bad[0]
+    with pytest.raises(ValueError):
+        greedy_coloring_ordered(nei, bad)
+
+
+def test_reduce_colors_removes_the_top_colour():
+    from ucfd_sparse_amd.layout import greedy_coloring_ordered, reduce_colors
+    mesh = M.tet_mesh(14, seed=2, permute=True)
+    nei = np.ascontiguousarray(mesh.nei_ele)
+    order = np.argsort(M.hilbert_keys(mesh, "tet", (14,)), kind="stable").astype(np.int32)
+    nc, lc, ic, off = greedy_coloring_ordered(nei, order)
+    nc2, lc2, ic2, off2 = reduce_colors(nei, lc)
+    assert nc2 <= nc
+    assert L.load().ucfd_coloring_conflicts(mesh.neles, 4, nei.ctypes.data, lc2.ctypes.data) == 0
+    assert np.array_equal(np.diff(off2), np.bincount(lc2)[1:])
+    assert np.array_equal(np.sort(ic2), np.arange(mesh.neles))
+    # cells that did not have the top colour keep theirs unless a chain moved them
+    changed = np.count_nonzero(lc2 != lc)
+    assert changed <= (lc == nc).sum() * 4097
+    # a 2-D red-black grid cannot lose a colour
+    hexm = M.hex_mesh(4, 4, 4, seed=0, permute=False)
+    nch, lch, _, _ = L and (lambda r: r)(__import__("ucfd_sparse_amd.layout",
+                                                    fromlist=["x"]).greedy_coloring(hexm.nei_ele))
+    nr, lr, _, _ = reduce_colors(hexm.nei_ele, lch)
+    assert nch == 2 and nr == 2

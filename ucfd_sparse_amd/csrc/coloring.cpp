@@ -45,24 +45,22 @@ void sym_graph(int N, int F, const int *nei, std::vector<int> &ptr, std::vector<
 
 }  // namespace
 
-// Returns the number of colours (>= 1), or a negative value on bad input.
-UCFD_API int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
-                                  int *lcolor, int *color_off)
+namespace {
+
+// first-fit in the given visiting order (nullptr: ascending id)
+int greedy(int N, int nface, const int *nei_ele, const int *order, int *icolor, int *lcolor,
+           int *color_off)
 {
-    if (neles < 0 || nface < 1 || !nei_ele || !lcolor) return -1;
-    if (neles == 0) {
-        if (color_off) color_off[0] = 0;
-        return 0;
-    }
-    const int N = neles;
     std::vector<int> ptr, adj;
     sym_graph(N, nface, nei_ele, ptr, adj);
     std::vector<int> mark(2*nface + 8, -1);
+    std::vector<char> done(N, 0);
     int ncol = 0;
-    for (int i = 0; i < N; ++i) {
+    for (int k = 0; k < N; ++k) {
+        const int i = order ? order[k] : k;
         for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
             const int nb = adj[e];
-            if (nb < i) {
+            if (done[nb]) {
                 const int c = lcolor[nb];
                 if (c >= (int)mark.size()) mark.resize(2*c + 2, -1);
                 mark[c] = i;
@@ -72,6 +70,7 @@ UCFD_API int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int 
         while (c < (int)mark.size() && mark[c] == i) ++c;
         if (c >= (int)mark.size()) mark.resize(2*c + 2, -1);
         lcolor[i] = c;
+        done[i] = 1;
         ncol = std::max(ncol, c);
     }
     std::vector<int> off(ncol + 2, 0);
@@ -84,6 +83,40 @@ UCFD_API int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int 
     if (color_off)
         for (int c = 0; c <= ncol; ++c) color_off[c] = off[c + 1];
     return ncol;
+}
+
+}  // namespace
+
+// Returns the number of colours (>= 1), or a negative value on bad input.
+UCFD_API int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
+                                  int *lcolor, int *color_off)
+{
+    if (neles < 0 || nface < 1 || !nei_ele || !lcolor) return -1;
+    if (neles == 0) {
+        if (color_off) color_off[0] = 0;
+        return 0;
+    }
+    return greedy(neles, nface, nei_ele, nullptr, icolor, lcolor, color_off);
+}
+
+// First-fit visiting the cells in order[0..neles) (a permutation), e.g. along
+// a space-filling curve: spatially coherent visiting packs most cells into
+// the first two colours, and the remaining sparse colours are small.
+UCFD_API int ucfd_greedy_coloring_ordered(int neles, int nface, const int *nei_ele,
+                                          const int *order, int *icolor, int *lcolor,
+                                          int *color_off)
+{
+    if (neles < 0 || nface < 1 || !nei_ele || !lcolor || !order) return -1;
+    if (neles == 0) {
+        if (color_off) color_off[0] = 0;
+        return 0;
+    }
+    std::vector<char> seen(neles, 0);
+    for (int k = 0; k < neles; ++k) {
+        if (order[k] < 0 || order[k] >= neles || seen[order[k]]) return -1;
+        seen[order[k]] = 1;
+    }
+    return greedy(neles, nface, nei_ele, order, icolor, lcolor, color_off);
 }
 
 // Balanced colouring: cells in ascending id take, among the colours 1..C not
@@ -134,6 +167,88 @@ UCFD_API int ucfd_balanced_coloring(int neles, int nface, const int *nei_ele, in
     if (color_off)
         for (int c = 0; c <= C; ++c) color_off[c] = off[c + 1];
     return C;
+}
+
+// Removes the highest colour when possible: each of its cells takes a colour
+// no neighbour uses, or one freed by swapping a two-colour Kempe chain
+// (colours a, b) that starts at its only a-coloured neighbour and holds none
+// of its b-coloured neighbours; chains longer than max_chain cells are not
+// tried.  A cell left over keeps the top colour.  lcolor is updated in
+// place, icolor / color_off rebuilt.  Returns the new number of colours.
+UCFD_API int ucfd_reduce_colors(int neles, int nface, const int *nei_ele, int max_chain,
+                                int *icolor, int *lcolor, int *color_off)
+{
+    if (neles < 0 || nface < 1 || !nei_ele || !lcolor || max_chain < 1) return -1;
+    const int N = neles;
+    if (N == 0) return 0;
+    std::vector<int> ptr0, adj0, ptr(N + 1, 0), adj;
+    sym_graph(N, nface, nei_ele, ptr0, adj0);
+    for (int i = 0; i < N; ++i) {          // distinct neighbours only
+        std::sort(adj0.begin() + ptr0[i], adj0.begin() + ptr0[i + 1]);
+        const auto end = std::unique(adj0.begin() + ptr0[i], adj0.begin() + ptr0[i + 1]);
+        adj.insert(adj.end(), adj0.begin() + ptr0[i], end);
+        ptr[i + 1] = (int)adj.size();
+    }
+    int top = 0;
+    for (int i = 0; i < N; ++i) top = std::max(top, lcolor[i]);
+    if (top <= 1) return top;
+    std::vector<int> stamp(N, -1), comp;
+    int tag = 0;
+    for (int v = 0; v < N; ++v) {
+        if (lcolor[v] != top) continue;
+        std::vector<int> cnt(top + 1, 0);
+        for (int e = ptr[v]; e < ptr[v + 1]; ++e)
+            if (adj[e] != v) cnt[lcolor[adj[e]]]++;
+        int c = 1;
+        while (c < top && cnt[c]) ++c;
+        if (c < top) {
+            lcolor[v] = c;
+            continue;
+        }
+        bool done = false;
+        for (int a = top - 1; a >= 1 && !done; --a) {
+            int ua = -1;
+            for (int e = ptr[v]; e < ptr[v + 1]; ++e)
+                if (adj[e] != v && lcolor[adj[e]] == a) ua = adj[e];
+            if (cnt[a] != 1 || ua < 0) continue;
+            for (int b = top - 1; b >= 1 && !done; --b) {
+                if (b == a) continue;
+                ++tag;
+                comp.assign(1, ua);
+                stamp[ua] = tag;
+                bool ok = true;
+                for (size_t q = 0; q < comp.size() && ok; ++q) {
+                    const int x = comp[q];
+                    for (int e = ptr[x]; e < ptr[x + 1]; ++e) {
+                        const int y = adj[e];
+                        if (y == v || stamp[y] == tag || (lcolor[y] != a && lcolor[y] != b)) continue;
+                        stamp[y] = tag;
+                        comp.push_back(y);
+                        if ((int)comp.size() > max_chain) { ok = false; break; }
+                    }
+                }
+                if (!ok) continue;
+                for (int e = ptr[v]; e < ptr[v + 1] && ok; ++e)
+                    if (adj[e] != v && lcolor[adj[e]] == b && stamp[adj[e]] == tag) ok = false;
+                if (!ok) continue;
+                for (int x : comp) lcolor[x] = lcolor[x] == a ? b : a;
+                lcolor[v] = a;
+                done = true;
+            }
+        }
+    }
+    int ncol = 0;
+    for (int i = 0; i < N; ++i) ncol = std::max(ncol, lcolor[i]);
+    std::vector<int> off(ncol + 2, 0);
+    for (int i = 0; i < N; ++i) off[lcolor[i] + 1]++;
+    for (int c = 1; c <= ncol; ++c) off[c + 1] += off[c];
+    if (icolor) {
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (int i = 0; i < N; ++i) icolor[pos[lcolor[i]]++] = i;
+    }
+    if (color_off)
+        for (int c = 0; c <= ncol; ++c) color_off[c] = off[c + 1];
+    return ncol;
 }
 
 // Number of face pairs (i, nb != i) with lcolor[i] == lcolor[nb]; 0 means a

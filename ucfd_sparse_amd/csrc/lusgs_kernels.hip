@@ -19,6 +19,7 @@
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kSweepBlock = 128;   // sweeps: more workgroups for the small colours
 
 __device__ __forceinline__ void raise_flag(int *flag, int bit)
 {
@@ -77,7 +78,7 @@ template <typename T> __device__ __forceinline__ T ld_stream(const T *p)
 // input and output) use non-temporal accesses so they do not evict the
 // gathered neighbour lines from L2.
 template <int DIR, int MODE, int ND, int FC>
-__global__ __launch_bounds__(kBlock) void k_sweep(
+__global__ __launch_bounds__(kSweepBlock) void k_sweep(
     int n0, int cnt, int N, int nface, const int *__restrict__ nei,
     const int *__restrict__ order, const int *__restrict__ lcolor,
     const double *__restrict__ fnv, const double *__restrict__ vfn,
@@ -85,9 +86,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
     const double *__restrict__ diag, const double *__restrict__ fspr, int *flag)
 {
     constexpr int NV = ND + 2;
-    constexpr int G = 2;
+    // all faces' loads in flight at once for tets; pairs of faces for hexes
+    constexpr int G = FC == 4 ? 4 : 2;
     static_assert(FC % G == 0, "face count must be a multiple of the group size");
-    const int t = xcd_block()*kBlock + threadIdx.x;
+    const int t = xcd_block()*kSweepBlock + threadIdx.x;
     if (t >= cnt) return;
     const int i = MODE == 2 ? n0 + t : ld_stream(order + n0 + t);
     if ((unsigned)i >= (unsigned)N) {
@@ -101,6 +103,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
     bool tk[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) nbv[f] = ld_stream(nei + (size_t)f*N + i);
+    // the cell's own rows are loaded before the neighbour gathers so their
+    // latency overlaps (in may alias out: only other cells' out is gathered)
+    const double dg = ld_stream(diag + i);
+    double xin[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) xin[k] = ld_stream(in + (size_t)k*N + i);
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
         const int nb = nbv[f];
@@ -147,13 +155,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
             for (int k = 0; k < NV; ++k) df[k] += (dfj[k] - lam[g]*du[g][k])*fv[g];
         }
     }
-    const double dg = ld_stream(diag + i);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const size_t o = (size_t)k*N + i;
-        const double x = ld_stream(in + o);
-        if (DIR == 0) __builtin_nontemporal_store((x - 0.5*df[k])/dg, out + o);
-        else          __builtin_nontemporal_store(x - (0.5*df[k]/dg), out + o);
+        if (DIR == 0) __builtin_nontemporal_store((xin[k] - 0.5*df[k])/dg, out + o);
+        else          __builtin_nontemporal_store(xin[k] - (0.5*df[k]/dg), out + o);
     }
 }
 
@@ -233,13 +239,14 @@ int launch_sweep_f(hipStream_t s, int n0, int cnt, int N, int nface, const int *
                    const double *fspr, int *flag)
 {
     const dim3 grid((cnt + kBlock - 1)/kBlock), block(kBlock);
+    const dim3 sgrid((cnt + kSweepBlock - 1)/kSweepBlock), sblock(kSweepBlock);
     switch (nface) {
     case 4:
-        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 4>), grid, block, 0, s, n0, cnt, N, nface, nei,
+        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 4>), sgrid, sblock, 0, s, n0, cnt, N, nface, nei,
                            order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
         break;
     case 6:
-        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 6>), grid, block, 0, s, n0, cnt, N, nface, nei,
+        hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 6>), sgrid, sblock, 0, s, n0, cnt, N, nface, nei,
                            order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
         break;
     default:

@@ -32,6 +32,38 @@ def greedy_coloring(nei_ele: np.ndarray):
     return nc, lcolor, icolor, off[:nc + 1].copy()
 
 
+def greedy_coloring_ordered(nei_ele: np.ndarray, order: np.ndarray):
+    """First-fit greedy visiting cells in `order`; returns (ncolors, lcolor,
+    icolor, color_off)."""
+    nei = np.ascontiguousarray(nei_ele, np.int32)
+    F, N = nei.shape
+    o = np.ascontiguousarray(order, np.int32)
+    lcolor = np.empty(N, np.int32)
+    icolor = np.empty(N, np.int32)
+    off = np.empty(max(2 * F + 4, 64), np.int32)
+    nc = L.load().ucfd_greedy_coloring_ordered(N, F, nei.ctypes.data, o.ctypes.data,
+                                               icolor.ctypes.data, lcolor.ctypes.data,
+                                               off.ctypes.data)
+    if nc <= 0 and N > 0:
+        raise ValueError("colouring failed")
+    return nc, lcolor, icolor, off[:nc + 1].copy()
+
+
+def reduce_colors(nei_ele: np.ndarray, lcolor: np.ndarray, max_chain: int = 4096):
+    """Try to remove the highest colour by Kempe-chain swaps
+    (ucfd_reduce_colors); returns (ncolors, lcolor, icolor, color_off)."""
+    nei = np.ascontiguousarray(nei_ele, np.int32)
+    F, N = nei.shape
+    lc = np.array(lcolor, np.int32)
+    icolor = np.empty(N, np.int32)
+    off = np.empty(int(lc.max()) + 2 if N else 2, np.int32)
+    nc = L.load().ucfd_reduce_colors(N, F, nei.ctypes.data, int(max_chain), icolor.ctypes.data,
+                                     lc.ctypes.data, off.ctypes.data)
+    if nc <= 0 and N > 0:
+        raise ValueError("colour reduction failed")
+    return nc, lc, icolor, off[:nc + 1].copy()
+
+
 def balanced_coloring(nei_ele: np.ndarray, ncolors: int = 0):
     """Balanced colouring (ucfd_balanced_coloring); returns (ncolors, lcolor,
     icolor, color_off)."""
@@ -94,10 +126,18 @@ class ColoredLayout:
 
 def colored_layout(mesh: Mesh, flow: Flow | None = None, use_key: bool = True,
                    coloring: str = "greedy", ncolors: int = 0) -> ColoredLayout:
-    """Colour the mesh (first-fit greedy, or balanced) and renumber it
-    colour-contiguously (locality-key order inside each colour when use_key)."""
+    """Colour the mesh and renumber it colour-contiguously (locality-key order
+    inside each colour when use_key).  coloring: "greedy" (first-fit in
+    ascending cell id), "greedy-key" (first-fit visiting cells in locality-key
+    order), "greedy-key-reduced" (the same, then the highest colour removed
+    by Kempe-chain swaps where possible) or "balanced"."""
     if coloring == "balanced":
         nc, lcolor, icolor, off = balanced_coloring(mesh.nei_ele, ncolors)
+    elif coloring in ("greedy-key", "greedy-key-reduced"):
+        order = np.argsort(np.asarray(mesh.key, np.uint64), kind="stable").astype(np.int32)
+        nc, lcolor, icolor, off = greedy_coloring_ordered(mesh.nei_ele, order)
+        if coloring == "greedy-key-reduced":
+            nc, lcolor, icolor, off = reduce_colors(mesh.nei_ele, lcolor)
     else:
         nc, lcolor, icolor, off = greedy_coloring(mesh.nei_ele)
     perm, off2 = color_reorder(lcolor, mesh.key if use_key else None)

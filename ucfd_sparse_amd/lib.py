@@ -49,6 +49,9 @@ _SIGS = {
     "ucfd_greedy_coloring": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "ucfd_coloring_conflicts": [c_int, c_int, c_void_p, c_void_p],
     "ucfd_balanced_coloring": [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p],
+    "ucfd_greedy_coloring_ordered": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p],
+    "ucfd_reduce_colors": [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_color_reorder": [c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_rows": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_nei": [c_int, c_int, c_void_p, c_void_p, c_void_p],
