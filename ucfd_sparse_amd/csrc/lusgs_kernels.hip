@@ -42,6 +42,42 @@ __global__ __launch_bounds__(kBlock) void k_pre(int N, int nface, double factor,
     diag[i] = d + sd;
 }
 
+// The same for a compile-time face count, CPT cells per thread (cells
+// t, t + S, ..., S = threads in the grid): every load of a thread is issued
+// before the first sum, so more bytes are in flight per wave.
+template <int FC, int CPT>
+__global__ __launch_bounds__(kBlock) void k_pre_f(int N, double factor,
+                                                  const double *__restrict__ fnv,
+                                                  const double *__restrict__ dt,
+                                                  double *__restrict__ diag,
+                                                  const double *__restrict__ fspr)
+{
+    const int S = gridDim.x*kBlock;
+    const int i0 = blockIdx.x*kBlock + threadIdx.x;
+    double t[CPT], a[CPT][FC], b[CPT][FC];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int i = i0 + c*S;
+        if (i >= N) continue;
+        t[c] = __builtin_nontemporal_load(dt + i);
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            a[c][f] = __builtin_nontemporal_load(fspr + (size_t)f*N + i);
+            b[c][f] = __builtin_nontemporal_load(fnv + (size_t)f*N + i);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int i = i0 + c*S;
+        if (i >= N) continue;
+        const double d = 1.0/(t[c]*factor) + 0.0 + 0.0;
+        double sd = 0.0;
+#pragma unroll
+        for (int f = 0; f < FC; ++f) sd += 0.5*a[c][f]*b[c][f];
+        __builtin_nontemporal_store(d + sd, diag + i);
+    }
+}
+
 // Workgroups are dealt round-robin over the 8 XCDs of the MI355X, each with
 // its own L2.  Remapping the hardware workgroup id so that each XCD owns one
 // contiguous eighth of the launched cell range keeps the neighbour rows that
@@ -276,8 +312,17 @@ int ucfd_launch_pre(hipStream_t s, int neles, int nface, double factor, const do
                     const double *dt, double *diag, const double *fspr)
 {
     if (neles <= 0) return UCFD_OK;
-    hipLaunchKernelGGL(k_pre, dim3((neles + kBlock - 1)/kBlock), dim3(kBlock), 0, s, neles,
-                       nface, factor, fnorm_vol, dt, diag, fspr);
+    constexpr int CPT = 2;
+    const dim3 g2((unsigned)((neles + CPT*kBlock - 1)/(CPT*kBlock)));
+    if (nface == 4)
+        hipLaunchKernelGGL((k_pre_f<4, CPT>), g2, dim3(kBlock), 0, s, neles, factor, fnorm_vol, dt,
+                           diag, fspr);
+    else if (nface == 6)
+        hipLaunchKernelGGL((k_pre_f<6, CPT>), g2, dim3(kBlock), 0, s, neles, factor, fnorm_vol, dt,
+                           diag, fspr);
+    else
+        hipLaunchKernelGGL(k_pre, dim3((neles + kBlock - 1)/kBlock), dim3(kBlock), 0, s, neles,
+                           nface, factor, fnorm_vol, dt, diag, fspr);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 

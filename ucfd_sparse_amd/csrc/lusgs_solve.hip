@@ -36,6 +36,7 @@
 namespace {
 
 constexpr int kB = 256;
+constexpr int kSB = 128;    // sweep and residual kernels
 constexpr int NV = 5;
 
 template <typename T> struct alignas(4*sizeof(T)) Face4 {
@@ -107,33 +108,38 @@ __device__ __forceinline__ void face_term(const T *U, const T *d, const Face4<T>
     out[4] = (d[4] + dp)*thp + (U[4] + p)*dth - g.l*d[4];
 }
 
-// block sum of v (FP64) written to part[blockIdx]; deterministic order
+// block sum of v (FP64) written to part[slot]; deterministic order
+template <int BS>
 __device__ __forceinline__ void block_sum(double v, double *part, int slot)
 {
-    __shared__ double red[kB/64];
+    __shared__ double red[BS/64];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int w = threadIdx.x/64, l = threadIdx.x%64;
     if (l == 0) red[w] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int q = 0; q < kB/64; ++q) s += red[q];
+        for (int q = 0; q < BS/64; ++q) s += red[q];
         part[slot] = s;
     }
 }
+
+// Every kernel below issues all loads a thread needs (its own rows, the face
+// records and the neighbour states of every face it uses) before the first
+// arithmetic, so the gathers of all faces are in flight together.
 
 // ---- lower sweep of colour [n0, n0+cnt), with the defect-correction residual
 //   s_i = rhs_i - A(e)_i        (WITH_RES; otherwise s_i = rhs_i, e = 0)
 //   dstar_i = (s_i - 1/2 sum_{nb < n0} term(Qb_nb; dstar_nb)) / D_i
 // Writes sum_i |s_i|^2 of this launch's blocks to part[pofs + block].
 template <typename T, int FC, bool WITH_RES>
-__global__ __launch_bounds__(kB) void k_lower_res(
+__global__ __launch_bounds__(kSB) void k_lower_res(
     int n0, int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
     const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ rhs,
     const T *__restrict__ e, T *dstar, double *part, int pofs)
 {
     const int b = xcd_remap();
-    const int t = b*kB + threadIdx.x;
+    const int t = b*kSB + threadIdx.x;
     double ss = 0.0;
     if (t < cnt) {
         const int i = n0 + t;
@@ -141,40 +147,54 @@ __global__ __launch_bounds__(kB) void k_lower_res(
 #pragma unroll
         for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
         const T dg = ldg_nt(diag + i);
-        T s[NV], acc[NV];
+        T s[NV], ei[NV];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) { s[q] = ldg_nt(rhs + (size_t)q*N + i); acc[q] = T(0); }
+        for (int q = 0; q < NV; ++q) {
+            s[q] = ldg_nt(rhs + (size_t)q*N + i);
+            if (WITH_RES) ei[q] = ldg_nt(e + (size_t)q*N + i);
+        }
+        bool res_f[FC], low_f[FC];
+        Face4<T> g[FC];
+        T U[FC][NV], dr[FC][NV], dl[FC][NV];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const int nb = nbv[f];
+            res_f[f] = WITH_RES && nb != i;
+            low_f[f] = nb < n0;
+            if (!(res_f[f] || low_f[f])) continue;
+            g[f] = ldg_nt(geo + (size_t)f*N + i);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                U[f][q] = qb[(size_t)q*N + nb];
+                if (res_f[f]) dr[f][q] = e[(size_t)q*N + nb];
+                if (low_f[f]) dl[f][q] = dstar[(size_t)q*N + nb];
+            }
+        }
         if (WITH_RES) {
             T res[NV];
 #pragma unroll
             for (int q = 0; q < NV; ++q) res[q] = T(0);
 #pragma unroll
             for (int f = 0; f < FC; ++f) {
-                const int nb = nbv[f];
-                if (nb == i) continue;
-                const Face4<T> g = ldg_nt(geo + (size_t)f*N + i);
-                T U[NV], d[NV], o[NV];
-#pragma unroll
-                for (int q = 0; q < NV; ++q) { U[q] = qb[(size_t)q*N + nb]; d[q] = e[(size_t)q*N + nb]; }
-                face_term<T>(U, d, g, o);
+                if (!res_f[f]) continue;
+                T o[NV];
+                face_term<T>(U[f], dr[f], g[f], o);
 #pragma unroll
                 for (int q = 0; q < NV; ++q) res[q] += o[q];
             }
 #pragma unroll
-            for (int q = 0; q < NV; ++q)
-                s[q] = s[q] - (dg*ldg_nt(e + (size_t)q*N + i) + T(0.5)*res[q]);
+            for (int q = 0; q < NV; ++q) s[q] = s[q] - (dg*ei[q] + T(0.5)*res[q]);
         }
 #pragma unroll
         for (int q = 0; q < NV; ++q) ss += (double)s[q]*(double)s[q];
+        T acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] = T(0);
 #pragma unroll
         for (int f = 0; f < FC; ++f) {
-            const int nb = nbv[f];
-            if (nb >= n0) continue;
-            const Face4<T> g = ldg_nt(geo + (size_t)f*N + i);
-            T U[NV], d[NV], o[NV];
-#pragma unroll
-            for (int q = 0; q < NV; ++q) { U[q] = qb[(size_t)q*N + nb]; d[q] = dstar[(size_t)q*N + nb]; }
-            face_term<T>(U, d, g, o);
+            if (!low_f[f]) continue;
+            T o[NV];
+            face_term<T>(U[f], dl[f], g[f], o);
 #pragma unroll
             for (int q = 0; q < NV; ++q) acc[q] += o[q];
         }
@@ -182,91 +202,132 @@ __global__ __launch_bounds__(kB) void k_lower_res(
         for (int q = 0; q < NV; ++q)
             __builtin_nontemporal_store((s[q] - T(0.5)*acc[q])/dg, dstar + (size_t)q*N + i);
     }
-    block_sum(ss, part, pofs + b);
+    block_sum<kSB>(ss, part, pofs + b);
 }
 
 // ---- upper sweep of colour [n0, n0+cnt), then e_i += delta_i
 //   delta_i = dstar_i - 1/2 sum_{nb >= ne} term(Qb_nb; delta_nb) / D_i
 template <typename T, int FC>
-__global__ __launch_bounds__(kB) void k_upper_upd(
+__global__ __launch_bounds__(kSB) void k_upper_upd(
     int n0, int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
     const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ dstar,
     T *delta, T *e)
 {
-    const int t = xcd_remap()*kB + threadIdx.x;
+    const int t = xcd_remap()*kSB + threadIdx.x;
     if (t >= cnt) return;
     const int i = n0 + t, ne = n0 + cnt;
     int nbv[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
+    const T dg = ldg_nt(diag + i);
+    T ds[NV], ei[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        ds[q] = ldg_nt(dstar + (size_t)q*N + i);
+        ei[q] = ldg_nt(e + (size_t)q*N + i);
+    }
+    bool up_f[FC];
+    Face4<T> g[FC];
+    T U[FC][NV], d[FC][NV];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+        const int nb = nbv[f];
+        up_f[f] = nb >= ne;
+        if (!up_f[f]) continue;
+        g[f] = ldg_nt(geo + (size_t)f*N + i);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            U[f][q] = qb[(size_t)q*N + nb];
+            d[f][q] = delta[(size_t)q*N + nb];
+        }
+    }
     T acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = T(0);
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
-        const int nb = nbv[f];
-        if (nb < ne) continue;
-        const Face4<T> g = ldg_nt(geo + (size_t)f*N + i);
-        T U[NV], d[NV], o[NV];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) { U[q] = qb[(size_t)q*N + nb]; d[q] = delta[(size_t)q*N + nb]; }
-        face_term<T>(U, d, g, o);
+        if (!up_f[f]) continue;
+        T o[NV];
+        face_term<T>(U[f], d[f], g[f], o);
 #pragma unroll
         for (int q = 0; q < NV; ++q) acc[q] += o[q];
     }
-    const T dg = ldg_nt(diag + i);
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         const size_t o = (size_t)q*N + i;
-        const T dl = ldg_nt(dstar + o) - T(0.5)*acc[q]/dg;
+        const T dl = ds[q] - T(0.5)*acc[q]/dg;
         __builtin_nontemporal_store(dl, delta + o);
-        __builtin_nontemporal_store(ldg_nt(e + o) + dl, e + o);
+        __builtin_nontemporal_store(ei[q] + dl, e + o);
     }
 }
 
 // ---- FP64 residual r = b - A(dq) on base Q over all cells.  Optionally
 // writes fl32(r) and the FP32 base state fl32(Q + dq) for the inner loop.
+// Launched once per colour range [n0, n0+cnt) like the sweeps: all XCDs
+// then work on slices of one colour and its neighbours at a time.
 template <int FC, bool OUT32>
-__global__ __launch_bounds__(kB) void k_residual64(
-    int N, const int *__restrict__ nei, const Face4<double> *__restrict__ geo,
+__global__ __launch_bounds__(kSB) void k_residual64(
+    int n0, int cnt, int N, const int *__restrict__ nei, const Face4<double> *__restrict__ geo,
     const double *__restrict__ diag, const double *__restrict__ q, const double *__restrict__ b,
     const double *__restrict__ dq, double *__restrict__ r, float *__restrict__ r32,
-    float *__restrict__ qb32, double *part)
+    float *__restrict__ qb32, double *part, int pofs)
 {
-    const int blk = blockIdx.x;
-    const int i = blk*kB + threadIdx.x;
+    const int blk = xcd_remap();
+    const int t = blk*kSB + threadIdx.x;
+    const int i = n0 + t;
     double ss = 0.0;
-    if (i < N) {
+    if (t < cnt) {
+        int nbv[FC];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
+        const double dg = ldg_nt(diag + i);
+        double bi[NV], dqi[NV], qi[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const size_t o = (size_t)k*N + i;
+            bi[k] = ldg_nt(b + o);
+            dqi[k] = dq[o];
+            if (OUT32) qi[k] = q[o];
+        }
+        bool in_f[FC];
+        Face4<double> g[FC];
+        double U[FC][NV], d[FC][NV];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const int nb = nbv[f];
+            in_f[f] = nb != i;
+            if (!in_f[f]) continue;
+            g[f] = ldg_nt(geo + (size_t)f*N + i);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                U[f][k] = q[(size_t)k*N + nb];
+                d[f][k] = dq[(size_t)k*N + nb];
+            }
+        }
         double res[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) res[k] = 0.0;
 #pragma unroll
         for (int f = 0; f < FC; ++f) {
-            const int nb = ldg_nt(nei + (size_t)f*N + i);
-            if (nb == i) continue;
-            const Face4<double> g = ldg_nt(geo + (size_t)f*N + i);
-            double U[NV], d[NV], o[NV];
-#pragma unroll
-            for (int k = 0; k < NV; ++k) { U[k] = q[(size_t)k*N + nb]; d[k] = dq[(size_t)k*N + nb]; }
-            face_term<double>(U, d, g, o);
+            if (!in_f[f]) continue;
+            double o[NV];
+            face_term<double>(U[f], d[f], g[f], o);
 #pragma unroll
             for (int k = 0; k < NV; ++k) res[k] += o[k];
         }
-        const double dg = ldg_nt(diag + i);
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             const size_t o = (size_t)k*N + i;
-            const double dqi = dq[o];
-            const double rv = ldg_nt(b + o) - (dg*dqi + 0.5*res[k]);
+            const double rv = bi[k] - (dg*dqi[k] + 0.5*res[k]);
             ss += rv*rv;
             if (r) r[o] = rv;
             if (OUT32) {
                 r32[o] = (float)rv;
-                qb32[o] = (float)(q[o] + dqi);
+                qb32[o] = (float)(qi[k] + dqi[k]);
             }
         }
     }
-    block_sum(ss, part, blk);
+    block_sum<kSB>(ss, part, pofs + blk);
 }
 
 // Sum of the block partials in a fixed order (thread t sums t, t+1024, ...,
@@ -380,10 +441,10 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
     for (int c = 0; c < s->nc; ++c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
-        const dim3 g((cnt + kB - 1)/kB);
-#define L4(RES) hipLaunchKernelGGL((k_lower_res<T, 4, RES>), g, dim3(kB), 0, st, n0, cnt, s->N, \
+        const dim3 g((cnt + kSB - 1)/kSB);
+#define L4(RES) hipLaunchKernelGGL((k_lower_res<T, 4, RES>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
                                    s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
-#define L6(RES) hipLaunchKernelGGL((k_lower_res<T, 6, RES>), g, dim3(kB), 0, st, n0, cnt, s->N, \
+#define L6(RES) hipLaunchKernelGGL((k_lower_res<T, 6, RES>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
                                    s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
         if (s->F == 4) { if (with_res) L4(true); else L4(false); }
         else           { if (with_res) L6(true); else L6(false); }
@@ -402,12 +463,12 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
     for (int c = s->nc - 1; c >= 0; --c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
-        const dim3 g((cnt + kB - 1)/kB);
+        const dim3 g((cnt + kSB - 1)/kSB);
         if (s->F == 4)
-            hipLaunchKernelGGL((k_upper_upd<T, 4>), g, dim3(kB), 0, st, n0, cnt, s->N, s->nei, geo,
+            hipLaunchKernelGGL((k_upper_upd<T, 4>), g, dim3(kSB), 0, st, n0, cnt, s->N, s->nei, geo,
                                diag, qb, dstar, delta, e);
         else
-            hipLaunchKernelGGL((k_upper_upd<T, 6>), g, dim3(kB), 0, st, n0, cnt, s->N, s->nei, geo,
+            hipLaunchKernelGGL((k_upper_upd<T, 6>), g, dim3(kSB), 0, st, n0, cnt, s->N, s->nei, geo,
                                diag, qb, dstar, delta, e);
     }
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
@@ -416,13 +477,19 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
 int residual64(ucfd_lusgs_solver *s, hipStream_t st, const double *b, const double *dq, double *r,
                bool out32)
 {
-    const dim3 g((s->N + kB - 1)/kB);
-#define R(FC, O) hipLaunchKernelGGL((k_residual64<FC, O>), g, dim3(kB), 0, st, s->N, s->nei, s->g64, \
-                                    s->d64, s->q64, b, dq, r, s->r32, s->qb32, s->part)
-    if (s->F == 4) { if (out32) R(4, true); else R(4, false); }
-    else           { if (out32) R(6, true); else R(6, false); }
+    int pofs = 0;
+    for (int c = 0; c < s->nc; ++c) {
+        const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
+        if (cnt <= 0) continue;
+        const dim3 g((cnt + kSB - 1)/kSB);
+#define R(FC, O) hipLaunchKernelGGL((k_residual64<FC, O>), g, dim3(kSB), 0, st, n0, cnt, s->N, s->nei, \
+                                    s->g64, s->d64, s->q64, b, dq, r, s->r32, s->qb32, s->part, pofs)
+        if (s->F == 4) { if (out32) R(4, true); else R(4, false); }
+        else           { if (out32) R(6, true); else R(6, false); }
 #undef R
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, (int)g.x, s->norm);
+        pofs += (int)g.x;
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
@@ -479,8 +546,8 @@ UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nf
     s->nc = ncolors;
     s->off.assign(color_off, color_off + ncolors + 1);
     int maxparts = 0;
-    for (int c = 0; c < ncolors; ++c) maxparts += (s->off[c + 1] - s->off[c] + kB - 1)/kB;
-    s->nparts = std::max(maxparts, (neles + kB - 1)/kB) + 8;
+    for (int c = 0; c < ncolors; ++c) maxparts += (s->off[c + 1] - s->off[c] + kSB - 1)/kSB;
+    s->nparts = std::max(maxparts, (neles + kSB - 1)/kSB) + 8;
     const hipStream_t st = ucfd_stream();
     double *tfnv = nullptr, *tvfn = nullptr, *tsp = nullptr, *tdt = nullptr;
     if ((rc = dalloc(s, &s->nei, F*N, fn)) || (rc = dalloc(s, &s->g64, F*N, fn)) ||
