@@ -330,14 +330,27 @@ __global__ __launch_bounds__(kSB) void k_residual64(
     block_sum<kSB>(ss, part, pofs + blk);
 }
 
-// Sum of the block partials in a fixed order (thread t sums t, t+1024, ...,
-// then a fixed tree), so the norm and every convergence decision are
-// reproducible run to run.
-__global__ __launch_bounds__(1024) void k_sum_parts(const double *part, int n, double *out)
+// Sum of the block partials in a fixed order (thread t sums t, t+1024, ...
+// with eight loads in flight, then a fixed tree), so the norm and every
+// convergence decision are reproducible run to run.
+// The result goes to out (device) and to hout (pinned host memory, read by
+// the host after a stream synchronisation: no copy kernel).
+__global__ __launch_bounds__(1024) void k_sum_parts(const double *part, int n, double *out,
+                                                    double *hout)
 {
+    constexpr int U = 8;
     __shared__ double red[16];
     double s = 0.0;
-    for (int j = threadIdx.x; j < n; j += 1024) s += part[j];
+    for (int j0 = threadIdx.x; j0 < n; j0 += 1024*U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u*1024;
+            v[u] = j < n ? part[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += v[u];
+    }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (threadIdx.x % 64 == 0) red[threadIdx.x/64] = s;
     __syncthreads();
@@ -345,6 +358,7 @@ __global__ __launch_bounds__(1024) void k_sum_parts(const double *part, int n, d
         double t = 0.0;
         for (int w = 0; w < 16; ++w) t += red[w];
         *out = t;
+        if (hout) *hout = t;
     }
 }
 
@@ -406,6 +420,7 @@ struct ucfd_lusgs_solver {
     float *d32 = nullptr, *qb32 = nullptr, *r32 = nullptr, *e32 = nullptr, *dstar32 = nullptr,
           *delta32 = nullptr;
     double *part = nullptr, *norm = nullptr, *hnorm = nullptr;
+    double *hnorm_dev = nullptr;   // device address of the pinned hnorm
     int nparts = 0;
     std::vector<void *> allocs;
 };
@@ -452,7 +467,8 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
 #undef L6
         pofs += (int)g.x;
     }
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm,
+                       s->hnorm_dev);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
@@ -489,15 +505,15 @@ int residual64(ucfd_lusgs_solver *s, hipStream_t st, const double *b, const doub
 #undef R
         pofs += (int)g.x;
     }
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm,
+                       s->hnorm_dev);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
 int read_norm(ucfd_lusgs_solver *s, hipStream_t st, double *out, const char *fn)
 {
-    int rc = ucfd_check_hip(hipMemcpyAsync(s->hnorm, s->norm, sizeof(double), hipMemcpyDeviceToHost,
-                                           st), fn, "hipMemcpyAsync(norm)");
-    if (!rc) rc = ucfd_check_hip(hipStreamSynchronize(st), fn, "hipStreamSynchronize");
+    // the last k_sum_parts wrote the norm into the pinned hnorm
+    int rc = ucfd_check_hip(hipStreamSynchronize(st), fn, "hipStreamSynchronize");
     *out = sqrt(*s->hnorm);
     return rc;
 }
@@ -565,7 +581,9 @@ UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nf
         return rc;
     }
     if ((rc = ucfd_check_hip(hipHostMalloc((void **)&s->hnorm, 64, hipHostMallocDefault), fn,
-                             "hipHostMalloc"))) {
+                             "hipHostMalloc")) ||
+        (rc = ucfd_check_hip(hipHostGetDevicePointer((void **)&s->hnorm_dev, s->hnorm, 0), fn,
+                             "hipHostGetDevicePointer"))) {
         free_all(s);
         delete s;
         return rc;
