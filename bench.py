@@ -444,11 +444,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--coloring", choices=("greedy", "greedy-key", "greedy-key-reduced",
-                                           "balanced"),
-                    default="greedy-key-reduced",
+                                           "greedy-bfs", "balanced"),
+                    default="greedy-bfs",
                     help="colouring of the headline mesh: first-fit in cell id "
                          "(ucfd_greedy_coloring), first-fit along the Hilbert curve "
-                         "(ucfd_greedy_coloring_ordered) or balanced")
+                         "(ucfd_greedy_coloring_ordered, -reduced: then Kempe-chain "
+                         "reduction), first-fit in breadth-first order (ucfd_bfs_order) "
+                         "or balanced")
     ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint"), default="lusgs",
                     help="lusgs: the headline colored LU-SGS line (default); krylov: the "
                          "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system; adjoint: "
@@ -528,7 +530,7 @@ def main():
     torch.cuda.synchronize()
     L.call("ucfd_set_stream", stream.cuda_stream)
 
-    def step(ev=None):
+    def step(ev=None, cev=None):
         if ev:
             ev[0].record(stream)
         lib.parallel_pre_lusgs(N, F, 1.0, P["fnv"], P["dt"], P["diag"], P["fsp"])
@@ -538,12 +540,16 @@ def main():
             lib.ns_parallel_lower_sweep(off[c], off[c + 1], N, F, P["nei"], P["ic"], P["lc"],
                                         P["fnv"], P["vfn"], P["upts"], P["rhs"], P["dstar"],
                                         P["diag"], P["fsp"])
+            if cev:
+                cev[0][c].record(stream)
         if ev:
             ev[2].record(stream)
         for c in reversed(range(nc)):
             lib.ns_parallel_upper_sweep(off[c], off[c + 1], N, F, P["nei"], P["ic"], P["lc"],
                                         P["fnv"], P["vfn"], P["upts"], P["dq"], P["dstar"],
                                         P["diag"], P["fsp"])
+            if cev:
+                cev[1][c].record(stream)
         if ev:
             ev[3].record(stream)
         lib.lusgs_parallel_ns_update(N, P["upts"], P["dq"])
@@ -563,7 +569,21 @@ def main():
     t_end.record(stream)
     L.call("ucfd_synchronize")
     torch.cuda.synchronize()
+    # per-colour launch times from separate instrumented steps (events
+    # between every launch), so the timed steps above carry no extra markers
+    reps = 3
+    cevs = [[[torch.cuda.Event(enable_timing=True) for _ in range(nc)] for _ in range(2)]
+            for _ in range(reps)]
+    pevs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(reps)]
+    for r in range(reps):
+        step(pevs[r], cevs[r])
+    L.call("ucfd_synchronize")
+    torch.cuda.synchronize()
     L.call("ucfd_set_stream", None)
+    lower_c = [sum((pevs[r][1] if c == 0 else cevs[r][0][c - 1]).elapsed_time(cevs[r][0][c])
+                   for r in range(reps)) / reps for c in range(nc)]
+    upper_c = [sum((pevs[r][2] if c == nc - 1 else cevs[r][1][c + 1]).elapsed_time(cevs[r][1][c])
+                   for r in range(reps)) / reps for c in range(nc)]
     total_ms = t_start.elapsed_time(t_end)
     ms_step = total_ms / args.steps
     pre_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
@@ -572,6 +592,8 @@ def main():
     upd_ms = sum(e[3].elapsed_time(e[4]) for e in evs) / args.steps
     log(f"per step {ms_step:.3f} ms: pre {pre_ms:.3f} lower {low_ms:.3f} upper {upp_ms:.3f} "
         f"update {upd_ms:.3f}")
+    log("launch ms per colour: lower " + " ".join(f"{x:.3f}" for x in lower_c)
+        + " | upper " + " ".join(f"{x:.3f}" for x in upper_c))
 
     b_pre, b_sweep, b_upd = algorithmic_bytes(F)
     bytes_cell = b_pre + 2 * b_sweep + b_upd
@@ -583,7 +605,9 @@ def main():
     achieved = lower_bytes_launch / (lower_launch_ms * 1e-3) / 1e9
     traffic = None
     tr = read_traffic(os.path.join(REPO, "profiles", "traffic_lower_sweep_tet96.txt"))
-    if tr and "hbm_bytes_per_launch" in tr:
+    if (tr and "hbm_bytes_per_launch" in tr and tr.get("m") == args.m
+            and tr.get("ncolors") == nc and tr.get("seed") == args.seed
+            and args.coloring == "greedy-bfs"):
         traffic = tr["hbm_bytes_per_launch"]
 
     mixed = None
@@ -621,6 +645,7 @@ def main():
         "hbm_gbs_algorithmic": gbs,
         "bytes_per_cell_iter": bytes_cell,
         "phase_ms": {"pre": pre_ms, "lower": low_ms, "upper": upp_ms, "update": upd_ms},
+        "sweep_launch_ms": {"lower": lower_c, "upper": upper_c},
         "roofline": {
             "kernel": "lower sweep (k_sweep<0,2,3,4>), one launch per colour",
             "bound": "hbm",
@@ -629,6 +654,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "hbm_gbs_from_traffic": (traffic / (lower_launch_ms * 1e-3) / 1e9
+                                     if traffic else None),
             "algorithmic_bytes_per_launch": lower_bytes_launch,
             "avg_launch_ms": lower_launch_ms,
         },

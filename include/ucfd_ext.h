@@ -49,6 +49,11 @@ int ucfd_greedy_coloring(int neles, int nface, const int *nei_ele, int *icolor,
  * ncolors, or < 0 on bad input (order not a permutation). */
 int ucfd_greedy_coloring_ordered(int neles, int nface, const int *nei_ele, const int *order,
                                  int *icolor, int *lcolor, int *color_off);
+/* Breadth-first visiting order of the face graph (no counterpart in the
+ * reference), components entered at their first cell in seeds (NULL:
+ * ascending id).  Greedy colouring in this order 2-colours any bipartite
+ * face graph.  Returns the number of components, or -1 on bad input. */
+int ucfd_bfs_order(int neles, int nface, const int *nei_ele, const int *seeds, int *order);
 /* Removes the highest colour where possible (no counterpart in the
  * reference): each of its cells takes a colour no neighbour uses, or one
  * freed by swapping a two-colour Kempe chain of at most max_chain cells.

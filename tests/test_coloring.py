@@ -209,3 +209,74 @@ def test_reduce_colors_removes_the_top_colour():
                                                     fromlist=["x"]).greedy_coloring(hexm.nei_ele))
     nr, lr, _, _ = reduce_colors(hexm.nei_ele, lch)
     assert nch == 2 and nr == 2
+
+
+def _bfs_dist(nei, seeds):
+    """Independent breadth-first distances (and component roots) of the
+    symmetrised face graph; distances do not depend on neighbour order."""
+    from collections import deque
+    F, N = nei.shape
+    adj = [set() for _ in range(N)]
+    for f in range(F):
+        for i in range(N):
+            j = int(nei[f, i])
+            if j != i:
+                adj[i].add(j)
+                adj[j].add(i)
+    dist = -np.ones(N, np.int64)
+    root = -np.ones(N, np.int64)
+    for s in seeds:
+        if dist[s] >= 0:
+            continue
+        dist[s], root[s] = 0, s
+        q = deque([s])
+        while q:
+            i = q.popleft()
+            for j in adj[i]:
+                if dist[j] < 0:
+                    dist[j], root[j] = dist[i] + 1, s
+                    q.append(j)
+    return dist, root
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_bfs_order_and_two_colouring(kind):
+    from ucfd_sparse_amd.layout import bfs_order, colored_layout, greedy_coloring_ordered
+    mesh = M.tet_mesh(6, seed=4) if kind == "tet" else M.hex_mesh(7, 6, 5, seed=4)
+    nei = np.ascontiguousarray(mesh.nei_ele)
+    F, N = nei.shape
+    seeds = np.random.default_rng(5).permutation(N).astype(np.int32)
+    order = bfs_order(nei, seeds)
+    assert np.array_equal(np.sort(order), np.arange(N))
+    dist, root = _bfs_dist(nei, seeds)
+    # components are contiguous blocks entered at their root, distances
+    # non-decreasing inside each block: a breadth-first order
+    d, r = dist[order], root[order]
+    starts = np.flatnonzero(np.r_[True, r[1:] != r[:-1]])
+    assert np.array_equal(order[starts], r[starts]) and len(set(r[starts])) == len(starts)
+    for a, b in zip(starts, list(starts[1:]) + [N]):
+        assert np.all(np.diff(d[a:b]) >= 0)
+    # the Kuhn split and the hex grid are bipartite: first-fit in this order
+    # gives exactly two conflict-free colours
+    nc, lc, ic, off = greedy_coloring_ordered(nei, order)
+    assert nc == 2
+    assert L.load().ucfd_coloring_conflicts(N, F, nei.ctypes.data, lc.ctypes.data) == 0
+    lay = colored_layout(mesh, None, coloring="greedy-bfs")
+    assert lay.ncolors == 2 and abs(int(np.diff(lay.off)[0]) - N // 2) <= N % 2 + 1
+    bad = seeds.copy()
+    bad[0] = bad[1]
+    with pytest.raises(ValueError):
+        bfs_order(nei, bad)
+
+
+def test_bfs_greedy_on_an_odd_cycle_needs_three_colours():
+    from ucfd_sparse_amd.layout import bfs_order, greedy_coloring_ordered
+    # 5-cycle plus a boundary self-face column: not bipartite
+    N = 5
+    nei = np.array([[(i + 1) % N for i in range(N)], [(i - 1) % N for i in range(N)],
+                    list(range(N))], np.int32)
+    order = bfs_order(nei)
+    assert order[0] == 0 and np.array_equal(np.sort(order), np.arange(N))
+    nc, lc, _, _ = greedy_coloring_ordered(nei, order)
+    assert nc == 3
+    assert L.load().ucfd_coloring_conflicts(N, 3, nei.ctypes.data, lc.ctypes.data) == 0

@@ -119,6 +119,48 @@ UCFD_API int ucfd_greedy_coloring_ordered(int neles, int nface, const int *nei_e
     return greedy(neles, nface, nei_ele, order, icolor, lcolor, color_off);
 }
 
+// Breadth-first visiting order of the face graph: components are entered at
+// their first cell in seeds[0..neles) (ascending id when seeds is NULL) and
+// each cell's neighbours are queued in face order.  First-fit colouring in
+// this order gives every cell of BFS layer k a neighbour in layer k-1, so on
+// a bipartite face graph (structured hex grids, Kuhn/Freudenthal tet splits)
+// it returns the 2-colouring; on other meshes it is one more greedy order.
+// Returns the number of connected components, or -1 on bad input.
+UCFD_API int ucfd_bfs_order(int neles, int nface, const int *nei_ele, const int *seeds,
+                            int *order)
+{
+    if (neles < 0 || nface < 1 || !nei_ele || !order) return -1;
+    if (seeds) {
+        std::vector<char> seen(neles, 0);
+        for (int k = 0; k < neles; ++k) {
+            if (seeds[k] < 0 || seeds[k] >= neles || seen[seeds[k]]) return -1;
+            seen[seeds[k]] = 1;
+        }
+    }
+    std::vector<int> ptr, adj;
+    sym_graph(neles, nface, nei_ele, ptr, adj);
+    std::vector<char> vis(neles, 0);
+    int head = 0, tail = 0, ncomp = 0;
+    for (int k = 0; k < neles; ++k) {
+        const int s = seeds ? seeds[k] : k;
+        if (vis[s]) continue;
+        ++ncomp;
+        vis[s] = 1;
+        order[tail++] = s;
+        while (head < tail) {
+            const int i = order[head++];
+            for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+                const int nb = adj[e];
+                if (!vis[nb]) {
+                    vis[nb] = 1;
+                    order[tail++] = nb;
+                }
+            }
+        }
+    }
+    return ncomp;
+}
+
 // Balanced colouring: cells in ascending id take, among the colours 1..C not
 // used by an already coloured face neighbour, the one holding the fewest
 // cells so far (ties: the smallest colour).  C = max(ncolors, maximum degree

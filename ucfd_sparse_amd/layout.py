@@ -49,6 +49,20 @@ def greedy_coloring_ordered(nei_ele: np.ndarray, order: np.ndarray):
     return nc, lcolor, icolor, off[:nc + 1].copy()
 
 
+def bfs_order(nei_ele: np.ndarray, seeds: np.ndarray | None = None) -> np.ndarray:
+    """Breadth-first visiting order of the face graph (ucfd_bfs_order),
+    components entered at their first cell in `seeds` (ascending id if None)."""
+    nei = np.ascontiguousarray(nei_ele, np.int32)
+    F, N = nei.shape
+    order = np.empty(N, np.int32)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.int32)
+    rc = L.load().ucfd_bfs_order(N, F, nei.ctypes.data, None if sd is None else sd.ctypes.data,
+                                 order.ctypes.data)
+    if rc < 0:
+        raise ValueError("ucfd_bfs_order: bad input")
+    return order
+
+
 def reduce_colors(nei_ele: np.ndarray, lcolor: np.ndarray, max_chain: int = 4096):
     """Try to remove the highest colour by Kempe-chain swaps
     (ucfd_reduce_colors); returns (ncolors, lcolor, icolor, color_off)."""
@@ -130,9 +144,17 @@ def colored_layout(mesh: Mesh, flow: Flow | None = None, use_key: bool = True,
     inside each colour when use_key).  coloring: "greedy" (first-fit in
     ascending cell id), "greedy-key" (first-fit visiting cells in locality-key
     order), "greedy-key-reduced" (the same, then the highest colour removed
-    by Kempe-chain swaps where possible) or "balanced"."""
+    by Kempe-chain swaps where possible), "greedy-bfs" (first-fit in
+    breadth-first order from the lowest locality key: the 2-colouring of a
+    bipartite face graph, then Kempe-chain reduction) or "balanced"."""
     if coloring == "balanced":
         nc, lcolor, icolor, off = balanced_coloring(mesh.nei_ele, ncolors)
+    elif coloring == "greedy-bfs":
+        seeds = np.argsort(np.asarray(mesh.key, np.uint64), kind="stable").astype(np.int32)
+        order = bfs_order(mesh.nei_ele, seeds)
+        nc, lcolor, icolor, off = greedy_coloring_ordered(mesh.nei_ele, order)
+        if nc > 2:
+            nc, lcolor, icolor, off = reduce_colors(mesh.nei_ele, lcolor)
     elif coloring in ("greedy-key", "greedy-key-reduced"):
         order = np.argsort(np.asarray(mesh.key, np.uint64), kind="stable").astype(np.int32)
         nc, lcolor, icolor, off = greedy_coloring_ordered(mesh.nei_ele, order)

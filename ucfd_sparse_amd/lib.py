@@ -51,6 +51,7 @@ _SIGS = {
     "ucfd_balanced_coloring": [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_greedy_coloring_ordered": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p],
+    "ucfd_bfs_order": [c_int, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_reduce_colors": [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p],
     "ucfd_color_reorder": [c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "ucfd_permute_rows": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
