@@ -174,7 +174,7 @@ def krylov_workload(args, log):
     mesh = M.tet_mesh(m, seed=seed)
     flow = M.flow_fields(mesh, seed=seed)
     mesh.key = M.hilbert_keys(mesh, "tet", (m,))
-    lay = colored_layout(mesh, flow)
+    lay = colored_layout(mesh, flow, coloring=args.coloring)
     A = M.block_system(lay.mesh, lay.flow, blk=5, seed=seed)
     n = A.bn * A.blk
     nnz = int(A.rowptr[-1])
@@ -277,6 +277,211 @@ def krylov_workload(args, log):
     print(json.dumps(rec), flush=True)
 
 
+def flowsys_bytes(npos, N, nv, fp32):
+    """Algorithmic bytes of one FlowSys block sweep (all levels) with nv x nv
+    blocks: per position the column index, the packed neighbour block, fv and
+    the neighbour's du row (FP32 copies in the FP32 form); per cell rowptr,
+    its rhs row, its FP64 diagonal LU and the du row written (plus the FP32
+    shadow row in the FP32 form)."""
+    vs = 4 if fp32 else 8
+    per_pos = 4 + vs * nv * nv + vs + vs * nv
+    per_cell = 4 + 8 * nv + 8 * nv * nv + 8 * nv + (4 * nv if fp32 else 0)
+    return npos * per_pos + N * per_cell + 4
+
+
+def blusgs_workload(args, log):
+    """Block LU-SGS at scale: the FlowSys API (NS 5x5 blocks plus the 1x1
+    Spalart-Allmaras turbulence system) on the 96^3 x 6 tet mesh, FP64
+    factors with FP64 sweeps vs FP32 sweep products, and the Krylov-path
+    coloured block LU-SGS apply on the BSR block Jacobian of the same mesh
+    (FP64 vs FP32 apply).  All arrays device-resident; colour-contiguous
+    layout, so each sweep is one launch per colour."""
+    import numpy as np
+    import torch
+
+    from ucfd_sparse_amd import flowsys as FS
+    from ucfd_sparse_amd import krylov as K
+    from ucfd_sparse_amd import lib as L
+    from ucfd_sparse_amd import mesh as M
+    from ucfd_sparse_amd.layout import colored_layout
+    from ucfd_sparse_amd.mesh import FlowSysMesh
+
+    t0 = time.time()
+    mesh = M.tet_mesh(args.m, seed=args.seed)
+    flow = M.flow_fields(mesh, seed=args.seed)
+    mesh.key = M.hilbert_keys(mesh, "tet", (args.m,))
+    lay = colored_layout(mesh, flow, coloring=args.coloring)
+    del mesh, flow
+    m, f = lay.mesh, lay.flow
+    N, nc = m.neles, lay.ncolors
+    fs = M.flowsys_system(m, f, nv=5, seed=args.seed)
+    tj = M.flowsys_system(m, f, nv=1, seed=args.seed).jmat
+    el = M.flowsys_elements(m, f, nvars=6, seed=args.seed)
+    npos = int(fs.rowptr[-1])
+    log(f"blusgs: {N} cells, {npos} positions, {nc} colours, setup {time.time() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fsd = FlowSysMesh(fs.nlocal, fs.nfaces, up(fs.rowptr), up(fs.colidx), up(fs.slots),
+                      up(fs.sides), up(fs.face_area), up(fs.face_normal), up(fs.rcp_vol),
+                      up(fs.jmat))
+    tjd = up(tj)
+    del fs, tj
+    stream = torch.cuda.Stream(device=dev)
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    flows = {}
+    du_after = {}
+    for prec in ("fp64", "fp32"):
+        fp32 = prec == "fp32"
+        S = FS.FlowSys(fsd, 6, 5, tjmat=tjd, fp32=fp32)
+        S.set_element(0, {k: up(v) for k, v in el.items()})
+        torch.cuda.synchronize()
+        L.call("ucfd_set_stream", stream.cuda_stream)
+        e = [ev() for _ in range(3)]
+        e[0].record(stream)
+        S.call("UCFDBLUSGS_Pack", 0, 0.3)
+        S.call("UCFDBLUSGS_SA_Pack", 0, 1.5, 0.3)
+        S.call("UCFDBLUSGS_NSPrepare")
+        e[1].record(stream)
+        S.call("UCFDBLUSGS_RANSPrepare")
+        e[2].record(stream)
+        L.call("ucfd_synchronize")
+        prep = {"ns_pack_prepare_ms": e[0].elapsed_time(e[1]),
+                "rans_prepare_ms": e[1].elapsed_time(e[2])}
+        S.call("UCFDBLUSGS_Reset")
+
+        def it(evs=None):
+            if evs:
+                evs[0].record(stream)
+            S.call("UCFDBLUSGS_NSLowerSweep")
+            if evs:
+                evs[1].record(stream)
+            S.call("UCFDBLUSGS_NSUpperSweep")
+            if evs:
+                evs[2].record(stream)
+            S.call("UCFDBLUSGS_RANSLowerSweep")
+            S.call("UCFDBLUSGS_RANSUpperSweep")
+            if evs:
+                evs[3].record(stream)
+        it()
+        L.call("ucfd_synchronize")
+        du_after[(prec, 1)] = S.export(FS.DU)
+        for _ in range(4):
+            it()
+        L.call("ucfd_synchronize")
+        du_after[(prec, 5)] = S.export(FS.DU)
+        reps = args.steps
+        evs = [[ev() for _ in range(4)] for _ in range(reps)]
+        for r in range(reps):
+            it(evs[r])
+        L.call("ucfd_synchronize")
+        torch.cuda.synchronize()
+        L.call("ucfd_set_stream", None)
+        lo = sum(x[0].elapsed_time(x[1]) for x in evs) / reps
+        upp = sum(x[1].elapsed_time(x[2]) for x in evs) / reps
+        rans = sum(x[2].elapsed_time(x[3]) for x in evs) / reps
+        S.close()
+        b_ns = flowsys_bytes(npos, N, 5, fp32)
+        b_rans = flowsys_bytes(npos, N, 1, fp32)
+        flows[prec] = dict(prep, **{
+            "ns_lower_ms": lo, "ns_upper_ms": upp, "rans_lower_upper_ms": rans,
+            "ns_cell_iters_per_s": N / ((lo + upp) * 1e-3),
+            "ns_sweep_algorithmic_bytes": b_ns,
+            "ns_sweep_gbs": b_ns / (0.5 * (lo + upp)) / 1e6,
+            "ns_launch_ms": 0.5 * (lo + upp) / nc,
+            "rans_sweep_algorithmic_bytes": b_rans,
+            "rans_sweep_gbs": 2 * b_rans / rans / 1e6})
+        log(f"flowsys {prec}: NS lower {lo:.3f} upper {upp:.3f} ms "
+            f"({flows[prec]['ns_sweep_gbs']:.0f} GB/s alg), RANS lower+upper {rans:.3f} ms, "
+            f"prepare {prep['ns_pack_prepare_ms']:.2f} + {prep['rans_prepare_ms']:.2f} ms")
+
+    def rel(a, b, sl):
+        a = a.reshape(N, 6)[:, sl]
+        b = b.reshape(N, 6)[:, sl]
+        return float(np.linalg.norm(a - b) / np.linalg.norm(a))
+    errs = {f"{part}_after_{k}": rel(du_after[("fp64", k)], du_after[("fp32", k)], sl)
+            for k in (1, 5) for part, sl in (("ns", slice(0, 5)), ("rans", slice(5, 6)))}
+    log(f"flowsys fp32 vs fp64 du: {errs}")
+    del fsd, tjd
+    torch.cuda.empty_cache()
+
+    # Krylov-path coloured block LU-SGS apply on the BSR block Jacobian
+    A = M.block_system(m, f, blk=5, seed=args.seed)
+    nnz = int(A.rowptr[-1])
+
+    class Dv:
+        pass
+    Dv.bn, Dv.blk = A.bn, A.blk
+    Dv.rowptr, Dv.colidx, Dv.diagslots, Dv.values = (up(A.rowptr), up(A.colidx),
+                                                     up(A.diagslots), up(A.values))
+    del A
+    n = Dv.bn * Dv.blk
+    b = up(M.normal_fill(n, 2024))
+    L.call("ucfd_set_stream", stream.cuda_stream)
+    applies = {}
+    outs = {}
+    for prec in ("fp64", "fp32"):
+        pc = K.Precon.from_bsr(Dv, icolors=lay.off, fp32=prec == "fp32")
+        w = b.clone()
+        pc.apply(w)
+        L.call("ucfd_synchronize")
+        outs[prec] = w.clone()
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for _ in range(args.steps):
+            pc.apply(w)
+        e1.record(stream)
+        L.call("ucfd_synchronize")
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        bb = krylov_bytes(Dv, nnz, n, prec == "fp32")[1]
+        applies[prec] = {"ms": ms, "algorithmic_bytes": bb, "gbs": bb / ms / 1e6,
+                         "launches": 2 * nc}
+        pc.close()
+        log(f"block LU-SGS apply {prec}: {ms:.3f} ms, {bb / ms / 1e6:.0f} GB/s algorithmic")
+    L.call("ucfd_set_stream", None)
+    apply_err = float((outs["fp32"] - outs["fp64"]).norm() / outs["fp64"].norm())
+    f64, f32 = flows["fp64"], flows["fp32"]
+    ns_ms64 = f64["ns_lower_ms"] + f64["ns_upper_ms"]
+    ns_ms32 = f32["ns_lower_ms"] + f32["ns_upper_ms"]
+    launch_ms = f64["ns_launch_ms"]
+    lb = f64["ns_sweep_algorithmic_bytes"] / nc
+    rec = {
+        "metric": "block LU-SGS cell-iters/s (NS 5x5 lower + upper sweeps), FP32 sweeps vs FP64",
+        "value": f64["ns_cell_iters_per_s"],
+        "unit": "cell-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 5,
+        "ms_per_step": ns_ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (factors and sweeps); fp32 sweep form reported beside",
+        "data": "synthetic (seeded Kuhn-tet mesh; FlowSys face-slot Jacobian ucfdgen_flowsys_fill; "
+                "BSR block Jacobian ucfdgen_bsr_fill)",
+        "config": {"workload": f"FlowSys block LU-SGS (NS 5x5 + SA 1x1), {args.m}^3 hex x 6 tets, "
+                               f"{args.coloring} coloured, colour-ordered numbering",
+                   "neles": N, "positions": npos, "ncolors": nc, "seed": args.seed},
+        "flowsys": flows,
+        "fp32_speedup_ns_sweeps": ns_ms64 / ns_ms32,
+        "fp32_speedup_rans_sweeps": f64["rans_lower_upper_ms"] / f32["rans_lower_upper_ms"],
+        "fp32_rel_err_vs_fp64": errs,
+        "krylov_precon_apply": applies,
+        "krylov_precon_apply_fp32_rel_err": apply_err,
+        "roofline": {
+            "kernel": "FlowSys NS block sweep (k_sweep<5,false>), one launch per colour",
+            "bound": "hbm", "achieved": lb / (launch_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": lb / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": lb, "avg_launch_ms": launch_ms},
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def adjoint_workload(args, log):
     """Differentiation form: the FP64 coloured LU-SGS pass (pre + lower +
     upper) on a 55^3 x 6 tet mesh (998,250 cells) behind
@@ -297,7 +502,7 @@ def adjoint_workload(args, log):
     mesh = M.tet_mesh(m, seed=seed)
     flow = M.flow_fields(mesh, seed=seed)
     mesh.key = M.hilbert_keys(mesh, "tet", (m,))
-    lay = colored_layout(mesh, flow)
+    lay = colored_layout(mesh, flow, coloring=args.coloring)
     del mesh, flow
     N = lay.mesh.neles
     log(f"adjoint mesh {N} cells, {lay.ncolors} colours, setup {time.time() - t0:.1f} s")
@@ -451,10 +656,13 @@ def main():
                          "(ucfd_greedy_coloring_ordered, -reduced: then Kempe-chain "
                          "reduction), first-fit in breadth-first order (ucfd_bfs_order) "
                          "or balanced")
-    ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint"), default="lusgs",
+    ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint", "blusgs"),
+                    default="lusgs",
                     help="lusgs: the headline colored LU-SGS line (default); krylov: the "
                          "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system; adjoint: "
-                         "the differentiation form on a 55^3 x 6 tet mesh")
+                         "the differentiation form on a 55^3 x 6 tet mesh; blusgs: block "
+                         "LU-SGS (FlowSys NS + SA sweeps, Krylov-path apply) on the 96^3 x 6 "
+                         "tet mesh, FP64 vs FP32 sweeps")
     ap.add_argument("--adjoint-m", type=int, default=55)
     ap.add_argument("--adjoint-reps", type=int, default=10)
     ap.add_argument("--adjoint-dirs", type=int, default=64)
@@ -496,6 +704,9 @@ def main():
         return
     if args.workload == "adjoint":
         adjoint_workload(args, log)
+        return
+    if args.workload == "blusgs":
+        blusgs_workload(args, log)
         return
 
     t0 = time.time()

@@ -10,8 +10,11 @@
 // every dot product is reproducible run to run.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "block_lu.h"
 #include "krylov_internal.h"
+#include "wave_rows.h"
 
 namespace {
 
@@ -68,6 +71,104 @@ __global__ __launch_bounds__(KB) void k_spmv_bsr(int bn, const int *__restrict__
     for (int k = 0; k < B; ++k) {
         const int64_t o = (int64_t)idx*B + k;
         y[o] = alpha*arr[k] + beta*y[o];
+    }
+    if (bad) atomicOr(flag, UCFD_KFLAG_INDEX);
+}
+
+// Block sizes 2..7: one wave per G = 64/B block rows (wave_rows.h).  The
+// wave's blocks [rowptr[i0], rowptr[i0+G]) are contiguous; they are staged
+// through LDS in chunks of CB blocks with all of a chunk's loads in flight,
+// the column indices and x rows of each lane's first KP blocks are fetched
+// before the staging loads, and lane (g, r) sums scalar row r of block row g
+// in the reference order (per block: v = sum_q A[r][q] x[q], then arr += v).
+constexpr int KSTAGE = 1600;    // staged doubles per wave (12.8 KB of LDS)
+
+template <int B>
+__global__ __launch_bounds__(64) void k_spmv_bsr_w(int bn, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ colidx,
+                                                   const double *__restrict__ values,
+                                                   double alpha, const double *__restrict__ x,
+                                                   double beta, double *__restrict__ y, int *flag)
+{
+    constexpr int B2 = B*B, G = 64/B, CB = KSTAGE/B2, U = (CB*B2 + 63)/64, KP = 5;
+    __shared__ double sv[CB*B2];
+    const int lane = threadIdx.x;
+    const int i0 = ucfd::wave_xcd_index()*G;
+    if (i0 >= bn) return;
+    const int iend = min(i0 + G, bn);
+    const int g = lane/B, r = lane%B;
+    const int i = i0 + g;
+    const bool act = g < G && i < iend;
+    const int s0 = rowptr[i0], s1 = rowptr[iend];
+    const int st = act ? rowptr[i] : 0, ed = act ? rowptr[i + 1] : 0;
+    bool bad = false;
+    double xp[KP][B];
+    int cp[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        cp[k] = -1;
+        if (st + k < ed) {
+            const int c = colidx[st + k];
+            if ((unsigned)c < (unsigned)bn) {
+                cp[k] = c;
+#pragma unroll
+                for (int q = 0; q < B; ++q) xp[k][q] = x[(int64_t)c*B + q];
+            } else {
+                bad = true;
+            }
+        }
+    }
+    double arr = 0.0;
+    for (int c0 = s0; c0 < s1; c0 += CB) {
+        const int c1 = min(c0 + CB, s1);
+        const double *src = values + (int64_t)c0*B2;
+        const int ne = (c1 - c0)*B2;
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = u*64 + lane;
+            t[u] = e < ne ? __builtin_nontemporal_load(src + e) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = u*64 + lane;
+            if (e < ne) sv[e] = t[u];
+        }
+        ucfd::wave_lds_sync();
+        if (act) {
+            const int ja = max(st, c0), jb = min(ed, c1);
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                const int j = st + k;
+                if (j >= ja && j < jb && cp[k] >= 0) {
+                    const double *blk = sv + (j - c0)*B2 + r*B;
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; ++q) v += blk[q]*xp[k][q];
+                    arr += v;
+                }
+            }
+            for (int j = max(ja, st + KP); j < jb; ++j) {
+                const int c = colidx[j];
+                if ((unsigned)c >= (unsigned)bn) {
+                    bad = true;
+                    continue;
+                }
+                const double *blk = sv + (j - c0)*B2 + r*B;
+                double xq[B];
+#pragma unroll
+                for (int q = 0; q < B; ++q) xq[q] = x[(int64_t)c*B + q];
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < B; ++q) v += blk[q]*xq[q];
+                arr += v;
+            }
+        }
+        ucfd::wave_lds_sync();
+    }
+    if (act) {
+        const int64_t o = (int64_t)i*B + r;
+        y[o] = alpha*arr + beta*y[o];
     }
     if (bad) atomicOr(flag, UCFD_KFLAG_INDEX);
 }
@@ -216,6 +317,163 @@ __global__ __launch_bounds__(KB) void k_sweep32(int base, int cnt, const int *__
             b[o] = v;
         }
     }
+}
+
+// Block sweeps on one wave per G = 64/B rows of a contiguous row range
+// (block sizes 2..7).  Lane (g, r) accumulates scalar row r of row g over
+// the row's lower (lower sweep) or upper (upper sweep) blocks in the order
+// of k_sweep64 / k_sweep32 above; the group then exchanges its B sums with
+// lane shuffles and every lane runs the row's diagonal solve (bit-identical
+// to the thread-per-row kernels).  When the wave's blocks fit KSTAGE words
+// they are staged through LDS (per row, contiguous loads), with the rows'
+// LU blocks; otherwise the lanes read them from HBM directly.
+template <int B, bool UPPER, bool F32, bool STAGED>
+__device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0, int n, int off,
+                                          const int *__restrict__ colidx,
+                                          const typename std::conditional<F32, float, double>::type
+                                              *blocks,
+                                          const double *lu_src, float *x32, double *b)
+{
+    using T = typename std::conditional<F32, float, double>::type;
+    constexpr int B2 = B*B, KP = 5;
+    // column x rows of the first KP blocks
+    T xp[KP][B];
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+        if (k < n) {
+            const int c = colidx[j0 + k];
+#pragma unroll
+            for (int q = 0; q < B; ++q)
+                xp[k][q] = F32 ? (T)x32[(int64_t)c*B + q] : (T)b[(int64_t)c*B + q];
+        }
+    T acc = T(0);
+    double a = 0.0;
+    if (!F32 && !UPPER && act) a = b[(int64_t)i*B + r];
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+        if (k < n) {
+            const T *blk = blocks + (STAGED ? (int64_t)(off + k)*B2 : (int64_t)(j0 + k)*B2) + r*B;
+            T v = T(0);
+#pragma unroll
+            for (int q = 0; q < B; ++q) v += blk[q]*xp[k][q];
+            if (F32) acc += v;
+            else if (UPPER) a += (double)v;
+            else a -= (double)v;
+        }
+    for (int k = KP; k < n; ++k) {
+        const int c = colidx[j0 + k];
+        const T *blk = blocks + (STAGED ? (int64_t)(off + k)*B2 : (int64_t)(j0 + k)*B2) + r*B;
+        T xq[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) xq[q] = F32 ? (T)x32[(int64_t)c*B + q] : (T)b[(int64_t)c*B + q];
+        T v = T(0);
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += blk[q]*xq[q];
+        if (F32) acc += v;
+        else if (UPPER) a += (double)v;
+        else a -= (double)v;
+    }
+    if (F32) {
+        if (UPPER) a = (double)acc;
+        else if (act) a = b[(int64_t)i*B + r] - (double)acc;
+    }
+    // the group's B sums, then the diagonal solve in every lane of the group
+    double arr[B], lu[B2];
+#pragma unroll
+    for (int k = 0; k < B; ++k) arr[k] = __shfl(a, g*B + k, 64);
+    if (!act) return;
+#pragma unroll
+    for (int q = 0; q < B2; ++q) lu[q] = lu_src[q];
+    ucfd::lusub<B>(lu, arr);
+    const int64_t o = (int64_t)i*B + r;
+    if (F32) {
+        if (UPPER) {
+            const double v = (double)x32[o] - arr[r];
+            x32[o] = (float)v;
+            b[o] = v;
+        } else {
+            x32[o] = (float)arr[r];
+        }
+    } else {
+        if (UPPER) b[o] -= arr[r];
+        else b[o] = arr[r];
+    }
+}
+
+template <int B, bool UPPER, bool F32>
+__global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, const int *__restrict__ rowptr,
+                                               const int *__restrict__ colidx,
+                                               const int *__restrict__ diagslots,
+                                               const typename std::conditional<F32, float,
+                                                                               double>::type
+                                                   *__restrict__ vals,
+                                               const double *__restrict__ dv, float *x32,
+                                               double *b)
+{
+    using T = typename std::conditional<F32, float, double>::type;
+    constexpr int B2 = B*B, G = 64/B, CAP = KSTAGE*8/(int)sizeof(T), PER = (5*B2 + 63)/64;
+    constexpr int NLU = (G*B2 + 63)/64;
+    __shared__ T sv[CAP];
+    __shared__ double slu[G*B2];
+    const int lane = threadIdx.x;
+    const int t0 = ucfd::wave_xcd_index()*G;
+    if (t0 >= cnt) return;
+    const int g = lane/B, r = lane%B;
+    const int i = base + t0 + g;
+    const bool act = g < G && t0 + g < cnt;
+    int j0 = 0, n = 0;
+    if (act) {
+        const int dd = diagslots[i];
+        j0 = UPPER ? dd + 1 : rowptr[i];
+        n = (UPPER ? rowptr[i + 1] : dd) - j0;
+    }
+    // block offset of each row in the staged list; the total decides the path
+    const int lead = (act && r == 0) ? n : 0;
+    const int ex = ucfd::wave_excl_scan(lead);
+    const int total = __shfl(ex + lead, 63, 64);
+    const int off = __shfl(ex, g*B < 64 ? g*B : 0, 64);
+    const int nrows = min(G, cnt - t0);
+    if (total*B2 > CAP) {
+        bsgs_rows<B, UPPER, F32, false>(r, g, act, i, j0, n, 0, colidx, vals,
+                                        dv + (int64_t)i*B2, x32, b);
+        return;
+    }
+    // stage: every row's blocks [j0, j0+n) to sv[off*B2 ...], and the rows' LU
+    T t[G][PER];
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+        const int jg = __shfl(j0, gg*B, 64), ng = __shfl(n, gg*B, 64)*B2;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = k*64 + lane;
+            t[gg][k] = e < ng ? __builtin_nontemporal_load(vals + (int64_t)jg*B2 + e) : T(0);
+        }
+    }
+    double tl[NLU];
+    const double *lsrc = dv + (int64_t)(base + t0)*B2;
+#pragma unroll
+    for (int k = 0; k < NLU; ++k) {
+        const int e = k*64 + lane;
+        tl[k] = e < nrows*B2 ? __builtin_nontemporal_load(lsrc + e) : 0.0;
+    }
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+        const int jg = __shfl(j0, gg*B, 64), ng = __shfl(n, gg*B, 64)*B2;
+        const int og = __shfl(ex, gg*B, 64)*B2;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = k*64 + lane;
+            if (e < ng) sv[og + e] = t[gg][k];
+        }
+        for (int e = PER*64 + lane; e < ng; e += 64) sv[og + e] = vals[(int64_t)jg*B2 + e];
+    }
+#pragma unroll
+    for (int k = 0; k < NLU; ++k) {
+        const int e = k*64 + lane;
+        if (e < nrows*B2) slu[e] = tl[k];
+    }
+    ucfd::wave_lds_sync();
+    bsgs_rows<B, UPPER, F32, true>(r, g, act, i, j0, n, off, colidx, sv, slu + g*B2, x32, b);
 }
 
 // --------------------------------------------------------------- BLAS
@@ -413,8 +671,14 @@ template <int B> struct SpmvL {
     static int run(hipStream_t s, int bn, const int *rp, const int *ci, const double *v, double al,
                    const double *x, double be, double *y, int *flag)
     {
-        hipLaunchKernelGGL(k_spmv_bsr<B>, dim3((bn + KB - 1)/KB), dim3(KB), 0, s, bn, rp, ci, v, al,
-                           x, be, y, flag);
+        if (B == 1) {
+            hipLaunchKernelGGL(k_spmv_bsr<B>, dim3((bn + KB - 1)/KB), dim3(KB), 0, s, bn, rp, ci,
+                               v, al, x, be, y, flag);
+        } else {
+            constexpr int G = 64/B;
+            hipLaunchKernelGGL(k_spmv_bsr_w<B>, dim3((bn + G - 1)/G), dim3(64), 0, s, bn, rp, ci,
+                               v, al, x, be, y, flag);
+        }
         return ok();
     }
 };
@@ -432,6 +696,16 @@ template <int B> struct Sw64L {
     static int run(hipStream_t s, int upper, int base, int cnt, const int *ord, const int *rp,
                    const int *ci, const int *ds, const double *v, const double *dv, double *b)
     {
+        if (B > 1 && !ord) {
+            const dim3 g((cnt + 64/B - 1)/(64/B));
+            if (upper)
+                hipLaunchKernelGGL((k_bsgs_w<B, true, false>), g, dim3(64), 0, s, base, cnt, rp, ci,
+                                   ds, v, dv, nullptr, b);
+            else
+                hipLaunchKernelGGL((k_bsgs_w<B, false, false>), g, dim3(64), 0, s, base, cnt, rp, ci,
+                                   ds, v, dv, nullptr, b);
+            return ok();
+        }
         const dim3 g((cnt + KB - 1)/KB);
         if (upper)
             hipLaunchKernelGGL((k_sweep64<B, true>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci, ds, v,
@@ -448,6 +722,16 @@ template <int B> struct Sw32L {
                    const int *ci, const int *ds, const float *v, const double *dv, float *x32,
                    double *b)
     {
+        if (B > 1 && !ord) {
+            const dim3 g((cnt + 64/B - 1)/(64/B));
+            if (upper)
+                hipLaunchKernelGGL((k_bsgs_w<B, true, true>), g, dim3(64), 0, s, base, cnt, rp, ci,
+                                   ds, v, dv, x32, b);
+            else
+                hipLaunchKernelGGL((k_bsgs_w<B, false, true>), g, dim3(64), 0, s, base, cnt, rp, ci,
+                                   ds, v, dv, x32, b);
+            return ok();
+        }
         const dim3 g((cnt + KB - 1)/KB);
         if (upper)
             hipLaunchKernelGGL((k_sweep32<B, true>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci, ds, v,
