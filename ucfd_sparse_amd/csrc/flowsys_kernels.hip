@@ -8,12 +8,17 @@
 // without FMA contraction, so they agree bit for bit with the CPU oracle.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "block_lu.h"
 #include "flowsys_internal.h"
+#include "wave_rows.h"
 
 namespace {
 
 constexpr int FB = 256;
+constexpr int FSTAGE = 16000;   // LDS bytes of staged blocks per wave
+constexpr int FKP = 4;          // positions per cell whose neighbour rows are fetched first
 
 __device__ __forceinline__ int fxcd()
 {
@@ -119,48 +124,137 @@ __global__ __launch_bounds__(FB) void k_subres(int neles, int nlocal, int nvars,
     }
 }
 
-// pre_blusgs (blusgs.c:200-246) plus the packing of every position's
-// off-diagonal block (the sweep's jval, blusgs.c:297-300) and fv
+// pre_blusgs (blusgs.c:200-246): per cell, D += (side ? A+ : -A-)*fv over
+// its positions in order, then LU; plus the packing of every position's
+// neighbour block (the sweep's jval, side ? A- : -A+, blusgs.c:297-300) and
+// fv = face_area*rcp_vol.  Two passes:
+// (1) Positions in face-slot order (sorted[k], built with the structure):
+// consecutive lanes read consecutive slots of the face-major jmat, so its
+// loads are coalesced; each lane forms its position's neighbour block j and
+// diagonal contribution v*fv, the wave transposes them through LDS and
+// writes whole blocks with consecutive lanes on consecutive words.
 template <int NV>
-__global__ __launch_bounds__(FB) void k_prepare(int nlocal, int64_t nfaces,
-                                                const int *__restrict__ rowptr,
-                                                const int *__restrict__ slots,
-                                                const signed char *__restrict__ sides,
-                                                const double *__restrict__ face_area,
-                                                const double *__restrict__ rcp_vol,
-                                                double *diag, const double *__restrict__ jmat,
-                                                double *jp, double *fvp, float *jp32, float *fv32,
-                                                int *flag)
+__global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
+                                                 const int *__restrict__ sorted,
+                                                 const int *__restrict__ pos_cell,
+                                                 const int *__restrict__ slots,
+                                                 const signed char *__restrict__ sides,
+                                                 const double *__restrict__ face_area,
+                                                 const double *__restrict__ rcp_vol,
+                                                 const double *__restrict__ jmat, double *jp,
+                                                 double *dj, double *fvp, float *jp32, float *fv32)
 {
     constexpr int D2 = NV*NV;
-    const int r = blockIdx.x*FB + threadIdx.x;
-    if (r >= nlocal) return;
-    double dm[D2];
-#pragma unroll
-    for (int q = 0; q < D2; ++q) dm[q] = diag[(int64_t)r*D2 + q];
+    __shared__ double sb[64*D2];
+    __shared__ int sp[64];
+    const int lane = threadIdx.x;
+    const int64_t k0 = (int64_t)ucfd::wave_xcd_index()*64;
+    if (k0 >= npos) return;
+    const int64_t k = k0 + lane;
+    const bool act = k < npos;
+    const int nact = (int)min((int64_t)64, npos - k0);
+    int p = 0;
+    int64_t s = 0;
+    bool plus = false;
+    double fv = 0.0;
+    if (act) {
+        p = sorted[k];
+        s = slots[p];
+        plus = sides[p] == 1;
+        fv = face_area[s]*rcp_vol[pos_cell[p]];
+    }
+    sp[lane] = p;
     const int64_t half = (int64_t)D2*nfaces;
-    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
-        const int64_t s = slots[p];
-        const bool plus = sides[p] == 1;
-        const double fv = face_area[s]*rcp_vol[r];
+    double a[D2], b[D2];
+#pragma unroll
+    for (int q = 0; q < D2; ++q) {
+        a[q] = act ? jmat[s + (int64_t)q*nfaces] : 0.0;
+        b[q] = act ? jmat[s + (int64_t)q*nfaces + half] : 0.0;
+    }
+    if (act) {
         fvp[p] = fv;
         if (fv32) fv32[p] = (float)fv;
-#pragma unroll
-        for (int row = 0; row < NV; ++row)
-#pragma unroll
-            for (int col = 0; col < NV; ++col) {
-                const int64_t o = s + (int64_t)col*nfaces + (int64_t)row*NV*nfaces;
-                const double a = jmat[o], b = jmat[o + half];
-                const double v = plus ? a : -b;       // diagonal contribution
-                const double j = plus ? b : -a;       // neighbour coupling
-                dm[col + row*NV] += v*fv;
-                jp[(int64_t)p*D2 + row*NV + col] = j;
-                if (jp32) jp32[(int64_t)p*D2 + row*NV + col] = (float)j;
-            }
     }
-    if (!ucfd::ludcmp<NV>(dm)) atomicOr(flag, UCFD_KFLAG_PIVOT);
+    // neighbour coupling (the sweep's jval)
 #pragma unroll
-    for (int q = 0; q < D2; ++q) diag[(int64_t)r*D2 + q] = dm[q];
+    for (int q = 0; q < D2; ++q) sb[lane*D2 + q] = plus ? b[q] : -a[q];
+    ucfd::wave_lds_sync();
+    for (int e = lane; e < nact*D2; e += 64) {
+        const int blk = e/D2;
+        const int64_t o = (int64_t)sp[blk]*D2 + (e - blk*D2);
+        const double j = sb[e];
+        jp[o] = j;
+        if (jp32) jp32[o] = (float)j;
+    }
+    ucfd::wave_lds_sync();
+    // diagonal contribution v*fv
+#pragma unroll
+    for (int q = 0; q < D2; ++q) sb[lane*D2 + q] = (plus ? a[q] : -b[q])*fv;
+    ucfd::wave_lds_sync();
+    for (int e = lane; e < nact*D2; e += 64) {
+        const int blk = e/D2;
+        dj[(int64_t)sp[blk]*D2 + (e - blk*D2)] = sb[e];
+    }
+}
+
+// (2) Cells, one wave per G = 64/NV cells: lane (g, row) adds row `row` of
+// the cell's contributions to the packed diagonal in position order (the
+// wave's contributions staged through LDS when they fit), the group
+// exchanges the rows with lane shuffles, every lane factors the block, and
+// lane (g, row) writes row `row` of the LU.
+template <int NV>
+__global__ __launch_bounds__(64) void k_prep_diag(int nlocal, const int *__restrict__ rowptr,
+                                                  const double *__restrict__ dj, double *diag,
+                                                  int *flag)
+{
+    constexpr int D2 = NV*NV, G = 64/NV;
+    constexpr int CP = FSTAGE/(8*D2) < 8*G ? FSTAGE/(8*D2) : 8*G;
+    constexpr int UJ = (CP*D2 + 63)/64;
+    __shared__ double sd[CP*D2];
+    const int lane = threadIdx.x;
+    const int t0 = ucfd::wave_xcd_index()*G;
+    if (t0 >= nlocal) return;
+    const int g = lane/NV, row = lane%NV;
+    const int nrows = min(G, nlocal - t0);
+    const int r = t0 + g;
+    const bool act = g < nrows;
+    const int p0w = rowptr[t0], p1w = rowptr[t0 + nrows];
+    const int st = act ? rowptr[r] : 0, ed = act ? rowptr[r + 1] : 0;
+    double dm[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) dm[c] = act ? diag[(int64_t)r*D2 + row*NV + c] : 0.0;
+    if (p1w - p0w <= CP) {
+        const int nj = (p1w - p0w)*D2;
+        const double *src = dj + (int64_t)p0w*D2;
+        double t[UJ];
+#pragma unroll
+        for (int u = 0; u < UJ; ++u) {
+            const int e = u*64 + lane;
+            t[u] = e < nj ? __builtin_nontemporal_load(src + e) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UJ; ++u) {
+            const int e = u*64 + lane;
+            if (e < nj) sd[e] = t[u];
+        }
+        ucfd::wave_lds_sync();
+        for (int p = st; p < ed; ++p)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) dm[c] += sd[(p - p0w)*D2 + row*NV + c];
+    } else {
+        for (int p = st; p < ed; ++p)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) dm[c] += dj[(int64_t)p*D2 + row*NV + c];
+    }
+    double m[D2];
+#pragma unroll
+    for (int rr = 0; rr < NV; ++rr)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) m[rr*NV + c] = __shfl(dm[c], g*NV + rr, 64);
+    if (!act) return;
+    if (!ucfd::ludcmp<NV>(m) && row == 0) atomicOr(flag, UCFD_KFLAG_PIVOT);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) diag[(int64_t)r*D2 + row*NV + c] = m[row*NV + c];
 }
 
 // block Gauss-Seidel row update over every position (blusgs.c:273-373)
@@ -228,6 +322,149 @@ __global__ __launch_bounds__(FB) void k_sweep(int base, int cnt, const int *__re
     }
 }
 
+// The same sweep on one wave per G = 64/NV rows of a contiguous row range:
+// lane (g, row) = (lane / NV, lane % NV) accumulates component `row` of
+// cell g over every position in the order of k_sweep above; the group
+// exchanges its NV values with lane shuffles and every lane runs the cell's
+// diagonal solve.  The wave's packed blocks, fv values and diagonal LU
+// blocks are contiguous (positions of consecutive cells are consecutive) and
+// are staged through LDS with coalesced loads when they fit FSTAGE bytes;
+// otherwise the lanes read them from HBM directly.  Results equal k_sweep's
+// bit for bit.
+
+template <int NV, bool F32, bool STAGED>
+__device__ __forceinline__ void fs_rows(int row, int g, bool act, int r, int st, int ed, int p0w,
+                                        int var0, int nvars, const int *__restrict__ colidx,
+                                        const typename std::conditional<F32, float, double>::type
+                                            *blocks,
+                                        const typename std::conditional<F32, float, double>::type
+                                            *fvs,
+                                        const double *lu_src, const double *__restrict__ rhs,
+                                        double *du, float *du32)
+{
+    using T = typename std::conditional<F32, float, double>::type;
+    constexpr int D2 = NV*NV;
+    T xn[FKP][NV];
+#pragma unroll
+    for (int k = 0; k < FKP; ++k)
+        if (st + k < ed) {
+            const int64_t nb = colidx[st + k];
+#pragma unroll
+            for (int c = 0; c < NV; ++c)
+                xn[k][c] = F32 ? (T)du32[var0 + c + nb*nvars] : (T)du[var0 + c + nb*nvars];
+        }
+    double x = act ? rhs[var0 + row + (int64_t)r*nvars] : 0.0;
+#pragma unroll
+    for (int k = 0; k < FKP; ++k) {
+        const int p = st + k;
+        if (p < ed) {
+            const int64_t q = STAGED ? p - p0w : p;
+            const T *blk = blocks + q*D2 + row*NV;
+            T val = T(0);
+#pragma unroll
+            for (int c = 0; c < NV; ++c) val += blk[c]*xn[k][c];
+            x -= (double)(val*fvs[q]);
+        }
+    }
+    for (int p = st + FKP; p < ed; ++p) {
+        const int64_t nb = colidx[p];
+        const int64_t q = STAGED ? p - p0w : p;
+        const T *blk = blocks + q*D2 + row*NV;
+        T val = T(0);
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+            val += blk[c]*(F32 ? (T)du32[var0 + c + nb*nvars] : (T)du[var0 + c + nb*nvars]);
+        x -= (double)(val*fvs[q]);
+    }
+    double xv[NV], lu[D2];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) xv[c] = __shfl(x, g*NV + c, 64);
+    if (!act) return;
+#pragma unroll
+    for (int q = 0; q < D2; ++q) lu[q] = lu_src[q];
+    ucfd::lusub<NV>(lu, xv);
+    const int64_t o = var0 + row + (int64_t)r*nvars;
+    du[o] = xv[row];
+    if (F32) du32[o] = (float)xv[row];
+}
+
+template <int NV, bool F32>
+__global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int var0, int nvars,
+                                                const int *__restrict__ rowptr,
+                                                const int *__restrict__ colidx,
+                                                const typename std::conditional<F32, float,
+                                                                                double>::type
+                                                    *__restrict__ jp,
+                                                const typename std::conditional<F32, float,
+                                                                                double>::type
+                                                    *__restrict__ fvp,
+                                                const double *__restrict__ rhs,
+                                                const double *__restrict__ diag, double *du,
+                                                float *du32)
+{
+    using T = typename std::conditional<F32, float, double>::type;
+    constexpr int D2 = NV*NV, G = 64/NV;
+    // staged positions per wave: FSTAGE bytes of blocks, at most 8 per cell
+    constexpr int CP = FSTAGE/((int)sizeof(T)*D2) < 8*G ? FSTAGE/((int)sizeof(T)*D2) : 8*G;
+    constexpr int UJ = (CP*D2 + 63)/64, UF = (CP + 63)/64, NLU = (G*D2 + 63)/64;
+    __shared__ T sj[CP*D2];
+    __shared__ T sf[CP];
+    __shared__ double slu[G*D2];
+    const int lane = threadIdx.x;
+    const int t0 = ucfd::wave_xcd_index()*G;
+    if (t0 >= cnt) return;
+    const int g = lane/NV, row = lane%NV;
+    const int nrows = min(G, cnt - t0);
+    const int r0 = base + t0, r = r0 + g;
+    const bool act = g < nrows;
+    const int p0w = rowptr[r0], p1w = rowptr[r0 + nrows];
+    const int st = act ? rowptr[r] : 0, ed = act ? rowptr[r + 1] : 0;
+    if (p1w - p0w > CP) {
+        fs_rows<NV, F32, false>(row, g, act, r, st, ed, 0, var0, nvars, colidx, jp, fvp,
+                                diag + (int64_t)r*D2, rhs, du, du32);
+        return;
+    }
+    const int nj = (p1w - p0w)*D2, nf = p1w - p0w, nl = nrows*D2;
+    const T *js = jp + (int64_t)p0w*D2;
+    const T *fs = fvp + p0w;
+    const double *ls = diag + (int64_t)r0*D2;
+    T tj[UJ], tf[UF];
+    double tl[NLU];
+#pragma unroll
+    for (int u = 0; u < UJ; ++u) {
+        const int e = u*64 + lane;
+        tj[u] = e < nj ? __builtin_nontemporal_load(js + e) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < UF; ++u) {
+        const int e = u*64 + lane;
+        tf[u] = e < nf ? __builtin_nontemporal_load(fs + e) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NLU; ++u) {
+        const int e = u*64 + lane;
+        tl[u] = e < nl ? __builtin_nontemporal_load(ls + e) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UJ; ++u) {
+        const int e = u*64 + lane;
+        if (e < nj) sj[e] = tj[u];
+    }
+#pragma unroll
+    for (int u = 0; u < UF; ++u) {
+        const int e = u*64 + lane;
+        if (e < nf) sf[e] = tf[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NLU; ++u) {
+        const int e = u*64 + lane;
+        if (e < nl) slu[e] = tl[u];
+    }
+    ucfd::wave_lds_sync();
+    fs_rows<NV, F32, true>(row, g, act, r, st, ed, p0w, var0, nvars, colidx, sj, sf,
+                           slu + (act ? g*D2 : 0), rhs, du, du32);
+}
+
 __global__ __launch_bounds__(FB) void k_zero(int64_t n, double *a, double *b, float *c)
 {
     const int64_t st = (int64_t)gridDim.x*FB;
@@ -239,14 +476,19 @@ __global__ __launch_bounds__(FB) void k_zero(int64_t n, double *a, double *b, fl
 }
 
 template <int NV> struct PrepL {
-    static int run(hipStream_t s, int nlocal, int64_t nfaces, const int *rowptr, const int *slots,
+    static int run(hipStream_t s, int nlocal, int64_t nfaces, int64_t npos, const int *rowptr,
+                   const int *sorted, const int *pos_cell, const int *slots,
                    const signed char *sides, const double *face_area, const double *rcp_vol,
-                   double *diag, const double *jmat, double *jp, double *fvp, float *jp32,
-                   float *fv32, int *flag)
+                   double *diag, const double *jmat, double *jp, double *dj, double *fvp,
+                   float *jp32, float *fv32, int *flag)
     {
-        hipLaunchKernelGGL(k_prepare<NV>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, rowptr,
-                           slots, sides, face_area, rcp_vol, diag, jmat, jp, fvp, jp32, fv32,
-                           flag);
+        if (npos > 0)
+            hipLaunchKernelGGL(k_prep_pos<NV>, dim3((unsigned)((npos + 63)/64)), dim3(64), 0, s,
+                               npos, nfaces, sorted, pos_cell, slots, sides, face_area, rcp_vol,
+                               jmat, jp, dj, fvp, jp32, fv32);
+        constexpr int G = 64/NV;
+        hipLaunchKernelGGL(k_prep_diag<NV>, dim3((unsigned)((nlocal + G - 1)/G)), dim3(64), 0, s,
+                           nlocal, rowptr, dj, diag, flag);
         return ok();
     }
 };
@@ -257,6 +499,16 @@ template <int NV> struct SweepL {
                    const double *fvp, const float *jp32, const float *fv32, const double *rhs,
                    const double *diag, double *du, float *du32)
     {
+        if (!order) {
+            const dim3 g((unsigned)((cnt + 64/NV - 1)/(64/NV)));
+            if (f32)
+                hipLaunchKernelGGL((k_sweep_w<NV, true>), g, dim3(64), 0, s, base, cnt, var0, nvars,
+                                   rowptr, colidx, jp32, fv32, rhs, diag, du, du32);
+            else
+                hipLaunchKernelGGL((k_sweep_w<NV, false>), g, dim3(64), 0, s, base, cnt, var0,
+                                   nvars, rowptr, colidx, jp, fvp, rhs, diag, du, du32);
+            return ok();
+        }
         if (f32)
             hipLaunchKernelGGL((k_sweep<NV, true>), grid(cnt), dim3(FB), 0, s, base, cnt, order,
                                var0, nvars, rowptr, colidx, jp, fvp, jp32, fv32, rhs, diag, du,
@@ -324,14 +576,15 @@ int fs_launch_subres(hipStream_t s, int neles, int nlocal, int nvars, const int 
     return ok();
 }
 
-int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, const int *rowptr,
-                      const int *slots, const signed char *sides, const double *face_area,
-                      const double *rcp_vol, double *diag, const double *jmat, double *jp,
-                      double *fvp, float *jp32, float *fv32, int *flag)
+int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, int64_t npos,
+                      const int *rowptr, const int *sorted, const int *pos_cell, const int *slots,
+                      const signed char *sides, const double *face_area, const double *rcp_vol,
+                      double *diag, const double *jmat, double *jp, double *dj, double *fvp,
+                      float *jp32, float *fv32, int *flag)
 {
     if (nlocal <= 0) return UCFD_OK;
-    return by_nv<PrepL>(nv, s, nlocal, nfaces, rowptr, slots, sides, face_area, rcp_vol, diag,
-                        jmat, jp, fvp, jp32, fv32, flag);
+    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, sorted, pos_cell, slots, sides,
+                        face_area, rcp_vol, diag, jmat, jp, dj, fvp, jp32, fv32, flag);
 }
 
 int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int cnt, const int *order,
