@@ -65,6 +65,32 @@ def test_stages_bitwise(where, colored, kind):
         assert np.array_equal(g, v), k
 
 
+@pytest.mark.parametrize("nvars,nfvars", [(7, 7), (9, 7)])
+def test_hex_wide_blocks_bitwise(nvars, nfvars):
+    """7x7 blocks on a colour-ordered hex mesh: one wave's positions exceed the
+    LDS staging of the wave kernels (their HBM-direct path), and a 2x2
+    turbulence block for nvars = 9."""
+    from ucfd_sparse_amd import flowsys as FS
+    from ucfd_sparse_amd.layout import colored_layout
+    say(f"hex mesh, nvars={nvars}, nfvars={nfvars}")
+    mesh = M.hex_mesh(7, 6, 5, seed=81, permute=True)
+    flow = M.flow_fields(mesh, seed=81)
+    lay = colored_layout(mesh, flow, coloring="greedy-bfs")
+    mesh, flow = lay.mesh, lay.flow
+    fs = M.flowsys_system(mesh, flow, nv=nfvars, seed=81)
+    tj = M.flowsys_system(mesh, flow, nv=nvars - nfvars, seed=81).jmat if nvars > nfvars else None
+    el = M.flowsys_elements(mesh, flow, nvars=nvars, seed=81)
+    ref = O.flowsys_run(fs, tj, el, nvars, nfvars, kind="kwsst", iterations=2)
+    fsd, tjd, eld = on_device(fs, tj if tj is not None else np.zeros(1), el)
+    S = FS.FlowSys(fsd, nvars, nfvars, tjmat=tjd if tj is not None else None)
+    S.set_element(0, eld)
+    got = FS.run(S, kind="kwsst", iterations=2)
+    S.close()
+    for k, v in ref.items():
+        g = np.asarray(got[k]).reshape(v.shape)
+        assert np.array_equal(g, v), k
+
+
 def test_fp32_sweeps_close_to_fp64():
     from ucfd_sparse_amd import flowsys as FS
     say("FP32 sweep products vs FP64")

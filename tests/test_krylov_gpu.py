@@ -65,7 +65,7 @@ def tet3():
     return A, M.normal_fill(A.bn * A.blk, 301), M.normal_fill(A.bn * A.blk, 302)
 
 
-@pytest.mark.parametrize("blk", [1, 3, 5])
+@pytest.mark.parametrize("blk", [1, 2, 3, 5, 7])
 def test_spmv_device_bitwise(K, blk):
     say(f"BSR SpMV blk={blk}, device arrays")
     A, _ = system("tet", 4, 11, blk=blk)
@@ -75,6 +75,56 @@ def test_spmv_device_bitwise(K, blk):
     yd = dev(y0)
     mat.mult(dev(x), yd, alpha=-1.0, beta=0.5)
     assert np.array_equal(yd.cpu().numpy(), O.spmv_bsr(A, x, -1.0, 0.5, y=y0))
+
+
+def two_colour_dense(blk, nrow, per_row, seed):
+    """A 2-colour BSR matrix with long rows: rows 0..nrow-1 (colour 0) couple
+    only to rows nrow..2nrow-1 (colour 1) and back, per_row off-diagonal
+    blocks per row, diagonally dominant.  One wave's rows then use more
+    blocks than the LDS staging holds (multi-chunk SpMV, the HBM-direct path
+    of the wave-staged sweeps)."""
+    rng = np.random.default_rng(seed)
+    bn = 2 * nrow
+    rows, cols = [], []
+    for i in range(bn):
+        other = np.arange(nrow, bn) if i < nrow else np.arange(0, nrow)
+        nb = np.sort(rng.choice(other, size=per_row, replace=False))
+        c = np.sort(np.r_[nb, i])
+        cols.append(c)
+    rowptr = np.zeros(bn + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(c) for c in cols])
+    colidx = np.concatenate(cols).astype(np.int32)
+    diagslots = np.array([rowptr[i] + int(np.searchsorted(cols[i], i)) for i in range(bn)],
+                         np.int32)
+    values = rng.uniform(-0.1, 0.1, size=(len(colidx), blk, blk))
+    for i in range(bn):
+        values[diagslots[i]] += np.eye(blk) * (2.0 + per_row)
+    return M.BSR(bn, blk, rowptr, colidx, diagslots, values), np.array([0, nrow, bn], np.int32)
+
+
+@pytest.mark.parametrize("blk", [2, 5, 7])
+def test_long_rows_spmv_and_colored_apply_bitwise(K, blk):
+    say(f"long rows (14 blocks per row), blk={blk}: multi-chunk SpMV, HBM-direct sweeps")
+    A, off = two_colour_dense(blk, 150, 13, 70 + blk)
+    x = M.normal_fill(A.bn * blk, 71)
+    y0 = M.normal_fill(A.bn * blk, 72)
+    d = dev_bsr(A)
+    mat = K.SpMat.from_bsr(d)
+    yd = dev(y0)
+    mat.mult(dev(x), yd, alpha=0.75, beta=-0.25)
+    assert np.array_equal(yd.cpu().numpy(), O.spmv_bsr(A, x, 0.75, -0.25, y=y0))
+    dv = O.blusgs_prepare(A)
+    ref = O.blusgs_apply(A, dv, x, icolors=off)
+    pc = K.Precon.from_bsr(d, icolors=off)
+    bd = dev(x)
+    pc.apply(bd)
+    assert np.array_equal(bd.cpu().numpy(), ref)
+    pc32 = K.Precon.from_bsr(d, icolors=off, fp32=True)
+    bd = dev(x)
+    pc32.apply(bd)
+    err = np.abs(bd.cpu().numpy() - ref).max() / np.abs(ref).max()
+    say(f"  mixed apply rel err {err:.2e}")
+    assert err <= 1e-5
 
 
 def test_spmv_host_arrays_staged(K, tet3):

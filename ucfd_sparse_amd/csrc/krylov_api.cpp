@@ -180,12 +180,12 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
         for (const UcfdLevel &l : pc->lower.lv)
             if ((rc = kry_bsgs_sweep64(s, 0, pc->blk, l.base, l.cnt,
                                        l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
-                                       d.diagslots, d.values, pc->diagvalues, b)))
+                                       d.diagslots, d.values, pc->diagvalues, b, l.maxw)))
                 return rc;
         for (const UcfdLevel &l : pc->upper.lv)
             if ((rc = kry_bsgs_sweep64(s, 1, pc->blk, l.base, l.cnt,
                                        l.ordered ? pc->upper.order : nullptr, d.rowptr, d.colidx,
-                                       d.diagslots, d.values, pc->diagvalues, b)))
+                                       d.diagslots, d.values, pc->diagvalues, b, l.maxw)))
                 return rc;
         return UCFD_OK;
     }
@@ -193,12 +193,12 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
     for (const UcfdLevel &l : pc->lower.lv)
         if ((rc = kry_bsgs_sweep32(s, 0, pc->blk, l.base, l.cnt,
                                    l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
-                                   d.diagslots, pc->vals32, pc->diagvalues, pc->x32, b)))
+                                   d.diagslots, pc->vals32, pc->diagvalues, pc->x32, b, l.maxw)))
             return rc;
     for (const UcfdLevel &l : pc->upper.lv)
         if ((rc = kry_bsgs_sweep32(s, 1, pc->blk, l.base, l.cnt,
                                    l.ordered ? pc->upper.order : nullptr, d.rowptr, d.colidx,
-                                   d.diagslots, pc->vals32, pc->diagvalues, pc->x32, b)))
+                                   d.diagslots, pc->vals32, pc->diagvalues, pc->x32, b, l.maxw)))
             return rc;
     return UCFD_OK;
 }
@@ -211,6 +211,29 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
 // descending with rows ascending.  A colour-ordered matrix whose ranges hold
 // no internal couplings (the intended use of the coloured form) gets one
 // launch per colour range without building levels.
+
+// Blocks that the rows of one wave of the wave-staged sweep kernels use, at
+// most, over the waves of each contiguous level (rows base .. base+cnt-1 in
+// groups of 64/blk): sizes the LDS of each launch, so that levels whose rows
+// have no block in the sweep's direction take no block staging space.
+void set_wave_blocks(UcfdSchedule &S, bool upper, int blk, const std::vector<int> &rp,
+                     const std::vector<int> &ds)
+{
+    const int G = blk > 1 ? 64/blk : 64;
+    for (UcfdLevel &l : S.lv) {
+        if (l.ordered) continue;
+        int mx = 0;
+        for (int w0 = 0; w0 < l.cnt; w0 += G) {
+            int sum = 0;
+            for (int t = w0; t < l.cnt && t < w0 + G; ++t) {
+                const int i = l.base + t;
+                sum += upper ? rp[i + 1] - ds[i] - 1 : ds[i] - rp[i];
+            }
+            mx = sum > mx ? sum : mx;
+        }
+        l.maxw = mx;
+    }
+}
 
 int make_schedules(Precon pc, const char *fn)
 {
@@ -265,6 +288,8 @@ int make_schedules(Precon pc, const char *fn)
             if (off[c + 1] > off[c]) pc->lower.lv.push_back(UcfdLevel{off[c], off[c + 1] - off[c], false});
         for (int c = pc->ncolors - 1; c >= 0; --c)
             if (off[c + 1] > off[c]) pc->upper.lv.push_back(UcfdLevel{off[c], off[c + 1] - off[c], false});
+        set_wave_blocks(pc->lower, false, pc->blk, rp, ds);
+        set_wave_blocks(pc->upper, true, pc->blk, rp, ds);
         pc->init32 = unvis_l;
         return UCFD_OK;
     }
@@ -282,6 +307,8 @@ int make_schedules(Precon pc, const char *fn)
     }
     ucfd_build_levels(bn, useq, jb, je, ci, levels, &unvis_u);
     if ((rc = ucfd_finish_schedule(levels, pc->upper, fn))) return rc;
+    set_wave_blocks(pc->lower, false, pc->blk, rp, ds);
+    set_wave_blocks(pc->upper, true, pc->blk, rp, ds);
     pc->init32 = unvis_l;
     return UCFD_OK;
 }

@@ -32,17 +32,19 @@ UCFD_HIDDEN int kry_bsgs_prepare(hipStream_t s, int bn, int blk, int64_t nnz, co
 // One sweep over rows base..base+cnt-1 (or order[base..base+cnt-1] when
 // order != NULL).  lower: x_i = D_i^-1 (b_i - sum_{j<diag} A_ij x_j);
 // upper: x_i -= D_i^-1 sum_{j>diag} A_ij x_j.  FP64 form: in place on b.
+// maxw: the most blocks the rows of one wave (64/blk consecutive rows) use
+// in this direction (UcfdLevel::maxw; -1 = unknown): sizes the LDS staging.
 UCFD_HIDDEN int kry_bsgs_sweep64(hipStream_t s, int upper, int blk, int base, int cnt,
                                  const int *order, const int *rowptr, const int *colidx,
                                  const int *diagslots, const double *values,
-                                 const double *diagvalues, double *b);
+                                 const double *diagvalues, double *b, int maxw);
 // Mixed form: block products in FP32 on FP32 values and an FP32 work vector
 // x32, the diagonal solve in FP64.  lower reads the right-hand side b (FP64)
 // and writes x32; upper updates x32 and writes the final value to b (FP64).
 UCFD_HIDDEN int kry_bsgs_sweep32(hipStream_t s, int upper, int blk, int base, int cnt,
                                  const int *order, const int *rowptr, const int *colidx,
                                  const int *diagslots, const float *vals32,
-                                 const double *diagvalues, float *x32, double *b);
+                                 const double *diagvalues, float *x32, double *b, int maxw);
 
 // BLAS-1/2 on device vectors.  Reductions write to out (device) in a fixed
 // order; part must hold kry_parts(n) doubles per reduced quantity.

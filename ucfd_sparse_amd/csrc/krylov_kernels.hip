@@ -324,9 +324,10 @@ __global__ __launch_bounds__(KB) void k_sweep32(int base, int cnt, const int *__
 // the row's lower (lower sweep) or upper (upper sweep) blocks in the order
 // of k_sweep64 / k_sweep32 above; the group then exchanges its B sums with
 // lane shuffles and every lane runs the row's diagonal solve (bit-identical
-// to the thread-per-row kernels).  When the wave's blocks fit KSTAGE words
-// they are staged through LDS (per row, contiguous loads), with the rows'
-// LU blocks; otherwise the lanes read them from HBM directly.
+// to the thread-per-row kernels).  When the wave's blocks fit the cap words
+// of LDS the launch provides (sized from the level's largest wave) they are
+// staged through LDS (per row, contiguous loads); otherwise the lanes read
+// them from HBM directly.  The rows' LU blocks are always staged.
 template <int B, bool UPPER, bool F32, bool STAGED>
 __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0, int n, int off,
                                           const int *__restrict__ colidx,
@@ -401,7 +402,8 @@ __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0,
 }
 
 template <int B, bool UPPER, bool F32>
-__global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, const int *__restrict__ rowptr,
+__global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
+                                               const int *__restrict__ rowptr,
                                                const int *__restrict__ colidx,
                                                const int *__restrict__ diagslots,
                                                const typename std::conditional<F32, float,
@@ -411,10 +413,12 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, const int *__r
                                                double *b)
 {
     using T = typename std::conditional<F32, float, double>::type;
-    constexpr int B2 = B*B, G = 64/B, CAP = KSTAGE*8/(int)sizeof(T), PER = (5*B2 + 63)/64;
+    constexpr int B2 = B*B, G = 64/B, PER = (5*B2 + 63)/64;
     constexpr int NLU = (G*B2 + 63)/64;
-    __shared__ T sv[CAP];
-    __shared__ double slu[G*B2];
+    // dynamic LDS: the rows' LU blocks, then cap staged block words
+    extern __shared__ __align__(16) unsigned char kdyn[];
+    double *slu = (double *)kdyn;
+    T *sv = (T *)(kdyn + G*B2*sizeof(double));
     const int lane = threadIdx.x;
     const int t0 = ucfd::wave_xcd_index()*G;
     if (t0 >= cnt) return;
@@ -433,7 +437,7 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, const int *__r
     const int total = __shfl(ex + lead, 63, 64);
     const int off = __shfl(ex, g*B < 64 ? g*B : 0, 64);
     const int nrows = min(G, cnt - t0);
-    if (total*B2 > CAP) {
+    if (total*B2 > cap) {
         bsgs_rows<B, UPPER, F32, false>(r, g, act, i, j0, n, 0, colidx, vals,
                                         dv + (int64_t)i*B2, x32, b);
         return;
@@ -692,18 +696,33 @@ template <int B> struct PrepL {
     }
 };
 
+// LDS words for the staged blocks of a wave-staged sweep launch
+template <int B, typename T> int stage_cap(int maxw)
+{
+    const int full = KSTAGE*8/(int)sizeof(T);
+    return maxw < 0 ? full : (maxw*B*B < full ? maxw*B*B : full);
+}
+
+template <int B, typename T> size_t stage_bytes(int cap)
+{
+    return (size_t)(64/B)*B*B*sizeof(double) + (size_t)cap*sizeof(T);
+}
+
 template <int B> struct Sw64L {
     static int run(hipStream_t s, int upper, int base, int cnt, const int *ord, const int *rp,
-                   const int *ci, const int *ds, const double *v, const double *dv, double *b)
+                   const int *ci, const int *ds, const double *v, const double *dv, double *b,
+                   int maxw)
     {
         if (B > 1 && !ord) {
             const dim3 g((cnt + 64/B - 1)/(64/B));
+            const int cap = stage_cap<B, double>(maxw);
+            const size_t lds = stage_bytes<B, double>(cap);
             if (upper)
-                hipLaunchKernelGGL((k_bsgs_w<B, true, false>), g, dim3(64), 0, s, base, cnt, rp, ci,
-                                   ds, v, dv, nullptr, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, true, false>), g, dim3(64), lds, s, base, cnt, cap,
+                                   rp, ci, ds, v, dv, nullptr, b);
             else
-                hipLaunchKernelGGL((k_bsgs_w<B, false, false>), g, dim3(64), 0, s, base, cnt, rp, ci,
-                                   ds, v, dv, nullptr, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, false, false>), g, dim3(64), lds, s, base, cnt, cap,
+                                   rp, ci, ds, v, dv, nullptr, b);
             return ok();
         }
         const dim3 g((cnt + KB - 1)/KB);
@@ -720,16 +739,18 @@ template <int B> struct Sw64L {
 template <int B> struct Sw32L {
     static int run(hipStream_t s, int upper, int base, int cnt, const int *ord, const int *rp,
                    const int *ci, const int *ds, const float *v, const double *dv, float *x32,
-                   double *b)
+                   double *b, int maxw)
     {
         if (B > 1 && !ord) {
             const dim3 g((cnt + 64/B - 1)/(64/B));
+            const int cap = stage_cap<B, float>(maxw);
+            const size_t lds = stage_bytes<B, float>(cap);
             if (upper)
-                hipLaunchKernelGGL((k_bsgs_w<B, true, true>), g, dim3(64), 0, s, base, cnt, rp, ci,
-                                   ds, v, dv, x32, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, true, true>), g, dim3(64), lds, s, base, cnt, cap,
+                                   rp, ci, ds, v, dv, x32, b);
             else
-                hipLaunchKernelGGL((k_bsgs_w<B, false, true>), g, dim3(64), 0, s, base, cnt, rp, ci,
-                                   ds, v, dv, x32, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, false, true>), g, dim3(64), lds, s, base, cnt, cap,
+                                   rp, ci, ds, v, dv, x32, b);
             return ok();
         }
         const dim3 g((cnt + KB - 1)/KB);
@@ -775,20 +796,21 @@ int kry_bsgs_prepare(hipStream_t s, int bn, int blk, int64_t nnz, const int *dia
 
 int kry_bsgs_sweep64(hipStream_t s, int upper, int blk, int base, int cnt, const int *order,
                      const int *rowptr, const int *colidx, const int *diagslots,
-                     const double *values, const double *diagvalues, double *b)
+                     const double *values, const double *diagvalues, double *b, int maxw)
 {
     if (cnt <= 0) return UCFD_OK;
     return by_block<Sw64L>(blk, s, upper, base, cnt, order, rowptr, colidx, diagslots, values,
-                           diagvalues, b);
+                           diagvalues, b, maxw);
 }
 
 int kry_bsgs_sweep32(hipStream_t s, int upper, int blk, int base, int cnt, const int *order,
                      const int *rowptr, const int *colidx, const int *diagslots,
-                     const float *vals32, const double *diagvalues, float *x32, double *b)
+                     const float *vals32, const double *diagvalues, float *x32, double *b,
+                     int maxw)
 {
     if (cnt <= 0) return UCFD_OK;
     return by_block<Sw32L>(blk, s, upper, base, cnt, order, rowptr, colidx, diagslots, vals32,
-                           diagvalues, x32, b);
+                           diagvalues, x32, b, maxw);
 }
 
 int kry_parts(int64_t n) { return KMAXPART; }

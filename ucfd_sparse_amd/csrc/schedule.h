@@ -17,6 +17,7 @@
 struct UcfdLevel {
     int base, cnt;
     bool ordered;   // rows are order[base .. base+cnt) instead of base .. base+cnt-1
+    int maxw = -1;  // wave-staged kernels: most blocks one wave's rows use (-1: unknown)
 };
 
 struct UcfdSchedule {
