@@ -265,7 +265,12 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
 // writes fl32(r) and the FP32 base state fl32(Q + dq) for the inner loop.
 // Launched once per colour range [n0, n0+cnt) like the sweeps: all XCDs
 // then work on slices of one colour and its neighbours at a time.
-template <int FC, bool OUT32>
+// ZERO: dq is known to be zero (the first outer step).  Every face term of a
+// zero increment is +0 for finite states (a flux difference of identical
+// states, or products with zero summed onto +0), so r = b - (D*0 + 0.5*0)
+// bit for bit without the neighbour gathers; the partial sums keep the
+// same blocks and order, so the norm is identical too.
+template <int FC, bool OUT32, bool ZERO = false>
 __global__ __launch_bounds__(kSB) void k_residual64(
     int n0, int cnt, int N, const int *__restrict__ nei, const Face4<double> *__restrict__ geo,
     const double *__restrict__ diag, const double *__restrict__ q, const double *__restrict__ b,
@@ -295,7 +300,7 @@ __global__ __launch_bounds__(kSB) void k_residual64(
 #pragma unroll
         for (int f = 0; f < FC; ++f) {
             const int nb = nbv[f];
-            in_f[f] = nb != i;
+            in_f[f] = !ZERO && nb != i;
             if (!in_f[f]) continue;
             g[f] = ldg_nt(geo + (size_t)f*N + i);
 #pragma unroll
@@ -490,18 +495,27 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
+// zero_dq: dq is known to be zero (only with out32, the first outer step)
 int residual64(ucfd_lusgs_solver *s, hipStream_t st, const double *b, const double *dq, double *r,
-               bool out32)
+               bool out32, bool zero_dq = false)
 {
     int pofs = 0;
     for (int c = 0; c < s->nc; ++c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
-#define R(FC, O) hipLaunchKernelGGL((k_residual64<FC, O>), g, dim3(kSB), 0, st, n0, cnt, s->N, s->nei, \
-                                    s->g64, s->d64, s->q64, b, dq, r, s->r32, s->qb32, s->part, pofs)
-        if (s->F == 4) { if (out32) R(4, true); else R(4, false); }
-        else           { if (out32) R(6, true); else R(6, false); }
+#define R(FC, O, Z) hipLaunchKernelGGL((k_residual64<FC, O, Z>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
+                                       s->nei, s->g64, s->d64, s->q64, b, dq, r, s->r32, s->qb32, \
+                                       s->part, pofs)
+        if (s->F == 4) {
+            if (out32 && zero_dq) R(4, true, true);
+            else if (out32) R(4, true, false);
+            else R(4, false, false);
+        } else {
+            if (out32 && zero_dq) R(6, true, true);
+            else if (out32) R(6, true, false);
+            else R(6, false, false);
+        }
 #undef R
         pofs += (int)g.x;
     }
@@ -691,7 +705,7 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     } else {
         // outer FP64 residual; inner FP32 defect correction on the correction
         while (!rc && info.passes < maxiter) {
-            rc = residual64(s, st, B, DQ, nullptr, true);
+            rc = residual64(s, st, B, DQ, nullptr, true, info.outer == 0);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
             if (info.outer == 0) bnorm = rn;
