@@ -48,9 +48,11 @@ UCFD_EXTERN ucfd_status_t UCFDSolverCreateBICGSTAB(Solver *solver, UCFDInt n, UC
 UCFD_EXTERN ucfd_status_t UCFDSolverCreateCUDAGMRES(Solver *solver, UCFDInt n, UCFDInt m,
                                                     UCFDInt maxiter, UCFDReal tol);
 
-/* Extension: right = true switches GMRES to right preconditioning, and both
- * solvers to the relative tolerance tol*||b||_2; GMRES then also leaves a
- * restart cycle as soon as the Givens residual estimate reaches it. */
+/* Extension: right = true switches GMRES to flexible right preconditioning
+ * (the preconditioned basis M^-1 v_j is kept, m more vectors of n doubles,
+ * so a preconditioner applied in FP32 keeps the FP64 Arnoldi relation), and
+ * both solvers to the relative tolerance tol*||b||_2; GMRES then also leaves
+ * a restart cycle as soon as the Givens residual estimate reaches it. */
 UCFD_EXTERN ucfd_status_t UCFDSolverSetRightPrecon(Solver solver, UCFDBool right);
 
 #if defined(__cplusplus)

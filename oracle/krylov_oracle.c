@@ -243,8 +243,9 @@ static void o_spmv(const OrcSystem *S, double alpha, const double *x, double bet
 /*
  * gmres.c:42-129.  right == 0: the reference (left preconditioning, absolute
  * tolerance on the true residual at the start of each restart, maxiter counts
- * restarts).  right == 1: right preconditioning, tolerance tol*||b||, early
- * exit inside a cycle when the Givens residual estimate drops below it.
+ * restarts).  right == 1: flexible right preconditioning (z_j = M^-1 v_j
+ * kept, x += Z y), tolerance tol*||b||, early exit inside a cycle when the
+ * Givens residual estimate drops below it.
  * Returns the restart count; stat: 0 converged, 1 happy breakdown,
  * 2 maxiter; hist[k] = residual at restart k (maxiter entries).
  */
@@ -259,7 +260,7 @@ ORC_EXPORT int orc_gmres(const OrcSystem *S, int m, int maxiter, double tol, int
     double *y = calloc(m + 1, sizeof(double)), *cs = calloc(m + 1, sizeof(double));
     double *sn = calloc(m + 1, sizeof(double)), *h2 = calloc(m + 1, sizeof(double));
     double *r = calloc(n, sizeof(double)), *w = calloc(n, sizeof(double));
-    double *z = calloc(n, sizeof(double));
+    double *Z = right ? calloc((size_t)n*m, sizeof(double)) : NULL;
     int iter = 0, st = -1;
     double abeta = 0.0;
     const double tolv = right ? tol*o_nrm2(n, b) : tol;
@@ -280,9 +281,10 @@ ORC_EXPORT int orc_gmres(const OrcSystem *S, int m, int maxiter, double tol, int
             const int kk = j + 1;
             double *vj = V + (size_t)j*n, *Hcol = H + (size_t)j*ld;
             if (right) {
-                memcpy(z, vj, sizeof(double)*n);
-                o_apply(S, z);
-                o_spmv(S, 1.0, z, 0.0, w);
+                double *zj = Z + (size_t)j*n;
+                memcpy(zj, vj, sizeof(double)*n);
+                o_apply(S, zj);
+                o_spmv(S, 1.0, zj, 0.0, w);
             } else {
                 o_spmv(S, 1.0, vj, 0.0, w);
                 o_apply(S, w);
@@ -321,10 +323,7 @@ ORC_EXPORT int orc_gmres(const OrcSystem *S, int m, int maxiter, double tol, int
             y[i] = sum/H[i + (size_t)i*ld];
         }
         if (right) {
-            memset(z, 0, sizeof(double)*n);
-            o_gemv(n, k, n, 1.0, V, y, 1.0, z);
-            o_apply(S, z);
-            o_axpy(n, 1.0, z, x);
+            o_gemv(n, k, n, 1.0, Z, y, 1.0, x);
         } else {
             o_gemv(n, k, n, 1.0, V, y, 1.0, x);
         }
@@ -334,7 +333,7 @@ ORC_EXPORT int orc_gmres(const OrcSystem *S, int m, int maxiter, double tol, int
     if (iter == maxiter) st = 2;
     *stat = st;
     *res_out = abeta;
-    free(V); free(H); free(y); free(cs); free(sn); free(h2); free(r); free(w); free(z);
+    free(V); free(H); free(y); free(cs); free(sn); free(h2); free(r); free(w); free(Z);
     return iter;
 }
 

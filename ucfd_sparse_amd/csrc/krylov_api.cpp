@@ -73,6 +73,7 @@ struct _Solver {
     int n, m;
     double *vec;             // device work vectors (one allocation)
     double *V, *w, *r, *z;   // GMRES
+    double *Z;               // GMRES, right mode: the preconditioned basis M^-1 v_j
     double *rt, *p, *pt, *v, *s, *shat, *t;   // BiCGSTAB (r shared)
     double *part;            // reduction partials
     double *dbuf;            // device scalars
@@ -369,8 +370,11 @@ void record(Solver S, int iter, double res)
     if (S->hist_residual && iter >= 0 && iter < S->maxiter) S->hist_residual[iter] = res;
 }
 
-// gmres.c:42-129; right mode: right preconditioning, tolerance tol*||b||,
-// exit inside a cycle when the Givens estimate reaches it.
+// gmres.c:42-129; right mode: flexible right preconditioning (the
+// preconditioned basis z_j = M^-1 v_j is kept and x += Z y, so the Arnoldi
+// relation A Z = V H holds in FP64 even when M^-1 is applied in FP32 and is
+// not exactly linear), tolerance tol*||b||, exit inside a cycle when the
+// Givens estimate reaches it.
 int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, double *x,
           const double *b, hipStream_t s, const char *fn)
 {
@@ -378,7 +382,7 @@ int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, doubl
     const double tol = S->tol, haptol = S->haptol;
     const int64_t N = n;
     std::vector<double> H((size_t)ld*m, 0.0), y(ld, 0.0), cs(ld, 0.0), sn(ld, 0.0);
-    double *V = S->V, *w = S->w, *r = S->r, *z = S->z, *part = S->part, *db = S->dbuf;
+    double *V = S->V, *w = S->w, *r = S->r, *part = S->part, *db = S->dbuf;
     double *hb = S->hbuf;
     int rc, iter = 0, k = 0;
     double abeta = 0.0, tolv = tol;
@@ -414,8 +418,10 @@ int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, doubl
             const int kk = j + 1;
             double *vj = V + (size_t)j*n;
             if (S->right) {
-                if ((rc = kry_copy(s, N, vj, z)) || (rc = pc_apply(s, pc, mp, z)) ||
-                    (rc = spmv(s, A, ma, 1.0, z, 0.0, w)))
+                // flexible form: z_j = M^-1 v_j is kept, x is updated with Z y
+                double *zj = S->Z + (size_t)j*n;
+                if ((rc = kry_copy(s, N, vj, zj)) || (rc = pc_apply(s, pc, mp, zj)) ||
+                    (rc = spmv(s, A, ma, 1.0, zj, 0.0, w)))
                     return rc;
             } else {
                 if ((rc = spmv(s, A, ma, 1.0, vj, 0.0, w)) || (rc = pc_apply(s, pc, mp, w)))
@@ -467,9 +473,7 @@ int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, doubl
                                          hipMemcpyHostToDevice, s), fn, "hipMemcpyAsync(y)")))
                 return rc;
             if (S->right) {
-                if ((rc = kry_fill(s, N, 0.0, z)) ||
-                    (rc = kry_gemv(s, N, k, V, N, 1.0, db + DB_Y, z, part, nullptr)) ||
-                    (rc = pc_apply(s, pc, mp, z)) || (rc = kry_axpy(s, N, 1.0, z, x)))
+                if ((rc = kry_gemv(s, N, k, S->Z, N, 1.0, db + DB_Y, x, part, nullptr)))
                     return rc;
             } else if ((rc = kry_gemv(s, N, k, V, N, 1.0, db + DB_Y, x, part, nullptr))) {
                 return rc;
@@ -841,9 +845,10 @@ UCFD_API ucfd_status_t UCFDSolverDestroy(Solver *solver)
 {
     if (!solver || !*solver) return UCFD_SUCCESS;
     Solver s = *solver;
-    if (s->vec) {
+    if (s->vec || s->Z) {
         (void)hipStreamSynchronize(ucfd_stream());
-        (void)hipFree(s->vec);
+        if (s->vec) (void)hipFree(s->vec);
+        if (s->Z) (void)hipFree(s->Z);
     }
     if (s->hbuf) (void)hipHostFree(s->hbuf);
     free(s->hist_residual);
@@ -875,6 +880,11 @@ UCFD_API ucfd_status_t UCFDSolverSetRightPrecon(Solver solver, UCFDBool right)
     if (!solver || !solver->type_name)
         return fail("UCFDSolverSetRightPrecon", "Solver must be initialized", UCFD_ERR_STATE);
     solver->right = right;
+    // flexible GMRES keeps the m preconditioned basis vectors
+    if (right && !solver->Z && strcmp(solver->type_name, GMRES) == 0 &&
+        hip(hipMalloc((void **)&solver->Z, sizeof(double)*((size_t)solver->n*solver->m + 1)),
+            "UCFDSolverSetRightPrecon", "hipMalloc(preconditioned basis)"))
+        return UCFD_FAILED;
     return UCFD_SUCCESS;
 }
 
