@@ -125,9 +125,14 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
     // all faces' loads in flight at once for tets; pairs of faces for hexes
     constexpr int G = FC == 4 ? 4 : 2;
     static_assert(FC % G == 0, "face count must be a multiple of the group size");
+    // MODE 3: a whole colour whose every neighbour lies in the sweep direction
+    // (the last colour of a lower sweep, the first of an upper sweep): the
+    // face geometry of all faces is requested before the neighbour indices
+    // arrive; the gathers stay predicated.  Same arithmetic as MODE 2.
+    constexpr bool RANGE = MODE >= 2, EAGER = MODE == 3;
     const int t = xcd_block()*kSweepBlock + threadIdx.x;
     if (t >= cnt) return;
-    const int i = MODE == 2 ? n0 + t : ld_stream(order + n0 + t);
+    const int i = RANGE ? n0 + t : ld_stream(order + n0 + t);
     if ((unsigned)i >= (unsigned)N) {
         raise_flag(flag, UCFD_KFLAG_INDEX);
         return;
@@ -139,6 +144,16 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
     bool tk[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) nbv[f] = ld_stream(nei + (size_t)f*N + i);
+    double enf[EAGER ? FC : 1][ND], efv[EAGER ? FC : 1], elam[EAGER ? FC : 1];
+    if (EAGER) {
+#pragma unroll
+        for (int f = 0; f < (EAGER ? FC : 0); ++f) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) enf[f][d] = ld_stream(vfn + (size_t)(f*ND + d)*N + i);
+            efv[f] = ld_stream(fnv + (size_t)f*N + i);
+            elam[f] = ld_stream(fspr + (size_t)f*N + i);
+        }
+    }
     // the cell's own rows are loaded before the neighbour gathers so their
     // latency overlaps (in may alias out: only other cells' out is gathered)
     const double dg = ld_stream(diag + i);
@@ -177,10 +192,17 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
                 u[g][k] = upts[(size_t)k*N + nb];
                 du[g][k] = out[(size_t)k*N + nb];
             }
+            if (EAGER) {
 #pragma unroll
-            for (int d = 0; d < ND; ++d) nf[g][d] = ld_stream(vfn + (size_t)(f*ND + d)*N + i);
-            fv[g] = ld_stream(fnv + (size_t)f*N + i);
-            lam[g] = ld_stream(fspr + (size_t)f*N + i);
+                for (int d = 0; d < ND; ++d) nf[g][d] = enf[EAGER ? f : 0][d];
+                fv[g] = efv[EAGER ? f : 0];
+                lam[g] = elam[EAGER ? f : 0];
+            } else {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) nf[g][d] = ld_stream(vfn + (size_t)(f*ND + d)*N + i);
+                fv[g] = ld_stream(fnv + (size_t)f*N + i);
+                lam[g] = ld_stream(fspr + (size_t)f*N + i);
+            }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -211,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_any(
     constexpr int NV = ND + 2;
     const int t = xcd_block()*kBlock + threadIdx.x;
     if (t >= cnt) return;
-    const int i = MODE == 2 ? n0 + t : order[n0 + t];
+    const int i = MODE >= 2 ? n0 + t : order[n0 + t];
     if ((unsigned)i >= (unsigned)N) {
         raise_flag(flag, UCFD_KFLAG_INDEX);
         return;
@@ -340,6 +362,16 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
                                     vec_fnorm, upts, in, out, diag, fspr, flag);
     if (dir == 1 && mode == UCFD_SWEEP_COLORED)
         return launch_sweep_d<1, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    // a whole colour with every neighbour in the sweep direction: the last
+    // colour of a lower sweep, the first of an upper sweep (tets and hexes)
+    const bool all_taken = mode == UCFD_SWEEP_RANGE && (nface == 4 || nface == 6) &&
+                           (dir == 0 ? ne == neles : n0 == 0);
+    if (dir == 0 && all_taken)
+        return launch_sweep_d<0, 3>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
+                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+    if (dir == 1 && all_taken)
+        return launch_sweep_d<1, 3>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
                                     vec_fnorm, upts, in, out, diag, fspr, flag);
     if (dir == 0 && mode == UCFD_SWEEP_RANGE)
         return launch_sweep_d<0, 2>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
