@@ -40,9 +40,17 @@ UCFD_EXTERN ucfd_status_t UCFDFlowSysSetElement(FlowSys *sys, UCFDInt eidx, UCFD
 /* flowsys.c:6-15 */
 UCFD_EXTERN ucfd_status_t UCFDFlowSysDestroy(FlowSys *sys);
 
-/* ucfdlusgs.h:17-25: the scalar LU-SGS method of the FlowSys is not part of
- * this library (the legacy array API in coloredlusgs.h / lusgs.h carries the
- * scalar path); these return UCFD_FAILED. */
+/* ucfdlusgs.h:17-25 and flowsys.c:64-81: the scalar LU-SGS method of the
+ * FlowSys.  fspr / tfspr: spectral radius per face slot (the pointers are
+ * kept and read at every Prepare and sweep); rank arrays u / du / diag are
+ * library-owned [nvars, nlocal].  Pack (lusgs.c:8-52): per element, u = uptsb,
+ * du = rhs, diag = 1/(dt*factor) + a0 + dsrc with factor = turb_factor on the
+ * turbulence rows.  Prepare (lusgs.c:87-140): diag += sum over the cell's
+ * positions of 0.5*kappa*fspr*face_area*rcp_vol (NS rows, or turbulence rows
+ * with tfspr).  Sweeps (lusgs.c:157-321): in-place Gauss-Seidel over the lower
+ * (upper) neighbours, NS rows with the Euler flux difference, turbulence rows
+ * with the RANS convective flux (flux.c:77-90; 1 to 4 turbulence rows).
+ * Update (lusgs.c:54-84): uptsb += du.  ndims 2 or 3, nfvars = ndims + 2. */
 UCFD_EXTERN ucfd_status_t UCFDFlowSysSetLUSGS(FlowSys *sys, UCFDReal *fspr, UCFDReal *tfspr);
 UCFD_EXTERN ucfd_status_t UCFDLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal factor, UCFDReal a0);
 UCFD_EXTERN ucfd_status_t UCFDLUSGS_Update(FlowSys sys, UCFDInt eidx);
@@ -52,6 +60,9 @@ UCFD_EXTERN ucfd_status_t UCFDLUSGS_NSLowerSweep(FlowSys sys, UCFDReal kappa);
 UCFD_EXTERN ucfd_status_t UCFDLUSGS_NSUpperSweep(FlowSys sys, UCFDReal kappa);
 UCFD_EXTERN ucfd_status_t UCFDLUSGS_RANSLowerSweep(FlowSys sys, UCFDReal kappa);
 UCFD_EXTERN ucfd_status_t UCFDLUSGS_RANSUpperSweep(FlowSys sys, UCFDReal kappa);
+/* Extension: copy a library-owned scalar LU-SGS array to out (host or
+ * device): which 0 = u, 1 = du, 2 = diag (each [nvars, nlocal]). */
+UCFD_EXTERN ucfd_status_t UCFDLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal *out);
 
 /* flowsys.c:91-117: jmat [2][nfvars][nfvars][nfaces] (A+, A- per face),
  * tjmat [2][nturbvars][nturbvars][nfaces] or NULL */

@@ -38,7 +38,7 @@
 
 /* flux.c:29-67: Euler normal flux with the pressure floor.  u may be
  * modified (energy rewritten when clamped), as in the reference. */
-static void orc_ns_flux(int nfvars, int ndims, double *u, const double *nf, double *f)
+void orc_ns_flux(int nfvars, int ndims, double *u, const double *nf, double *f)
 {
     double rho = u[0];
     double et = u[nfvars - 1];

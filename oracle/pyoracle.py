@@ -379,6 +379,99 @@ def flowsys_run(fs, tjmat, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kind="sa"
     return out
 
 
+# ------------------------------------------------- FlowSys scalar LU-SGS
+def _fsllib():
+    L = lib()
+    if not hasattr(L, "_fsl_bound"):
+        vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+        L.orc_fsl_pack.argtypes = [ci, ci, ci, ci, cd, cd] + [vp] * 8
+        L.orc_fsl_update.argtypes = [ci, ci, ci, vp, vp, vp]
+        L.orc_fsl_prepare.argtypes = [ci, ci, ci, cd] + [vp] * 6
+        L.orc_fsl_sweep.argtypes = [ci, ci, cd, vp, ci, ci, ci, ci, cl] + [vp] * 10
+        for f in ("orc_fsl_pack", "orc_fsl_update", "orc_fsl_prepare", "orc_fsl_sweep"):
+            getattr(L, f).restype = None
+        L._fsl_bound = True
+    return L
+
+
+class FlowSysLUSGSOracle:
+    """The FlowSys scalar LU-SGS state (rank u / du / diag, [nvars, nlocal])
+    and its operations, restated in oracle/flowsys_lusgs_oracle.c."""
+
+    def __init__(self, fs, fspr, tfspr, el, nvars, nfvars):
+        self.L = _fsllib()
+        self.fs, self.el = fs, el
+        self.fspr = _c(fspr, np.float64)
+        self.tfspr = None if tfspr is None else _c(tfspr, np.float64)
+        self.nvars, self.nfvars = nvars, nfvars
+        N = fs.nlocal
+        self.u = np.zeros(N * nvars)
+        self.du = np.zeros(N * nvars)
+        self.diag = np.zeros(N * nvars)
+        self.cid = _c(el["cell_ids"], np.int32)
+        self.upts = np.array(el["uptsb"], np.float64)
+        self.g = [_c(fs.rowptr, np.int32), _c(fs.colidx, np.int32), _c(fs.sides, np.int8),
+                  _c(fs.slots, np.int32), _c(fs.face_area, np.float64),
+                  _c(fs.face_normal, np.float64), _c(fs.rcp_vol, np.float64)]
+
+    def pack(self, turb_factor, a0):
+        e = self.el
+        self.L.orc_fsl_pack(self.fs.nlocal, len(self.cid), self.nvars, self.nfvars, turb_factor, a0,
+                            self.cid.ctypes.data, self.upts.ctypes.data,
+                            _c(e["rhs"], np.float64).ctypes.data, _c(e["dt"], np.float64).ctypes.data,
+                            _c(e["dsrc"], np.float64).ctypes.data, self.u.ctypes.data,
+                            self.du.ctypes.data, self.diag.ctypes.data)
+
+    def prepare(self, kappa, turb=False):
+        rp, ci, sd, sl, fa, fn, rv = self.g
+        nv0, nv1 = (self.nfvars, self.nvars) if turb else (0, self.nfvars)
+        lam = self.tfspr if turb else self.fspr
+        self.L.orc_fsl_prepare(self.fs.nlocal, nv0, nv1, kappa, rp.ctypes.data, sl.ctypes.data,
+                               fa.ctypes.data, rv.ctypes.data, lam.ctypes.data,
+                               self.diag.ctypes.data)
+
+    def sweep(self, upper, kappa, turb=False):
+        rp, ci, sd, sl, fa, fn, rv = self.g
+        lam = self.tfspr if turb else self.fspr
+        self.L.orc_fsl_sweep(int(upper), int(turb), kappa, lam.ctypes.data, self.fs.nlocal,
+                             self.nvars, self.nfvars, fn.shape[0], self.fs.nfaces, rp.ctypes.data,
+                             ci.ctypes.data, sd.ctypes.data, sl.ctypes.data, fa.ctypes.data,
+                             fn.ctypes.data, rv.ctypes.data, self.diag.ctypes.data,
+                             self.u.ctypes.data, self.du.ctypes.data)
+
+    def update(self):
+        self.L.orc_fsl_update(self.fs.nlocal, len(self.cid), self.nvars, self.cid.ctypes.data,
+                              self.du.ctypes.data, self.upts.ctypes.data)
+
+
+def flowsys_lusgs_run(fs, fspr, tfspr, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kappa=1.2,
+                      iterations=2):
+    """The stage sequence of tests/golden/refflowsys.run_lusgs on the oracle."""
+    O = FlowSysLUSGSOracle(fs, fspr, tfspr, el, nvars, nfvars)
+    nt = nvars - nfvars
+    out = {}
+    O.pack(turb_factor, a0)
+    out["u"], out["du_pack"], out["diag_pack"] = O.u.copy(), O.du.copy(), O.diag.copy()
+    O.prepare(kappa)
+    out["diag_ns"] = O.diag.copy()
+    if nt:
+        O.prepare(kappa, turb=True)
+        out["diag"] = O.diag.copy()
+    for it in range(iterations):
+        O.sweep(False, kappa)
+        out[f"du_nslower{it}"] = O.du.copy()
+        O.sweep(True, kappa)
+        out[f"du_nsupper{it}"] = O.du.copy()
+        if nt:
+            O.sweep(False, kappa, turb=True)
+            out[f"du_ranslower{it}"] = O.du.copy()
+            O.sweep(True, kappa, turb=True)
+            out[f"du_ransupper{it}"] = O.du.copy()
+    O.update()
+    out["upts"] = O.upts.copy()
+    return out
+
+
 # ---------------------------------------------------------------- adjoint
 def _adjlib():
     L = lib()

@@ -103,3 +103,68 @@ def run(fs, tjmat, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kind="sa", iterat
     out["upts"] = upts.copy()
     lib.UCFDFlowSysDestroy(ctypes.byref(sysp))
     return out
+
+
+def run_lusgs(fs, fspr, tfspr, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kappa=1.2,
+              iterations=2):
+    """The scalar LU-SGS sequence of the reference's FlowSys API
+    (UCFDFlowSysSetLUSGS, UCFDLUSGS_Pack, NS / RANS Prepare, per iteration NS
+    lower / upper and RANS lower / upper sweeps, Update; src/lusgs/lusgs.c),
+    reading the rank arrays u / du / diag [nvars, nlocal] through the struct
+    layout of src/lusgs/flowsys.h:20-24.  Returns the stages."""
+    lib = load()
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.UCFDLUSGS_Update.argtypes = [vp, ci]
+    for f in ("UCFDLUSGS_RANSPrepare", "UCFDLUSGS_NSLowerSweep", "UCFDLUSGS_NSUpperSweep",
+              "UCFDLUSGS_RANSLowerSweep", "UCFDLUSGS_RANSUpperSweep"):
+        getattr(lib, f).argtypes = [vp, cd]
+    N = fs.nlocal
+    keep = []
+
+    def c(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+    sysp = ctypes.c_void_p()
+    ndims = fs.face_normal.shape[0]
+    lib.UCFDFlowSysCreate(ctypes.byref(sysp), N, nvars, nfvars, ndims, fs.nfaces, 1,
+                          c(fs.rowptr, np.int32), c(fs.colidx, np.int32), c(fs.slots, np.int32),
+                          c(fs.sides, np.int8), c(fs.face_area, np.float64),
+                          c(fs.face_normal, np.float64), c(fs.rcp_vol, np.float64))
+    lib.UCFDFlowSysSetLUSGS(ctypes.byref(sysp), c(fspr, np.float64),
+                            None if tfspr is None else c(tfspr, np.float64))
+    upts = np.array(el["uptsb"], np.float64)
+    resid = np.zeros_like(upts)
+    lib.UCFDFlowSysSetElement(ctypes.byref(sysp), 0, len(el["cell_ids"]), 4,
+                              c(el["cell_ids"], np.int32), upts.ctypes.data,
+                              c(el["rhs"], np.float64), c(el["dt"], np.float64),
+                              c(el["dsrc"], np.float64), c(el["vol"], np.float64),
+                              resid.ctypes.data)
+    fsys = ctypes.cast(sysp, ctypes.POINTER(refbridge._FlowSys)).contents
+    Ls = ctypes.cast(fsys.data, ctypes.POINTER(refbridge._LUSGSSys)).contents
+    nt = nvars - nfvars
+
+    def arr(p):
+        return np.ctypeslib.as_array(p, shape=(N * nvars,)).copy()
+    out = {}
+    lib.UCFDLUSGS_Pack(sysp, 0, turb_factor, a0)
+    out["u"], out["du_pack"], out["diag_pack"] = arr(Ls.u), arr(Ls.du), arr(Ls.diag)
+    lib.UCFDLUSGS_NSPrepare(sysp, kappa)
+    out["diag_ns"] = arr(Ls.diag)
+    if nt:
+        lib.UCFDLUSGS_RANSPrepare(sysp, kappa)
+        out["diag"] = arr(Ls.diag)
+    for it in range(iterations):
+        lib.UCFDLUSGS_NSLowerSweep(sysp, kappa)
+        out[f"du_nslower{it}"] = arr(Ls.du)
+        lib.UCFDLUSGS_NSUpperSweep(sysp, kappa)
+        out[f"du_nsupper{it}"] = arr(Ls.du)
+        if nt:
+            lib.UCFDLUSGS_RANSLowerSweep(sysp, kappa)
+            out[f"du_ranslower{it}"] = arr(Ls.du)
+            lib.UCFDLUSGS_RANSUpperSweep(sysp, kappa)
+            out[f"du_ransupper{it}"] = arr(Ls.du)
+    lib.UCFDLUSGS_Update(sysp, 0)
+    out["upts"] = upts.copy()
+    lib.UCFDFlowSysDestroy(ctypes.byref(sysp))
+    return out

@@ -1,9 +1,12 @@
-"""FlowSys block LU-SGS on the GPU, through the C ABI (include/ucfdlusgs.h),
+"""FlowSys block and scalar LU-SGS on the GPU, through the C ABI (include/ucfdlusgs.h),
 against the CPU oracle on the same seeded inputs: every stage of an outer
 step (pack, turbulence pack, NS / RANS prepare, reset, lower / upper sweeps,
 sub-residual, update) bit for bit, on a randomly numbered mesh (many
 dependency levels) and on a colour-ordered one (one launch per colour), with
 host and device arrays.  The FP32 sweep form stays within 1e-5 of FP64."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -144,3 +147,53 @@ def test_state_and_index_errors():
     ref = O.flowsys_run(fs, tj, el, 6, 5, iterations=1)
     assert np.array_equal(got["du_upper0"], ref["du_upper0"])
     S.close()
+
+
+# ---------------------------------------------------------------- scalar LU-SGS
+def lusgs_case(seed=304, m=3, colored=False):
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_golden
+    return make_golden.flowsys_lusgs_case(seed=seed, m=m, colored=colored)
+
+
+@pytest.mark.parametrize("where,colored,m", [("host", False, 3), ("device", False, 3),
+                                             ("device", True, 4)])
+def test_lusgs_stages_bitwise(where, colored, m):
+    from ucfd_sparse_amd import flowsys as FS
+    say(f"scalar LU-SGS, arrays on {where}, colour-ordered={colored}, m={m}")
+    mesh, flow, fs, fspr, tfspr, el = lusgs_case(seed=304 + m, m=m, colored=colored)
+    ref = O.flowsys_lusgs_run(fs, fspr, tfspr, el, 6, 5, a0=0.2, turb_factor=1.7, kappa=1.1,
+                              iterations=3)
+    if where == "device":
+        fsd, _, eld = on_device(fs, np.zeros(1), el)
+        S = FS.FlowSysLUSGS(fsd, 6, 5, torch.from_numpy(fspr).cuda(),
+                            torch.from_numpy(tfspr).cuda())
+        S.set_element(0, eld)
+    else:
+        S = FS.FlowSysLUSGS(fs, 6, 5, fspr, tfspr)
+        S.set_element(0, el)
+    got = FS.run_lusgs(S, a0=0.2, turb_factor=1.7, kappa=1.1, iterations=3)
+    S.close()
+    assert sorted(got) == sorted(ref)
+    for k, v in ref.items():
+        g = np.asarray(got[k]).reshape(v.shape)
+        assert np.array_equal(g, v), k
+
+
+def test_lusgs_state_errors():
+    from ucfd_sparse_amd import flowsys as FS
+    from ucfd_sparse_amd import lib as L
+    say("scalar LU-SGS error paths")
+    mesh, flow, fs, fspr, tfspr, el = lusgs_case()
+    S = FS.FlowSysLUSGS(fs, 6, 5, fspr, None)
+    S.set_element(0, el)
+    S.call("UCFDLUSGS_Pack", 0, 1.5, 0.3)
+    with pytest.raises(L.UCFDError):
+        S.call("UCFDLUSGS_RANSPrepare", 1.0)       # no turbulence spectral radius
+    with pytest.raises(L.UCFDError):
+        S.call("UCFDLUSGS_Pack", 3, 1.5, 0.3)      # no such element
+    S.close()
+    B = FS.FlowSys(fs, 6, 5)                       # block LU-SGS handle
+    with pytest.raises(L.UCFDError):
+        B.call("UCFDLUSGS_NSPrepare", 1.0)
+    B.close()

@@ -130,3 +130,90 @@ def test_live_reference_on_a_colour_ordered_hex_mesh():
     ours = O.flowsys_run(fs, tj, el, 6, 5, a0=0.0, turb_factor=2.0, iterations=3)
     for k in ref:
         assert np.array_equal(ours[k], ref[k]), k
+
+
+# ---------------------------------------------------------------- scalar LU-SGS
+# oracle/flowsys_lusgs_oracle.c, pinned bit for bit against the reference's
+# FlowSys scalar LU-SGS (tests/golden/flowsys_lusgs_tet3p.npz, written by
+# make_golden.py --flowsys-lusgs), live against the reference build where it
+# exists, and against the legacy oracle through the renumbering bridge.
+FIXL = os.path.join(HERE, "golden", "flowsys_lusgs_tet3p.npz")
+
+
+@pytest.fixture(scope="module")
+def lcase():
+    return make_golden.flowsys_lusgs_case()
+
+
+@pytest.fixture(scope="module")
+def lfix():
+    return np.load(FIXL)
+
+
+def test_lusgs_fixture_inputs_regenerate(lcase, lfix):
+    mesh, flow, fs, fspr, tfspr, el = lcase
+    d = make_golden.input_digest(fs.rowptr, fs.colidx, fs.slots, fs.sides, fs.face_area,
+                                 fs.face_normal, fs.rcp_vol, fspr, tfspr, *el.values())
+    assert d == str(lfix["digest"])
+
+
+def test_lusgs_every_stage_bitwise_vs_reference(lcase, lfix):
+    mesh, flow, fs, fspr, tfspr, el = lcase
+    out = O.flowsys_lusgs_run(fs, fspr, tfspr, el, 6, 5)
+    assert sorted(out) == sorted(k for k in lfix.files if k != "digest")
+    for k, v in out.items():
+        assert np.array_equal(v, lfix[k]), k
+
+
+def test_lusgs_case_meets_the_pressure_floor(lcase):
+    """Some neighbour states of the NS sweeps have p < PMIN (the clamped
+    branch of flux.c:50-56 is part of what the fixture pins)."""
+    mesh, flow, fs, fspr, tfspr, el = lcase
+    up = el["uptsb"]
+    p = 0.4 * (up[4] - 0.5 * (up[1] ** 2 + up[2] ** 2 + up[3] ** 2) / up[0])
+    assert (p < 1e-13).sum() >= 10
+
+
+def test_lusgs_bridge_equals_the_legacy_oracle():
+    """With one slot per (cell, face), sides +1, face_area = fnorm_vol,
+    rcp_vol = 1, kappa = 1, a0 = dsrc = 0 and no turbulence rows, the FlowSys
+    scalar sweeps are the legacy serial sweeps (two independent restatements
+    of lusgs.c)."""
+    mesh = M.tet_mesh(4, seed=41, permute=True)
+    flow = M.flow_fields(mesh, seed=41)
+    N, F = mesh.neles, mesh.nface
+    nei = np.ascontiguousarray(mesh.nei_ele)
+    rowptr = np.arange(N + 1, dtype=np.int32) * F
+    fs = M.FlowSysMesh(N, F * N, rowptr, np.ascontiguousarray(nei.T.reshape(-1), np.int32),
+                       np.arange(F * N, dtype=np.int32), np.ones(F * N, np.int8),
+                       np.ascontiguousarray(mesh.fnorm_vol.T.reshape(-1)),
+                       np.ascontiguousarray(np.stack([mesh.vec_fnorm[:, d, :].T.reshape(-1)
+                                                      for d in range(3)])),
+                       np.ones(N), None)
+    lam = np.ascontiguousarray(flow.fspr.T.reshape(-1))
+    el = {"cell_ids": np.arange(N, dtype=np.int32), "uptsb": flow.upts.copy(),
+          "rhs": flow.rhsb.copy(), "dt": flow.dt.copy(), "dsrc": np.zeros_like(flow.upts),
+          "vol": np.ones(N)}
+    S = O.FlowSysLUSGSOracle(fs, lam, None, el, 5, 5)
+    S.pack(1.0, 0.0)
+    S.prepare(1.0)
+    S.sweep(False, 1.0)
+    dstar = S.du.reshape(5, N).copy()
+    S.sweep(True, 1.0)
+    dq = S.du.reshape(5, N).copy()
+    diag, ds, dqo, _ = O.serial_pass(nei, mesh.fnorm_vol, mesh.vec_fnorm, flow.fspr, flow.dt,
+                                     flow.upts, flow.rhsb)
+    assert np.array_equal(S.diag.reshape(5, N)[0], diag)
+    assert np.array_equal(dstar, ds)
+    assert np.array_equal(dq, dqo)
+
+
+@pytest.mark.skipif(not refflowsys.available(), reason="reference build absent")
+def test_lusgs_live_reference_colour_ordered():
+    mesh, flow, fs, fspr, tfspr, el = make_golden.flowsys_lusgs_case(seed=305, m=4, colored=True)
+    ref = refflowsys.run_lusgs(fs, fspr, tfspr, el, 6, 5, a0=0.1, turb_factor=2.0, kappa=0.8,
+                               iterations=3)
+    out = O.flowsys_lusgs_run(fs, fspr, tfspr, el, 6, 5, a0=0.1, turb_factor=2.0, kappa=0.8,
+                              iterations=3)
+    for k, v in ref.items():
+        assert np.array_equal(out[k], v), k

@@ -39,7 +39,7 @@ struct Packed {               // per-position blocks of one variable group
     bool ready = false;
 };
 
-enum { FS_NONE = 0, FS_BLUSGS = 2 };
+enum { FS_NONE = 0, FS_LUSGS = 1, FS_BLUSGS = 2 };
 
 }  // namespace
 
@@ -56,6 +56,10 @@ struct _FlowSys {
     float *du32 = nullptr;
     int *rowptr_d = nullptr, *colidx_d = nullptr;
     int *pos_sorted_d = nullptr, *pos_cell_d = nullptr;   // positions by face slot; cell of each
+    // scalar LU-SGS: rank arrays [nvars, nlocal] and the caller's per-slot
+    // spectral radii (src/lusgs/flowsys.h:20-24)
+    double *lu_u = nullptr, *lu_du = nullptr, *lu_diag = nullptr;
+    UCFDReal *fspr = nullptr, *tfspr = nullptr;
     int64_t npos = 0;
     bool fp32 = false;
     bool structured = false;
@@ -110,6 +114,9 @@ void free_device(FlowSys s)
     dfree(&s->colidx_d);
     dfree(&s->pos_sorted_d);
     dfree(&s->pos_cell_d);
+    dfree(&s->lu_u);
+    dfree(&s->lu_du);
+    dfree(&s->lu_diag);
     free_packed(s->ns);
     free_packed(s->rans);
     ucfd_free_schedule(s->lower);
@@ -118,6 +125,7 @@ void free_device(FlowSys s)
 }
 
 bool is_blusgs(FlowSys s) { return s && s->method == FS_BLUSGS; }
+bool is_lusgs(FlowSys s) { return s && s->method == FS_LUSGS; }
 
 // Validate the face-slot structure, copy it to the device and build the
 // forward (cells ascending) and backward (descending) schedules.
@@ -542,41 +550,190 @@ UCFD_API ucfd_status_t UCFDBLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal *o
 }
 
 // ======================================= scalar LU-SGS methods of the FlowSys
+//
+// src/lusgs/lusgs.c on the GPU: Pack / Update per element, Prepare and the
+// sweeps on the rank arrays u / du / diag [nvars, nlocal].  The sweeps are
+// the reference's sequential in-place Gauss-Seidel visits (rows ascending /
+// descending, lower / upper neighbours only), run level by level on the
+// schedules of the face-slot structure (one launch per colour on a
+// colour-ordered numbering): results equal the reference bit for bit.
+// fspr / tfspr (per face slot) are the caller's arrays, read at every call.
 
-#define UCFD_NOT_PROVIDED(name)                                                                  \
-    return fail(name, "the FlowSys scalar LU-SGS method is not part of this library; use the "  \
-                      "legacy array API (coloredlusgs.h / lusgs.h)", UCFD_ERR_STATE)
+UCFD_API ucfd_status_t UCFDFlowSysSetLUSGS(FlowSys *sys, UCFDReal *fspr, UCFDReal *tfspr)
+{
+    const char *fn = "UCFDFlowSysSetLUSGS";
+    if (!sys || !*sys) return fail(fn, "NULL system handle");
+    FlowSys s = *sys;
+    int rc = ucfd_ensure_device(fn);
+    if (rc) return UCFD_FAILED;
+    free_device(s);
+    s->method = FS_LUSGS;
+    s->fspr = fspr;
+    s->tfspr = tfspr;
+    const size_t nv = (size_t)s->nlocal*s->nvars;
+    if ((rc = dalloc(&s->lu_u, nv, fn)) || (rc = dalloc(&s->lu_du, nv, fn)) ||
+        (rc = dalloc(&s->lu_diag, nv, fn)))
+        return UCFD_FAILED;
+    // the reference's buffers come from malloc; start from zeros here
+    if ((rc = fs_launch_zero(ucfd_stream(), (int64_t)nv, s->lu_u, s->lu_du, nullptr)) ||
+        (rc = fs_launch_zero(ucfd_stream(), (int64_t)nv, s->lu_diag, s->lu_diag, nullptr)) ||
+        (rc = ucfd_sync_and_check(fn)))
+        return UCFD_FAILED;
+    return UCFD_SUCCESS;
+}
 
-UCFD_API ucfd_status_t UCFDFlowSysSetLUSGS(FlowSys *, UCFDReal *, UCFDReal *)
+UCFD_API ucfd_status_t UCFDLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal turb_factor, UCFDReal a0)
 {
-    UCFD_NOT_PROVIDED("UCFDFlowSysSetLUSGS");
+    const char *fn = "UCFDLUSGS_Pack";
+    if (!is_lusgs(sys)) return fail(fn, "LU-SGS is not set (UCFDFlowSysSetLUSGS)", UCFD_ERR_STATE);
+    FsElem *e = elem(sys, eidx, fn);
+    if (!e) return UCFD_FAILED;
+    const size_t ne = (size_t)e->neles, nv = (size_t)sys->nvars;
+    UcfdStage sg(fn);
+    const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
+    const int iu = sg.add(e->uptsb, sizeof(double)*ne*nv, UCFD_IN);
+    const int ir = sg.add(e->rhs, sizeof(double)*ne*nv, UCFD_IN);
+    const int it = sg.add(e->dt, sizeof(double)*ne, UCFD_IN);
+    const int isr = sg.add(e->dsrc, sizeof(double)*ne*nv, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_lpack(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, sys->nfvars, turb_factor,
+                         a0, (const int *)sg.dev(ic), (const double *)sg.dev(iu),
+                         (const double *)sg.dev(ir), (const double *)sg.dev(it),
+                         (const double *)sg.dev(isr), sys->lu_u, sys->lu_du, sys->lu_diag,
+                         ucfd_device_flag());
+    if (!rc) rc = ucfd_sync_and_check(fn);
+    return status(sg.leave(rc));
 }
-UCFD_API ucfd_status_t UCFDLUSGS_Pack(FlowSys, UCFDInt, UCFDReal, UCFDReal)
+
+UCFD_API ucfd_status_t UCFDLUSGS_Update(FlowSys sys, UCFDInt eidx)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_Pack");
+    const char *fn = "UCFDLUSGS_Update";
+    if (!is_lusgs(sys)) return fail(fn, "LU-SGS is not set (UCFDFlowSysSetLUSGS)", UCFD_ERR_STATE);
+    FsElem *e = elem(sys, eidx, fn);
+    if (!e) return UCFD_FAILED;
+    const size_t ne = (size_t)e->neles, nv = (size_t)sys->nvars;
+    UcfdStage sg(fn);
+    const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
+    const int iu = sg.add(e->uptsb, sizeof(double)*ne*nv, UCFD_INOUT);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_lupdate(ucfd_stream(), e->neles, sys->nlocal, sys->nvars,
+                           (const int *)sg.dev(ic), sys->lu_du, (double *)sg.dev(iu),
+                           ucfd_device_flag());
+    if (!rc) rc = ucfd_sync_and_check(fn);
+    return status(sg.leave(rc));
 }
-UCFD_API ucfd_status_t UCFDLUSGS_Update(FlowSys, UCFDInt) { UCFD_NOT_PROVIDED("UCFDLUSGS_Update"); }
-UCFD_API ucfd_status_t UCFDLUSGS_NSPrepare(FlowSys, UCFDReal)
+
+namespace {
+
+// NS rows (turb = false) or turbulence rows: the arguments both Prepare and
+// the sweeps check
+int lusgs_rows(FlowSys s, bool turb, const char *fn, const UCFDReal **lam)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_NSPrepare");
+    if (!is_lusgs(s)) return ucfd_report(UCFD_ERR_STATE, fn, "LU-SGS is not set (UCFDFlowSysSetLUSGS)");
+    if (turb) {
+        if (s->nturbvars < 1 || s->nturbvars > 4)
+            return ucfd_report(UCFD_ERR_ARG, fn, "the turbulence rows must number 1 .. 4");
+        if (!s->tfspr) return ucfd_report(UCFD_ERR_STATE, fn, "Turbulent spectral radius is not set");
+        *lam = s->tfspr;
+    } else {
+        if (!s->fspr) return ucfd_report(UCFD_ERR_STATE, fn, "spectral radius is not set");
+        *lam = s->fspr;
+    }
+    if ((s->ndims != 2 && s->ndims != 3) || s->nfvars != s->ndims + 2)
+        return ucfd_report(UCFD_ERR_ARG, fn, "the flux needs ndims 2 or 3 and nfvars = ndims + 2");
+    return build_structure(s, fn);
 }
-UCFD_API ucfd_status_t UCFDLUSGS_RANSPrepare(FlowSys, UCFDReal)
+
+ucfd_status_t lusgs_prepare(FlowSys s, bool turb, double kappa, const char *fn)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_RANSPrepare");
+    const UCFDReal *lam = nullptr;
+    if (lusgs_rows(s, turb, fn, &lam)) return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const int isl = sg.add(s->slots, sizeof(int)*(size_t)s->npos, UCFD_IN);
+    const int ifa = sg.add(s->face_area, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
+    const int irv = sg.add(s->rcp_vol, sizeof(double)*(size_t)s->nlocal, UCFD_IN);
+    const int ilm = sg.add(lam, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = fs_launch_lprep(ucfd_stream(), s->nlocal, turb ? s->nfvars : 0, turb ? s->nvars : s->nfvars,
+                         kappa, s->rowptr_d, (const int *)sg.dev(isl), (const double *)sg.dev(ifa),
+                         (const double *)sg.dev(irv), (const double *)sg.dev(ilm), s->lu_diag);
+    return status(sg.leave(rc));
 }
-UCFD_API ucfd_status_t UCFDLUSGS_NSLowerSweep(FlowSys, UCFDReal)
+
+ucfd_status_t lusgs_sweep(FlowSys s, bool turb, bool upper, double kappa, const char *fn)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_NSLowerSweep");
+    const UCFDReal *lam = nullptr;
+    if (lusgs_rows(s, turb, fn, &lam)) return UCFD_FAILED;
+    UcfdStage sg(fn);
+    const int isd = sg.add(s->sides, (size_t)s->npos, UCFD_IN);
+    const int isl = sg.add(s->slots, sizeof(int)*(size_t)s->npos, UCFD_IN);
+    const int ifa = sg.add(s->face_area, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
+    const int ifn = sg.add(s->face_normal, sizeof(double)*(size_t)s->nfaces*s->ndims, UCFD_IN);
+    const int irv = sg.add(s->rcp_vol, sizeof(double)*(size_t)s->nlocal, UCFD_IN);
+    const int ilm = sg.add(lam, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    const UcfdSchedule &S = upper ? s->upper : s->lower;
+    for (const UcfdLevel &l : S.lv)
+        if ((rc = fs_launch_lsweep(ucfd_stream(), upper, s->ndims, turb ? s->nturbvars : 0, l.base,
+                                   l.cnt, l.ordered ? S.order : nullptr, s->nvars, s->nfvars,
+                                   kappa, (const double *)sg.dev(ilm), s->nlocal, s->nfaces,
+                                   s->rowptr_d, s->colidx_d, (const signed char *)sg.dev(isd),
+                                   (const int *)sg.dev(isl), (const double *)sg.dev(ifa),
+                                   (const double *)sg.dev(ifn), (const double *)sg.dev(irv),
+                                   s->lu_diag, s->lu_u, s->lu_du)))
+            break;
+    return status(sg.leave(rc));
 }
-UCFD_API ucfd_status_t UCFDLUSGS_NSUpperSweep(FlowSys, UCFDReal)
+
+}  // namespace
+
+UCFD_API ucfd_status_t UCFDLUSGS_NSPrepare(FlowSys sys, UCFDReal kappa)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_NSUpperSweep");
+    return lusgs_prepare(sys, false, kappa, "UCFDLUSGS_NSPrepare");
 }
-UCFD_API ucfd_status_t UCFDLUSGS_RANSLowerSweep(FlowSys, UCFDReal)
+
+UCFD_API ucfd_status_t UCFDLUSGS_RANSPrepare(FlowSys sys, UCFDReal kappa)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_RANSLowerSweep");
+    return lusgs_prepare(sys, true, kappa, "UCFDLUSGS_RANSPrepare");
 }
-UCFD_API ucfd_status_t UCFDLUSGS_RANSUpperSweep(FlowSys, UCFDReal)
+
+UCFD_API ucfd_status_t UCFDLUSGS_NSLowerSweep(FlowSys sys, UCFDReal kappa)
 {
-    UCFD_NOT_PROVIDED("UCFDLUSGS_RANSUpperSweep");
+    return lusgs_sweep(sys, false, false, kappa, "UCFDLUSGS_NSLowerSweep");
+}
+
+UCFD_API ucfd_status_t UCFDLUSGS_NSUpperSweep(FlowSys sys, UCFDReal kappa)
+{
+    return lusgs_sweep(sys, false, true, kappa, "UCFDLUSGS_NSUpperSweep");
+}
+
+UCFD_API ucfd_status_t UCFDLUSGS_RANSLowerSweep(FlowSys sys, UCFDReal kappa)
+{
+    return lusgs_sweep(sys, true, false, kappa, "UCFDLUSGS_RANSLowerSweep");
+}
+
+UCFD_API ucfd_status_t UCFDLUSGS_RANSUpperSweep(FlowSys sys, UCFDReal kappa)
+{
+    return lusgs_sweep(sys, true, true, kappa, "UCFDLUSGS_RANSUpperSweep");
+}
+
+UCFD_API ucfd_status_t UCFDLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal *out)
+{
+    const char *fn = "UCFDLUSGS_Export";
+    if (!is_lusgs(sys)) return fail(fn, "LU-SGS is not set", UCFD_ERR_STATE);
+    const double *src = which == 0 ? sys->lu_u : which == 1 ? sys->lu_du : which == 2 ? sys->lu_diag
+                                                                                    : nullptr;
+    if (!src) return fail(fn, "which must be 0 (u), 1 (du) or 2 (diag)");
+    const size_t n = (size_t)sys->nlocal*sys->nvars;
+    UcfdStage sg(fn);
+    const int io = sg.add(out, sizeof(double)*n, UCFD_OUT);
+    int rc = sg.enter();
+    if (rc) return UCFD_FAILED;
+    rc = hip(hipMemcpyAsync(sg.dev(io), src, sizeof(double)*n, hipMemcpyDeviceToDevice,
+                            ucfd_stream()), fn, "hipMemcpyAsync(export)");
+    return status(sg.leave(rc));
 }

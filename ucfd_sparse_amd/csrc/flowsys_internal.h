@@ -44,3 +44,25 @@ UCFD_HIDDEN int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int c
                                 const double *diag, double *du, float *du32);
 // a = b = 0 (and c = 0 when not NULL), n entries
 UCFD_HIDDEN int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c);
+
+// Scalar LU-SGS (src/lusgs/lusgs.c); rank arrays u / du / diag are
+// [nvars, nlocal].  lusgs_pack / rank_lusgs_update / pre_lusgs:
+UCFD_HIDDEN int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
+                                double turb_factor, double a0, const int *cell_ids,
+                                const double *upts, const double *rhs, const double *dt,
+                                const double *dsrc, double *u, double *du, double *diag,
+                                int *flag);
+UCFD_HIDDEN int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars,
+                                  const int *cell_ids, const double *du, double *upts, int *flag);
+UCFD_HIDDEN int fs_launch_lprep(hipStream_t s, int nlocal, int nv0, int nv1, double kappa,
+                                const int *rowptr, const int *slots, const double *face_area,
+                                const double *rcp_vol, const double *lambdaf, double *diag);
+// One level of the lower (upper = false) or upper sweep: NS rows (nt = 0,
+// nfvars = ndims + 2) or the nt turbulence rows (1 <= nt <= 4).
+UCFD_HIDDEN int fs_launch_lsweep(hipStream_t s, bool upper, int ndims, int nt, int base, int cnt,
+                                 const int *order, int nvars, int nfvars, double kappa,
+                                 const double *lambdaf, int nlocal, int64_t nfaces,
+                                 const int *rowptr, const int *colidx, const signed char *sides,
+                                 const int *slots, const double *face_area,
+                                 const double *face_normal, const double *rcp_vol,
+                                 const double *diag, const double *u, double *du);

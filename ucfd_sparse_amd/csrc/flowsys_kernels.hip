@@ -12,6 +12,7 @@
 
 #include "block_lu.h"
 #include "flowsys_internal.h"
+#include "flux.h"
 #include "wave_rows.h"
 
 namespace {
@@ -465,6 +466,146 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int var0, int
                            slu + (act ? g*D2 : 0), rhs, du, du32);
 }
 
+// ------------------------------------------------------ scalar LU-SGS
+// Rank arrays u / du / diag are [nvars, nlocal] (src/lusgs/flowsys.h:20-24).
+
+// lusgs_pack (lusgs.c:8-37)
+__global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, int nfvars,
+                                              double turb_factor, double a0,
+                                              const int *__restrict__ cell_ids,
+                                              const double *__restrict__ upts,
+                                              const double *__restrict__ rhs,
+                                              const double *__restrict__ dt,
+                                              const double *__restrict__ dsrc, double *u,
+                                              double *du, double *diag, int *flag)
+{
+    const int e = fxcd()*FB + threadIdx.x;
+    if (e >= neles) return;
+    const int r = cell_ids[e];
+    if ((unsigned)r >= (unsigned)nlocal) {
+        atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    const double dte = dt[e];
+    for (int k = 0; k < nvars; ++k) {
+        const int64_t ri = r + (int64_t)k*nlocal, ei = e + (int64_t)k*neles;
+        const double factor = k < nfvars ? 1.0 : turb_factor;
+        u[ri] = upts[ei];
+        du[ri] = rhs[ei];
+        diag[ri] = 1.0/(dte*factor) + a0 + dsrc[ei];
+    }
+}
+
+// rank_lusgs_update (lusgs.c:54-70)
+__global__ __launch_bounds__(FB) void k_lupdate(int neles, int nlocal, int nvars,
+                                                const int *__restrict__ cell_ids,
+                                                const double *__restrict__ du, double *upts,
+                                                int *flag)
+{
+    const int e = fxcd()*FB + threadIdx.x;
+    if (e >= neles) return;
+    const int r = cell_ids[e];
+    if ((unsigned)r >= (unsigned)nlocal) {
+        atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    for (int k = 0; k < nvars; ++k) upts[e + (int64_t)k*neles] += du[r + (int64_t)k*nlocal];
+}
+
+// pre_lusgs (lusgs.c:87-113): the face sum first, then added to rows nv0..nv1-1
+__global__ __launch_bounds__(FB) void k_lprep(int nlocal, int nv0, int nv1, double kappa,
+                                              const int *__restrict__ rowptr,
+                                              const int *__restrict__ slots,
+                                              const double *__restrict__ face_area,
+                                              const double *__restrict__ rcp_vol,
+                                              const double *__restrict__ lambdaf, double *diag)
+{
+    const int r = fxcd()*FB + threadIdx.x;
+    if (r >= nlocal) return;
+    const double rv = rcp_vol[r];
+    double sd = 0.0;
+    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+        const int64_t sl = slots[p];
+        const double lamf = lambdaf[sl]*kappa;
+        sd += 0.5*lamf*face_area[sl]*rv;
+    }
+    for (int k = nv0; k < nv1; ++k) diag[r + (int64_t)k*nlocal] += sd;
+}
+
+// One level of lower_sweep / upper_sweep (lusgs.c:157-257): rows
+// base..base+cnt-1 (or order[...]), in place on du.  NS rows (NT = 0) with
+// ns_flux_container, turbulence rows (NT turbulence variables) with
+// rans_flux_container (flux.c:77-90); the flux difference of diff_flux
+// (lusgs.c:143-155) in the reference's order.
+template <int ND, int NT, bool UPPER>
+__global__ __launch_bounds__(FB) void k_lsweep(int base, int cnt, const int *__restrict__ order,
+                                               int nvars, int nfvars, double kappa,
+                                               const double *__restrict__ lambdaf, int nlocal,
+                                               int64_t nfaces, const int *__restrict__ rowptr,
+                                               const int *__restrict__ colidx,
+                                               const signed char *__restrict__ sides,
+                                               const int *__restrict__ slots,
+                                               const double *__restrict__ face_area,
+                                               const double *__restrict__ face_normal,
+                                               const double *__restrict__ rcp_vol,
+                                               const double *__restrict__ diag,
+                                               const double *__restrict__ u, double *du)
+{
+    constexpr int NF = ND + 2;                 // flow variables
+    constexpr int DNV = NT > 0 ? NT : NF;      // rows this sweep updates
+    const int t = fxcd()*FB + threadIdx.x;
+    if (t >= cnt) return;
+    const int r = order ? order[base + t] : base + t;
+    const int nv0 = NT > 0 ? nfvars : 0;
+    const double rv = rcp_vol[r];
+    double df[DNV];
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) df[k] = 0.0;
+    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+        const int64_t nb = colidx[p];
+        if (UPPER ? !(nb > r) : !(nb < r)) continue;
+        const int64_t sl = slots[p];
+        const double side = (double)sides[p];
+        double nf[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) nf[d] = side*face_normal[sl + (int64_t)d*nfaces];
+        const double fv = face_area[sl]*rv;
+        const double lam = kappa*lambdaf[sl];
+        double uf[NF], dnb[DNV], dfj[DNV];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) uf[k] = u[nb + (int64_t)k*nlocal];
+#pragma unroll
+        for (int k = 0; k < DNV; ++k) dnb[k] = du[nb + (int64_t)(k + nv0)*nlocal];
+        if (NT == 0) {
+            ucfd::diff_flux<double, ND>(uf, dnb, nf, dfj);
+        } else {
+            // flow rows of du are zero: du + u = u for them
+            double ut[DNV], c0 = 0.0, c1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < DNV; ++k) ut[k] = u[nb + (int64_t)(nfvars + k)*nlocal];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) c0 += uf[d + 1]*nf[d];
+            c0 /= uf[0];
+            double up[NF];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) up[k] = 0.0 + uf[k];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) c1 += up[d + 1]*nf[d];
+            c1 /= up[0];
+#pragma unroll
+            for (int k = 0; k < DNV; ++k) dfj[k] = (dnb[k] + ut[k])*c1 - ut[k]*c0;
+        }
+#pragma unroll
+        for (int k = 0; k < DNV; ++k) df[k] += (dfj[k] - lam*dnb[k])*fv;
+    }
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) {
+        const int64_t l = r + (int64_t)(k + nv0)*nlocal;
+        if (UPPER) du[l] -= (0.5*df[k]/diag[l]);
+        else du[l] = (du[l] - 0.5*df[k])/diag[l];
+    }
+}
+
 __global__ __launch_bounds__(FB) void k_zero(int64_t n, double *a, double *b, float *c)
 {
     const int64_t st = (int64_t)gridDim.x*FB;
@@ -603,4 +744,96 @@ int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c)
     const int64_t nb = (n + FB - 1)/FB;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)(nb > 2048 ? 2048 : nb)), dim3(FB), 0, s, n, a, b, c);
     return ok();
+}
+
+int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars, double turb_factor,
+                    double a0, const int *cell_ids, const double *upts, const double *rhs,
+                    const double *dt, const double *dsrc, double *u, double *du, double *diag,
+                    int *flag)
+{
+    if (neles <= 0) return UCFD_OK;
+    hipLaunchKernelGGL(k_lpack, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
+                       turb_factor, a0, cell_ids, upts, rhs, dt, dsrc, u, du, diag, flag);
+    return ok();
+}
+
+int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars, const int *cell_ids,
+                      const double *du, double *upts, int *flag)
+{
+    if (neles <= 0) return UCFD_OK;
+    hipLaunchKernelGGL(k_lupdate, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, cell_ids, du,
+                       upts, flag);
+    return ok();
+}
+
+int fs_launch_lprep(hipStream_t s, int nlocal, int nv0, int nv1, double kappa, const int *rowptr,
+                    const int *slots, const double *face_area, const double *rcp_vol,
+                    const double *lambdaf, double *diag)
+{
+    if (nlocal <= 0) return UCFD_OK;
+    hipLaunchKernelGGL(k_lprep, grid(nlocal), dim3(FB), 0, s, nlocal, nv0, nv1, kappa, rowptr,
+                       slots, face_area, rcp_vol, lambdaf, diag);
+    return ok();
+}
+
+namespace {
+
+template <int ND, int NT>
+int lsweep_launch(hipStream_t s, bool upper, int base, int cnt, const int *order, int nvars,
+                  int nfvars, double kappa, const double *lambdaf, int nlocal, int64_t nfaces,
+                  const int *rowptr, const int *colidx, const signed char *sides,
+                  const int *slots, const double *face_area, const double *face_normal,
+                  const double *rcp_vol, const double *diag, const double *u, double *du)
+{
+    if (upper)
+        hipLaunchKernelGGL((k_lsweep<ND, NT, true>), grid(cnt), dim3(FB), 0, s, base, cnt, order,
+                           nvars, nfvars, kappa, lambdaf, nlocal, nfaces, rowptr, colidx, sides,
+                           slots, face_area, face_normal, rcp_vol, diag, u, du);
+    else
+        hipLaunchKernelGGL((k_lsweep<ND, NT, false>), grid(cnt), dim3(FB), 0, s, base, cnt, order,
+                           nvars, nfvars, kappa, lambdaf, nlocal, nfaces, rowptr, colidx, sides,
+                           slots, face_area, face_normal, rcp_vol, diag, u, du);
+    return ok();
+}
+
+template <int ND>
+int lsweep_nt(hipStream_t s, bool upper, int nt, int base, int cnt, const int *order, int nvars,
+              int nfvars, double kappa, const double *lambdaf, int nlocal, int64_t nfaces,
+              const int *rowptr, const int *colidx, const signed char *sides, const int *slots,
+              const double *face_area, const double *face_normal, const double *rcp_vol,
+              const double *diag, const double *u, double *du)
+{
+#define LS(NT) return lsweep_launch<ND, NT>(s, upper, base, cnt, order, nvars, nfvars, kappa, \
+                                            lambdaf, nlocal, nfaces, rowptr, colidx, sides, slots, \
+                                            face_area, face_normal, rcp_vol, diag, u, du)
+    switch (nt) {
+    case 0: LS(0);
+    case 1: LS(1);
+    case 2: LS(2);
+    case 3: LS(3);
+    case 4: LS(4);
+    default: return UCFD_ERR_ARG;
+    }
+#undef LS
+}
+
+}  // namespace
+
+int fs_launch_lsweep(hipStream_t s, bool upper, int ndims, int nt, int base, int cnt,
+                     const int *order, int nvars, int nfvars, double kappa, const double *lambdaf,
+                     int nlocal, int64_t nfaces, const int *rowptr, const int *colidx,
+                     const signed char *sides, const int *slots, const double *face_area,
+                     const double *face_normal, const double *rcp_vol, const double *diag,
+                     const double *u, double *du)
+{
+    if (cnt <= 0) return UCFD_OK;
+    if (ndims == 3)
+        return lsweep_nt<3>(s, upper, nt, base, cnt, order, nvars, nfvars, kappa, lambdaf, nlocal,
+                            nfaces, rowptr, colidx, sides, slots, face_area, face_normal, rcp_vol,
+                            diag, u, du);
+    if (ndims == 2)
+        return lsweep_nt<2>(s, upper, nt, base, cnt, order, nvars, nfvars, kappa, lambdaf, nlocal,
+                            nfaces, rowptr, colidx, sides, slots, face_area, face_normal, rcp_vol,
+                            diag, u, du);
+    return UCFD_ERR_ARG;
 }

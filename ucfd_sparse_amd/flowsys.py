@@ -92,6 +92,69 @@ class FlowSys:
             pass
 
 
+class FlowSysLUSGS(FlowSys):
+    """The scalar LU-SGS method of the FlowSys (UCFDFlowSysSetLUSGS):
+    fspr / tfspr are per face slot (kept by the library, read at every
+    Prepare and sweep)."""
+
+    def __init__(self, fs, nvars, nfvars, fspr, tfspr=None, ndims=3, nelem=1):
+        self._lib = L.load()
+        self.keep = {}
+        k = self.keep
+        k["rowptr"], k["colidx"] = _conv(fs.rowptr, np.int32), _conv(fs.colidx, np.int32)
+        k["slots"], k["sides"] = _conv(fs.slots, np.int32), _conv(fs.sides, np.int8)
+        k["face_area"] = _conv(fs.face_area, np.float64)
+        k["face_normal"] = _conv(fs.face_normal, np.float64)
+        k["rcp_vol"] = _conv(fs.rcp_vol, np.float64)
+        self.nlocal, self.nvars, self.nfvars = int(fs.nlocal), int(nvars), int(nfvars)
+        h = ctypes.c_void_p()
+        L.check(self._lib.UCFDFlowSysCreate(
+            ctypes.byref(h), self.nlocal, self.nvars, self.nfvars, int(ndims), int(fs.nfaces),
+            int(nelem), L.ptr(k["rowptr"]), L.ptr(k["colidx"]), L.ptr(k["slots"]),
+            L.ptr(k["sides"]), L.ptr(k["face_area"]), L.ptr(k["face_normal"]),
+            L.ptr(k["rcp_vol"])), "UCFDFlowSysCreate")
+        self.handle = h
+        k["fspr"] = _conv(fspr, np.float64)
+        k["tfspr"] = None if tfspr is None else _conv(tfspr, np.float64)
+        L.check(self._lib.UCFDFlowSysSetLUSGS(ctypes.byref(h), L.ptr(k["fspr"]),
+                                              L.ptr(k["tfspr"])), "UCFDFlowSysSetLUSGS")
+
+    def export(self, which, out=None):
+        out = np.zeros(self.nlocal * self.nvars) if out is None else out
+        L.check(self._lib.UCFDLUSGS_Export(self.handle, int(which), L.ptr(out)),
+                "UCFDLUSGS_Export")
+        return out
+
+
+def run_lusgs(sysobj, a0=0.3, turb_factor=1.5, kappa=1.2, iterations=2):
+    """The stage sequence of tests/golden/refflowsys.run_lusgs on a
+    FlowSysLUSGS handle with element 0 set; returns host copies."""
+    s = sysobj
+    nt = s.nvars - s.nfvars
+    out = {}
+    s.call("UCFDLUSGS_Pack", 0, float(turb_factor), float(a0))
+    out["u"], out["du_pack"], out["diag_pack"] = s.export(0), s.export(1), s.export(2)
+    s.call("UCFDLUSGS_NSPrepare", float(kappa))
+    out["diag_ns"] = s.export(2)
+    if nt:
+        s.call("UCFDLUSGS_RANSPrepare", float(kappa))
+        out["diag"] = s.export(2)
+    for it in range(iterations):
+        s.call("UCFDLUSGS_NSLowerSweep", float(kappa))
+        out[f"du_nslower{it}"] = s.export(1)
+        s.call("UCFDLUSGS_NSUpperSweep", float(kappa))
+        out[f"du_nsupper{it}"] = s.export(1)
+        if nt:
+            s.call("UCFDLUSGS_RANSLowerSweep", float(kappa))
+            out[f"du_ranslower{it}"] = s.export(1)
+            s.call("UCFDLUSGS_RANSUpperSweep", float(kappa))
+            out[f"du_ransupper{it}"] = s.export(1)
+    s.call("UCFDLUSGS_Update", 0)
+    u = s.element_array(0, "uptsb")
+    out["upts"] = (u.cpu().numpy() if hasattr(u, "data_ptr") else u).copy()
+    return out
+
+
 def run(sysobj, a0=0.3, turb_factor=1.5, kind="sa", iterations=2):
     """The stage sequence of tests/golden/refflowsys.run on a FlowSys handle
     with element 0 set; returns host copies of every stage."""

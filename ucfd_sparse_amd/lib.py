@@ -109,6 +109,7 @@ _SIGS = {
     "UCFDLUSGS_NSUpperSweep": [c_void_p, c_double],
     "UCFDLUSGS_RANSLowerSweep": [c_void_p, c_double],
     "UCFDLUSGS_RANSUpperSweep": [c_void_p, c_double],
+    "UCFDLUSGS_Export": [c_void_p, c_int, c_void_p],
     "UCFDFlowSysSetBLUSGS": [c_void_p, c_void_p, c_void_p],
     "UCFDBLUSGS_Pack": [c_void_p, c_int, c_double],
     "UCFDBLUSGS_KWSST_Pack": [c_void_p, c_int, c_double, c_double],
