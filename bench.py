@@ -323,6 +323,7 @@ def blusgs_workload(args, log):
 
     def up(a):
         return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fs_nfaces = fs.nfaces
     fsd = FlowSysMesh(fs.nlocal, fs.nfaces, up(fs.rowptr), up(fs.colidx), up(fs.slots),
                       up(fs.sides), up(fs.face_area), up(fs.face_normal), up(fs.rcp_vol),
                       up(fs.jmat))
@@ -420,7 +421,58 @@ def blusgs_workload(args, log):
     errs = {f"{part}_after_{k}": rel(du_after[("fp64", k)], du_after[("fp32", k)], sl)
             for k in (1, 5) for part, sl in (("ns", slice(0, 5)), ("rans", slice(5, 6)))}
     log(f"flowsys fp32 vs fp64 du: {errs}")
-    del fsd, tjd
+    del tjd
+    torch.cuda.empty_cache()
+
+    # the FlowSys scalar LU-SGS method on the same structure: per-slot
+    # spectral radii, NS rows + 1 SA row
+    rng = np.random.default_rng(args.seed + 11)
+    fspr_s = up(rng.uniform(0.5, 2.0, size=fsd.nfaces))
+    tfspr_s = up(rng.uniform(0.2, 1.0, size=fsd.nfaces))
+    SL = FS.FlowSysLUSGS(fsd, 6, 5, fspr_s, tfspr_s)
+    SL.set_element(0, {k: up(v) for k, v in el.items()})
+    torch.cuda.synchronize()
+    L.call("ucfd_set_stream", stream.cuda_stream)
+    SL.call("UCFDLUSGS_Pack", 0, 1.5, 0.3)
+    SL.call("UCFDLUSGS_NSPrepare", 1.0)
+    SL.call("UCFDLUSGS_RANSPrepare", 1.0)
+    L.call("ucfd_synchronize")
+
+    def lit(evs=None):
+        if evs:
+            evs[0].record(stream)
+        SL.call("UCFDLUSGS_NSLowerSweep", 1.0)
+        SL.call("UCFDLUSGS_NSUpperSweep", 1.0)
+        if evs:
+            evs[1].record(stream)
+        SL.call("UCFDLUSGS_RANSLowerSweep", 1.0)
+        SL.call("UCFDLUSGS_RANSUpperSweep", 1.0)
+        if evs:
+            evs[2].record(stream)
+    for _ in range(3):
+        lit()
+    L.call("ucfd_synchronize")
+    evl = [[ev() for _ in range(3)] for _ in range(args.steps)]
+    for r in range(args.steps):
+        lit(evl[r])
+    L.call("ucfd_synchronize")
+    torch.cuda.synchronize()
+    L.call("ucfd_set_stream", None)
+    ns_l = sum(x[0].elapsed_time(x[1]) for x in evl) / args.steps
+    rans_l = sum(x[1].elapsed_time(x[2]) for x in evl) / args.steps
+    SL.close()
+    # algorithmic bytes of one NS scalar sweep, every array element counted
+    # once: per cell rowptr, rcp_vol and the 5 rows of diag, u, du (read) and
+    # du (written); per position colidx, slot, side; per face slot the normal
+    # (3), area and spectral radius
+    b_scalar = N * (4 + 8 + 8 * 5 * 4) + npos * (4 + 4 + 1) + fs_nfaces * 8 * 5
+    scalar = {"ns_lower_upper_ms": ns_l, "rans_lower_upper_ms": rans_l,
+              "ns_cell_iters_per_s": N / (ns_l * 1e-3),
+              "ns_sweep_algorithmic_bytes": b_scalar,
+              "ns_sweep_gbs": b_scalar / (0.5 * ns_l) / 1e6}
+    log(f"flowsys scalar LU-SGS: NS lower+upper {ns_l:.3f} ms "
+        f"({scalar['ns_sweep_gbs']:.0f} GB/s alg), RANS lower+upper {rans_l:.3f} ms")
+    del fsd
     torch.cuda.empty_cache()
 
     # Krylov-path coloured block LU-SGS apply on the BSR block Jacobian
@@ -485,6 +537,7 @@ def blusgs_workload(args, log):
         "fp32_speedup_ns_sweeps": ns_ms64 / ns_ms32,
         "fp32_speedup_rans_sweeps": f64["rans_lower_upper_ms"] / f32["rans_lower_upper_ms"],
         "fp32_rel_err_vs_fp64": errs,
+        "flowsys_scalar_lusgs": scalar,
         "krylov_precon_apply": applies,
         "krylov_precon_apply_fp32_rel_err": apply_err,
         "roofline": {
