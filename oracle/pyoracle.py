@@ -379,6 +379,54 @@ def flowsys_run(fs, tjmat, el, nvars, nfvars, a0=0.3, turb_factor=1.5, kind="sa"
     return out
 
 
+# --------------------------------------------- incomplete-LU preconditioners
+def _ilulib():
+    L = lib()
+    if not hasattr(L, "_ilu_bound"):
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        L.orc_ilu_prepare.argtypes = [ci, vp, vp, vp, vp]
+        L.orc_ilu_apply.argtypes = [ci, vp, vp, vp, vp, vp]
+        L.orc_bilu_prepare.argtypes = [ci, ci, vp, vp, vp, vp]
+        L.orc_bilu_apply.argtypes = [ci, ci, vp, vp, vp, vp, vp]
+        L.orc_pbilu_apply.argtypes = [ci, ci, ci, vp, vp, vp, vp, vp, vp]
+        for f in ("orc_ilu_prepare", "orc_ilu_apply", "orc_bilu_prepare", "orc_bilu_apply",
+                  "orc_pbilu_apply"):
+            getattr(L, f).restype = None
+        L._ilu_bound = True
+    return L
+
+
+def ilu_prepare(A, kind):
+    """Factored copy of A.values: kind "ilu" (blk 1) or "bilu" / "pbilu"."""
+    L = _ilulib()
+    v = np.array(A.values, np.float64)
+    rp, ci, ds = _c(A.rowptr, np.int32), _c(A.colidx, np.int32), _c(A.diagslots, np.int32)
+    if kind == "ilu":
+        L.orc_ilu_prepare(A.bn, rp.ctypes.data, ci.ctypes.data, ds.ctypes.data, v.ctypes.data)
+    else:
+        L.orc_bilu_prepare(A.bn, A.blk, rp.ctypes.data, ci.ctypes.data, ds.ctypes.data,
+                           v.ctypes.data)
+    return v
+
+
+def ilu_apply(A, factored, b, kind, icolors=None):
+    L = _ilulib()
+    x = np.array(b, np.float64)
+    f = _c(factored, np.float64)
+    rp, ci, ds = _c(A.rowptr, np.int32), _c(A.colidx, np.int32), _c(A.diagslots, np.int32)
+    if kind == "ilu":
+        L.orc_ilu_apply(A.bn, rp.ctypes.data, ci.ctypes.data, ds.ctypes.data, f.ctypes.data,
+                        x.ctypes.data)
+    elif kind == "bilu":
+        L.orc_bilu_apply(A.bn, A.blk, rp.ctypes.data, ci.ctypes.data, ds.ctypes.data,
+                         f.ctypes.data, x.ctypes.data)
+    else:
+        ic = _c(icolors, np.int32)
+        L.orc_pbilu_apply(A.bn, A.blk, len(ic) - 1, ic.ctypes.data, rp.ctypes.data,
+                          ci.ctypes.data, ds.ctypes.data, f.ctypes.data, x.ctypes.data)
+    return x
+
+
 # ------------------------------------------------- FlowSys scalar LU-SGS
 def _fsllib():
     L = lib()

@@ -68,6 +68,9 @@ def load():
         L.UCFDPreconCreateNone.argtypes = [vp]
         L.UCFDPreconSetBLUSGS.argtypes = [vp, ci, ci]
         L.UCFDPreconSetPBLUSGS.argtypes = [vp, ci, ci, ci, vp]
+        L.UCFDPreconSetILU.argtypes = [vp, ci]
+        L.UCFDPreconSetBILU.argtypes = [vp, ci, ci]
+        L.UCFDPreconSetPBILU.argtypes = [vp, ci, ci, ci, vp]
         L.UCFDPreconPrepare.argtypes = [vp]
         L.UCFDPreconDestroy.argtypes = [vp]
         L.UCFDSolverCreateGMRES.argtypes = [vp, ci, ci, ci, cd]
@@ -150,3 +153,28 @@ def solve(A, b, kind, m=30, maxiter=50, tol=1e-10, precon_kind="blusgs"):
         L.UCFDPreconDestroy(ctypes.byref(pc))
     L.UCFDMatDestroy(ctypes.byref(mat))
     return x, it.value, st.value, res.value, hist
+
+
+def ilu_prepare_apply(A, kind, b, icolors=None):
+    """The reference's incomplete-LU preconditioners: kind "ilu" (scalar,
+    A.blk == 1, UCFDPreconSetILU), "bilu" (UCFDPreconSetBILU) or "pbilu"
+    (UCFDPreconSetPBILU with colour offsets icolors); UCFDPreconPrepare
+    factors a copy of the values in place, then ops->apply runs on b.
+    Returns (factored values, M^-1 b)."""
+    L = load()
+    vals = np.array(A.values, np.float64)
+    pc = ctypes.c_void_p()
+    L.UCFDPreconCreatefromArrays(ctypes.byref(pc), A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                                 A.diagslots.ctypes.data, vals.ctypes.data)
+    keep = None
+    if kind == "ilu":
+        L.UCFDPreconSetILU(ctypes.byref(pc), A.bn)
+    elif kind == "bilu":
+        L.UCFDPreconSetBILU(ctypes.byref(pc), A.bn, A.blk)
+    else:
+        keep = np.ascontiguousarray(icolors, np.int32)
+        L.UCFDPreconSetPBILU(ctypes.byref(pc), A.bn, A.blk, len(keep) - 1, keep.ctypes.data)
+    L.UCFDPreconPrepare(pc)
+    x = precon_apply(pc, b)
+    L.UCFDPreconDestroy(ctypes.byref(pc))
+    return vals, x
