@@ -11,6 +11,9 @@
  *   src/krylov/utils/blas_basic.c:4-61     dot, nrm2, axpy, scal, gemv (col-major)
  *   src/krylov/solver/gmres.c:4-129        restarted GMRES, CGS2, Givens
  *   src/krylov/solver/bicgstab.c:5-131     BiCGSTAB with the fused 5-way reduction
+ * The solvers also take the ILU(0) family as preconditioner (ilu_oracle.c:
+ * factored values of a separate array, as the reference's preconditioner
+ * factors its own copy).
  * Deviations, each recorded: the coloured lower apply uses the diagonal block
  * of its own row (pblusgs.c:41 indexes diagvalues[idx+blkdim], a defect);
  * GMRES has an additional right-preconditioned / relative-tolerance mode
@@ -18,7 +21,8 @@
  *
  * Pins: bit-exact against oracle/_ref (the reference compiled from its own
  * sources, OpenMP at one thread) for SpMV, prepare, sequential apply, GMRES
- * and BiCGSTAB residual histories; scipy bsr_matrix / numpy.linalg.solve
+ * and BiCGSTAB residual histories (with block LU-SGS, none, and block ILU(0)
+ * preconditioners); scipy bsr_matrix / numpy.linalg.solve
  * closed forms; coloured apply = sequential apply on a colour-ordered matrix.
  */
 #include <math.h>
@@ -226,10 +230,32 @@ typedef struct {
     const double *diagvalues;   /* NULL: no preconditioner */
     int ncolors;
     const int32_t *icolors;
+    int pckind;                 /* 0 block LU-SGS (diagvalues), 1 block ILU(0), 2 ILU(0) */
+    const double *pcvalues;     /* ILU kinds: the factored values */
 } OrcSystem;
+
+void orc_ilu_apply(int n, const int32_t *rowptr, const int32_t *colidx, const int32_t *diagslots,
+                   const double *values, double *b);
+void orc_bilu_apply(int bn, int blk, const int32_t *rowptr, const int32_t *colidx,
+                    const int32_t *diagslots, const double *values, double *b);
+void orc_pbilu_apply(int bn, int blk, int ncolors, const int32_t *icolors, const int32_t *rowptr,
+                     const int32_t *colidx, const int32_t *diagslots, const double *values,
+                     double *b);
 
 static void o_apply(const OrcSystem *S, double *v)
 {
+    if (S->pckind == 1) {
+        if (S->ncolors > 0)
+            orc_pbilu_apply(S->bn, S->blk, S->ncolors, S->icolors, S->rowptr, S->colidx,
+                            S->diagslots, S->pcvalues, v);
+        else
+            orc_bilu_apply(S->bn, S->blk, S->rowptr, S->colidx, S->diagslots, S->pcvalues, v);
+        return;
+    }
+    if (S->pckind == 2) {
+        orc_ilu_apply(S->bn, S->rowptr, S->colidx, S->diagslots, S->pcvalues, v);
+        return;
+    }
     if (S->diagvalues)
         orc_blusgs_apply(S->bn, S->blk, S->rowptr, S->colidx, S->diagslots, S->values,
                          S->diagvalues, S->ncolors, S->icolors, v);
@@ -423,5 +449,30 @@ ORC_EXPORT int orc_bicgstab_solve(int bn, int blk, const int32_t *rowptr, const 
                                   const double *b, int *stat, double *res, double *hist)
 {
     OrcSystem S = {bn, blk, rowptr, colidx, diagslots, values, diagvalues, ncolors, icolors};
+    return orc_bicgstab(&S, maxiter, tol, relative, x, b, stat, res, hist);
+}
+
+/* the solvers with an ILU-family preconditioner: pckind 1 block ILU(0)
+ * (coloured when ncolors > 0), 2 scalar ILU(0); pcvalues = factored values */
+ORC_EXPORT int orc_gmres_solve_ilu(int bn, int blk, const int32_t *rowptr, const int32_t *colidx,
+                                   const int32_t *diagslots, const double *values, int pckind,
+                                   const double *pcvalues, int ncolors, const int32_t *icolors,
+                                   int m, int maxiter, double tol, int right, double *x,
+                                   const double *b, int *stat, double *res, double *hist)
+{
+    OrcSystem S = {bn, blk, rowptr, colidx, diagslots, values, NULL, ncolors, icolors, pckind,
+                   pcvalues};
+    return orc_gmres(&S, m, maxiter, tol, right, x, b, stat, res, hist);
+}
+
+ORC_EXPORT int orc_bicgstab_solve_ilu(int bn, int blk, const int32_t *rowptr,
+                                      const int32_t *colidx, const int32_t *diagslots,
+                                      const double *values, int pckind, const double *pcvalues,
+                                      int ncolors, const int32_t *icolors, int maxiter,
+                                      double tol, int relative, double *x, const double *b,
+                                      int *stat, double *res, double *hist)
+{
+    OrcSystem S = {bn, blk, rowptr, colidx, diagslots, values, NULL, ncolors, icolors, pckind,
+                   pcvalues};
     return orc_bicgstab(&S, maxiter, tol, relative, x, b, stat, res, hist);
 }

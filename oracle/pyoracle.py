@@ -184,8 +184,14 @@ def _klib():
         for f in ("orc_spmv_bsr", "orc_ludcmp", "orc_lusub", "orc_blusgs_prepare",
                   "orc_blusgs_apply"):
             getattr(L, f).restype = None
+        L.orc_gmres_solve_ilu.argtypes = [ci, ci] + [vp] * 4 + [ci, vp, ci, vp, ci, ci, cd, ci,
+                                                               vp, vp, vp, vp, vp]
+        L.orc_bicgstab_solve_ilu.argtypes = [ci, ci] + [vp] * 4 + [ci, vp, ci, vp, ci, cd, ci,
+                                                                  vp, vp, vp, vp, vp]
         L.orc_gmres_solve.restype = ci
         L.orc_bicgstab_solve.restype = ci
+        L.orc_gmres_solve_ilu.restype = ci
+        L.orc_bicgstab_solve_ilu.restype = ci
         L._krylov_bound = True
     return L
 
@@ -225,6 +231,38 @@ def blusgs_apply(A, dv, b, icolors=None):
                              A.diagslots.ctypes.data, A.values.ctypes.data, dv.ctypes.data, nc,
                              None if ic is None else ic.ctypes.data, x.ctypes.data)
     return x
+
+
+_ILU_KIND = {"bilu": 1, "pbilu": 1, "ilu": 2}
+
+
+def _ilu_solve(fn, A, b, kind, icolors, *mid):
+    """the oracle solvers with an ILU-family preconditioner factored from a
+    copy of A's values"""
+    f = ilu_prepare(A, kind)
+    x = np.zeros(A.bn * A.blk)
+    st, res = ctypes.c_int(), ctypes.c_double()
+    maxiter = mid[-3]           # (m, maxiter, tol, right) or (maxiter, tol, relative)
+    hist = np.zeros(maxiter + 1)
+    nc = 0 if (icolors is None or kind != "pbilu") else len(icolors) - 1
+    ic = None if nc == 0 else _c(icolors, np.int32)
+    it = getattr(_klib(), fn)(A.bn, A.blk, A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                              A.diagslots.ctypes.data, A.values.ctypes.data, _ILU_KIND[kind],
+                              f.ctypes.data, nc, None if ic is None else ic.ctypes.data, *mid,
+                              x.ctypes.data, _c(b, np.float64).ctypes.data, ctypes.byref(st),
+                              ctypes.byref(res), hist.ctypes.data)
+    return x, it, st.value, res.value, hist[:it + 1]
+
+
+def gmres_ilu(A, b, kind, m, maxiter, tol, right=False, icolors=None):
+    """GMRES preconditioned by ILU(0) ("ilu"), block ILU(0) ("bilu") or its
+    coloured form ("pbilu"); returns (x, restarts, stat, residual, history)"""
+    return _ilu_solve("orc_gmres_solve_ilu", A, b, kind, icolors, m, maxiter, tol, int(right))
+
+
+def bicgstab_ilu(A, b, kind, maxiter, tol, relative=False, icolors=None):
+    return _ilu_solve("orc_bicgstab_solve_ilu", A, b, kind, icolors, maxiter, tol,
+                      int(relative))
 
 
 def gmres(A, b, m, maxiter, tol, right=False, precon=True, icolors=None, x0=None):

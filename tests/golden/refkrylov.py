@@ -124,14 +124,40 @@ def precon_apply(pc, b):
     return x
 
 
-def solve(A, b, kind, m=30, maxiter=50, tol=1e-10, precon_kind="blusgs"):
+def ilu_precon(A, kind, icolors=None):
+    """Returns (handle, keepalive): the reference's ILU-family preconditioner
+    prepared (factored in place) on a copy of A's values."""
+    L = load()
+    vals = np.array(A.values, np.float64)
+    pc = ctypes.c_void_p()
+    L.UCFDPreconCreatefromArrays(ctypes.byref(pc), A.rowptr.ctypes.data, A.colidx.ctypes.data,
+                                 A.diagslots.ctypes.data, vals.ctypes.data)
+    keep = [vals]
+    if kind == "ilu":
+        L.UCFDPreconSetILU(ctypes.byref(pc), A.bn)
+    elif kind == "bilu":
+        L.UCFDPreconSetBILU(ctypes.byref(pc), A.bn, A.blk)
+    else:
+        ic = np.ascontiguousarray(icolors, np.int32)
+        keep.append(ic)
+        L.UCFDPreconSetPBILU(ctypes.byref(pc), A.bn, A.blk, len(ic) - 1, ic.ctypes.data)
+    L.UCFDPreconPrepare(pc)
+    return pc, keep
+
+
+def solve(A, b, kind, m=30, maxiter=50, tol=1e-10, precon_kind="blusgs", icolors=None):
+    """precon_kind: "none", "blusgs", or an ILU kind ("ilu", "bilu", "pbilu"
+    with icolors)."""
     L = load()
     mat = ctypes.c_void_p()
     L.UCFDMatCreateBSR(ctypes.byref(mat), A.bn, A.blk, A.rowptr.ctypes.data,
                        A.colidx.ctypes.data, A.values.ctypes.data)
+    keep = None
     if precon_kind == "none":
         pc = ctypes.c_void_p()
         L.UCFDPreconCreateNone(ctypes.byref(pc))
+    elif precon_kind in ("ilu", "bilu", "pbilu"):
+        pc, keep = ilu_precon(A, precon_kind, icolors)
     else:
         pc, _ = precon(A)
     s = ctypes.c_void_p()

@@ -91,6 +91,7 @@ def test_headers_declare_the_krylov_api():
               "UCFDPreconCreatefromArrays", "UCFDPreconCreateNone", "UCFDPreconPrepare",
               "UCFDPreconDestroy", "UCFDPreconSetILU", "UCFDPreconSetBILU",
               "UCFDPreconSetBLUSGS", "UCFDPreconSetPBILU", "UCFDPreconSetCUDABLUSGS",
+              "UCFDPreconSetCUDABILU",
               "UCFDSolverInit", "UCFDSolverDestroy", "UCFDSolve", "UCFDSolverGetResult",
               "UCFDSolverTraceResidualHistory", "UCFDSolverGetResidualHistory",
               "UCFDSolverCreateGMRES", "UCFDSolverCreateBICGSTAB", "UCFDSolverCreateCUDAGMRES"):

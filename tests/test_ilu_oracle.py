@@ -6,7 +6,8 @@
   src/krylov and driven through UCFDPreconSetILU / SetBILU / SetPBILU +
   UCFDPreconPrepare), on tet-mesh matrices and on random symmetric patterns
   with triangles (fill updates inside the pattern);
-* block applies bit for bit against the reference's apply;
+* block applies bit for bit against the reference's apply, and GMRES /
+  BiCGSTAB preconditioned by them (histories and solutions);
 * the scalar apply: the reference's backward sweep starts at the diagonal
   entry (src/krylov/precon/ilu.c:81); its output is reproduced exactly by
   this oracle's sweep plus that one extra term, and this oracle's apply is
@@ -141,6 +142,24 @@ def test_applies_solve_with_the_factors(cases):
         Lm, Um = dense_factors(A, f, kind)
         r = Lm @ (Um @ x) - b
         assert np.linalg.norm(r) <= 1e-12 * np.linalg.norm(b), name
+
+
+def test_preconditioned_solves_bitwise_vs_reference(cases, fix):
+    """GMRES / BiCGSTAB with the block forms as preconditioner: iterations,
+    status, residual history and solution equal the reference's bit for bit."""
+    for name, kind, A, ic, b in cases:
+        if kind == "ilu":
+            continue
+        for solver, kw in make_golden.ILU_SOLVES.items():
+            if solver == "gmres":
+                x, it, st, res, hist = O.gmres_ilu(A, b, kind, kw["m"], kw["maxiter"], kw["tol"],
+                                                   icolors=ic)
+            else:
+                x, it, st, res, hist = O.bicgstab_ilu(A, b, kind, kw["maxiter"], kw["tol"],
+                                                      icolors=ic)
+            assert [it, st] == list(fix[f"{name}_{solver}_it_st"]), (name, solver)
+            assert np.array_equal(hist[:it], fix[f"{name}_{solver}_hist"]), (name, solver)
+            assert np.array_equal(x, fix[f"{name}_{solver}_x"]), (name, solver)
 
 
 def test_colored_apply_equals_sequential_on_uncoupled_ranges(cases):

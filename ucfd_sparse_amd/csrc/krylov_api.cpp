@@ -1,6 +1,7 @@
-// Krylov API: sparse matrices, block LU-SGS preconditioners and restarted
-// GMRES / BiCGSTAB, executed on the GPU behind the reference's C interface
-// (src/krylov/include/ucfdmat.h, ucfdprecon.h, ucfdsolver.h).
+// Krylov API: sparse matrices, block LU-SGS and ILU(0)-family
+// preconditioners and restarted GMRES / BiCGSTAB, executed on the GPU behind
+// the reference's C interface (src/krylov/include/ucfdmat.h, ucfdprecon.h,
+// ucfdsolver.h).
 //
 // Handles keep the caller's array pointers, as the reference does
 // (sparsemat.c:32-50, precon.c:13-27): values written between calls are seen
@@ -11,7 +12,8 @@
 // schedules and the Krylov work vectors.
 //
 // Algorithms follow the reference step for step (gmres.c:4-129,
-// bicgstab.c:5-131, blusgs.c:7-100, pblusgs.c:7-104); the dot products are
+// bicgstab.c:5-131, blusgs.c:7-100, pblusgs.c:7-104, ilu.c:4-86,
+// bilu.c:5-130, pbilu.c:5-180); the dot products are
 // two-stage device reductions in a fixed order, so results are reproducible
 // but not bitwise equal to a sequential CPU sum.
 #include "krylov_internal.h"
@@ -43,12 +45,13 @@ struct _Precon {
     UCFDInt *rowptr, *colidx, *diagslots;
     UCFDReal *values;
     int bn, blk;
-    int ncolors;             // PBLUSGS: number of colour ranges
-    const UCFDInt *icolors;  // PBLUSGS: ncolors+1 offsets (caller's array)
+    int ncolors;             // PBLUSGS / PBILU: number of colour ranges
+    const UCFDInt *icolors;  // PBLUSGS / PBILU: ncolors+1 offsets (caller's array)
     bool fp32;               // mixed apply: FP32 blocks and work vector, FP64 diagonal LU
     bool prepared;
     int64_t nnz;
-    double *diagvalues;      // device, bn*blk*blk
+    double *diagvalues;      // device, bn*blk*blk: LU-SGS the factored diagonal
+                             // blocks, ILU the factored U_ii (scalar: the pivot)
     float *vals32;           // device, nnz*blk*blk (fp32 only)
     float *x32;              // device, bn*blk (fp32 only)
     bool init32;             // the lower pass reads rows it has not yet produced
@@ -150,10 +153,22 @@ struct PcDev {
     const double *values;
 };
 
-bool pc_is_block(Precon pc)
+// Sweep kind of the preconditioner's apply (KRY_SW_*), -1 when it has none.
+int pc_kind(Precon pc)
 {
-    return pc->type_name && (strcmp(pc->type_name, BLUSGS) == 0 ||
-                             strcmp(pc->type_name, PBLUSGS) == 0);
+    const char *t = pc->type_name;
+    if (!t) return -1;
+    if (strcmp(t, BLUSGS) == 0 || strcmp(t, PBLUSGS) == 0) return KRY_SW_LUSGS;
+    if (strcmp(t, BILU) == 0 || strcmp(t, PBILU) == 0) return KRY_SW_BILU;
+    if (strcmp(t, ILU) == 0) return KRY_SW_ILU;
+    return -1;
+}
+
+bool pc_active(Precon pc) { return pc_kind(pc) >= 0; }
+
+bool pc_colored(Precon pc)
+{
+    return strcmp(pc->type_name, PBLUSGS) == 0 || strcmp(pc->type_name, PBILU) == 0;
 }
 
 int stage_pc(UcfdStage &sg, Precon pc, int idx[4])
@@ -175,16 +190,17 @@ PcDev pc_dev(const UcfdStage &sg, const int idx[4])
 // Apply the preconditioner in place on the device vector b (n = bn*blk).
 int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
 {
-    if (!pc_is_block(pc)) return UCFD_OK;
+    const int kind = pc_kind(pc);
+    if (kind < 0) return UCFD_OK;
     int rc = UCFD_OK;
     if (!pc->fp32) {
         for (const UcfdLevel &l : pc->lower.lv)
-            if ((rc = kry_bsgs_sweep64(s, 0, pc->blk, l.base, l.cnt,
+            if ((rc = kry_bsgs_sweep64(s, kind, 0, pc->blk, l.base, l.cnt,
                                        l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
                                        d.diagslots, d.values, pc->diagvalues, b, l.maxw)))
                 return rc;
         for (const UcfdLevel &l : pc->upper.lv)
-            if ((rc = kry_bsgs_sweep64(s, 1, pc->blk, l.base, l.cnt,
+            if ((rc = kry_bsgs_sweep64(s, kind, 1, pc->blk, l.base, l.cnt,
                                        l.ordered ? pc->upper.order : nullptr, d.rowptr, d.colidx,
                                        d.diagslots, d.values, pc->diagvalues, b, l.maxw)))
                 return rc;
@@ -207,11 +223,17 @@ int pc_apply(hipStream_t s, Precon pc, const PcDev &d, double *b)
 // ------------------------------------------------------------ schedules
 //
 // Visit orders of the reference (see schedule.h for the levels):
-// blusgs.c:43-99 rows ascending, then descending; pblusgs.c:90-104 at one
-// thread: colour ranges ascending with rows ascending, then colour ranges
-// descending with rows ascending.  A colour-ordered matrix whose ranges hold
-// no internal couplings (the intended use of the coloured form) gets one
-// launch per colour range without building levels.
+// blusgs.c:43-99 and bilu.c:67-130 rows ascending, then descending;
+// pblusgs.c:90-104 and pbilu.c:80-180 at one thread: colour ranges ascending
+// with rows ascending, then colour ranges descending with rows ascending.  A
+// colour-ordered matrix whose ranges hold no internal couplings (the
+// intended use of the coloured forms) gets one launch per colour range
+// without building levels.
+//
+// The ILU(0) factorisations (ilu.c:4-48, bilu.c:5-65, pbilu.c:5-76) visit
+// rows in the lower sweep's order, and row i reads exactly the rows of its
+// lower slots (whole rows, where the sweep reads their vector entries) and
+// writes only itself: the lower schedule is also the factorisation's.
 
 // Blocks that the rows of one wave of the wave-staged sweep kernels use, at
 // most, over the waves of each contiguous level (rows base .. base+cnt-1 in
@@ -253,7 +275,7 @@ int make_schedules(Precon pc, const char *fn)
 
     std::vector<int> lseq, useq;
     std::vector<int> off;
-    const bool colored = strcmp(pc->type_name, PBLUSGS) == 0;
+    const bool colored = pc_colored(pc);
     if (colored) {
         if (pc->ncolors < 0 || !pc->icolors) return ucfd_report(UCFD_ERR_ARG, fn, "bad colour offsets");
         if ((rc = ucfd_fetch_ints(pc->icolors, (size_t)pc->ncolors + 1, off, fn))) return rc;
@@ -348,6 +370,7 @@ ucfd_status_t set_block(Precon *precon, UCFDInt bn, UCFDInt block, PreconType ty
     Precon pc = *precon;
     pc_free_device(pc);
     pc->type_name = type;
+    if (pc_kind(pc) != KRY_SW_LUSGS) pc->fp32 = false;
     pc->bn = bn;
     pc->blk = block;
     pc->ncolors = ncolors;
@@ -732,27 +755,31 @@ UCFD_API ucfd_status_t UCFDPreconSetCUDABLUSGS(Precon *precon, UCFDInt bn, UCFDI
     return set_block(precon, bn, block, PBLUSGS, ncolors, icolors, "UCFDPreconSetCUDABLUSGS");
 }
 
-UCFD_API ucfd_status_t UCFDPreconSetILU(Precon *, UCFDInt)
+UCFD_API ucfd_status_t UCFDPreconSetILU(Precon *precon, UCFDInt n)
 {
-    return fail("UCFDPreconSetILU", "ILU-type preconditioners are not part of this library",
-                UCFD_ERR_STATE);
+    return set_block(precon, n, 1, ILU, 0, nullptr, "UCFDPreconSetILU");
 }
 
-UCFD_API ucfd_status_t UCFDPreconSetBILU(Precon *, UCFDInt, UCFDInt)
+UCFD_API ucfd_status_t UCFDPreconSetBILU(Precon *precon, UCFDInt bn, UCFDInt block)
 {
-    return fail("UCFDPreconSetBILU", "ILU-type preconditioners are not part of this library",
-                UCFD_ERR_STATE);
+    return set_block(precon, bn, block, BILU, 0, nullptr, "UCFDPreconSetBILU");
 }
 
-UCFD_API ucfd_status_t UCFDPreconSetPBILU(Precon *, UCFDInt, UCFDInt, UCFDInt, UCFDInt *)
+UCFD_API ucfd_status_t UCFDPreconSetPBILU(Precon *precon, UCFDInt bn, UCFDInt block,
+                                          UCFDInt ncolors, UCFDInt *icolors)
 {
-    return fail("UCFDPreconSetPBILU", "ILU-type preconditioners are not part of this library",
-                UCFD_ERR_STATE);
+    return set_block(precon, bn, block, PBILU, ncolors, icolors, "UCFDPreconSetPBILU");
+}
+
+UCFD_API ucfd_status_t UCFDPreconSetCUDABILU(Precon *precon, UCFDInt bn, UCFDInt block,
+                                             UCFDInt ncolors, UCFDInt *icolors)
+{
+    return set_block(precon, bn, block, PBILU, ncolors, icolors, "UCFDPreconSetCUDABILU");
 }
 
 UCFD_API ucfd_status_t UCFDPreconSetPrecision(Precon precon, UCFDInt fp32)
 {
-    if (!precon || !pc_is_block(precon))
+    if (!precon || pc_kind(precon) != KRY_SW_LUSGS)
         return fail("UCFDPreconSetPrecision", "a block LU-SGS preconditioner is required");
     if (precon->fp32 != (fp32 != 0)) precon->prepared = false;
     precon->fp32 = fp32 != 0;
@@ -763,7 +790,7 @@ UCFD_API ucfd_status_t UCFDPreconPrepare(Precon precon)
 {
     const char *fn = "UCFDPreconPrepare";
     if (!precon || !precon->type_name) return fail(fn, "Preconditioner type must be set", UCFD_ERR_STATE);
-    if (!pc_is_block(precon)) return UCFD_SUCCESS;
+    if (!pc_active(precon)) return UCFD_SUCCESS;
     Precon pc = precon;
     int rc = pc_alloc(pc, fn);
     if (rc) return UCFD_FAILED;
@@ -771,6 +798,28 @@ UCFD_API ucfd_status_t UCFDPreconPrepare(Precon precon)
     if (!pc->colidx || !pc->diagslots || !pc->values) return fail(fn, "NULL preconditioner array");
     if ((rc = make_schedules(pc, fn))) return UCFD_FAILED;
     const size_t b2 = (size_t)pc->blk*pc->blk;
+    const int kind = pc_kind(pc);
+    if (kind != KRY_SW_LUSGS) {
+        // ILU(0): factored in place on the caller's values, as the reference does
+        UcfdStage sg(fn);
+        int pi[4];
+        pi[0] = sg.add(pc->rowptr, sizeof(int)*((size_t)pc->bn + 1), UCFD_IN);
+        pi[1] = sg.add(pc->colidx, sizeof(int)*(size_t)pc->nnz, UCFD_IN);
+        pi[2] = sg.add(pc->diagslots, sizeof(int)*(size_t)pc->bn, UCFD_IN);
+        pi[3] = sg.add(pc->values, sizeof(double)*(size_t)pc->nnz*b2, UCFD_INOUT);
+        if ((rc = sg.enter())) return UCFD_FAILED;
+        const PcDev d = pc_dev(sg, pi);
+        for (const UcfdLevel &l : pc->lower.lv)
+            if ((rc = kry_ilu_factor(ucfd_stream(), kind == KRY_SW_ILU, pc->blk, l.base, l.cnt,
+                                     l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
+                                     d.diagslots, (double *)d.values, pc->diagvalues,
+                                     ucfd_device_flag())))
+                break;
+        if (!rc) rc = ucfd_sync_and_check(fn);
+        rc = sg.leave(rc);
+        pc->prepared = rc == UCFD_OK;
+        return status(rc);
+    }
     if (pc->fp32) {
         if (pc->vals32) (void)hipFree(pc->vals32);
         if (pc->x32) (void)hipFree(pc->x32);
@@ -799,7 +848,7 @@ UCFD_API ucfd_status_t UCFDPreconApply(Precon precon, UCFDReal *b)
 {
     const char *fn = "UCFDPreconApply";
     if (!precon || !precon->type_name) return fail(fn, "Preconditioner type must be set", UCFD_ERR_STATE);
-    if (!pc_is_block(precon)) return UCFD_SUCCESS;
+    if (!pc_active(precon)) return UCFD_SUCCESS;
     if (!precon->prepared) return fail(fn, "UCFDPreconPrepare must be called first", UCFD_ERR_STATE);
     UcfdStage sg(fn);
     int pi[4];
@@ -896,7 +945,7 @@ UCFD_API ucfd_status_t UCFDSolve(Solver solver, Precon pc, SpMat A, UCFDReal *x,
     int rc = mat_ready(A, fn);
     if (rc) return UCFD_FAILED;
     if (A->n != solver->n) return fail(fn, "matrix and solver sizes differ");
-    const bool blockpc = pc_is_block(pc);
+    const bool blockpc = pc_active(pc);
     if (blockpc) {
         if (!pc->prepared) return fail(fn, "UCFDPreconPrepare must be called first", UCFD_ERR_STATE);
         if (pc->bn*pc->blk != solver->n) return fail(fn, "preconditioner and solver sizes differ");

@@ -29,12 +29,27 @@ UCFD_HIDDEN int kry_bsgs_prepare(hipStream_t s, int bn, int blk, int64_t nnz, co
                                  const double *values, double *diagvalues, float *vals32,
                                  int *flag);
 
+// One level of the in-place ILU(0) factorisation (rows base..base+cnt-1, or
+// order[base..]): scalar != 0 is ILU(0) of a CSR matrix (blk 1, ilu.c:4-48),
+// else block ILU(0) (bilu.c:5-65).  diagvalues[i] gets row i's factored
+// diagonal block (scalar: its pivot); raises UCFD_KFLAG_PIVOT on a zero pivot.
+UCFD_HIDDEN int kry_ilu_factor(hipStream_t s, int scalar, int blk, int base, int cnt,
+                               const int *order, const int *rowptr, const int *colidx,
+                               const int *diagslots, double *values, double *diagvalues,
+                               int *flag);
+
+// Sweep kinds of kry_bsgs_sweep64
+enum { KRY_SW_LUSGS = 0, KRY_SW_BILU = 1, KRY_SW_ILU = 2 };
+
 // One sweep over rows base..base+cnt-1 (or order[base..base+cnt-1] when
-// order != NULL).  lower: x_i = D_i^-1 (b_i - sum_{j<diag} A_ij x_j);
-// upper: x_i -= D_i^-1 sum_{j>diag} A_ij x_j.  FP64 form: in place on b.
+// order != NULL), in place on b.  KRY_SW_LUSGS: lower x_i = D_i^-1 (b_i -
+// sum_{j<diag} A_ij x_j), upper x_i -= D_i^-1 sum_{j>diag} A_ij x_j.
+// KRY_SW_BILU (factored values): lower x_i = b_i - sum_{j<diag} L_ij x_j,
+// upper x_i = U_ii^-1 (x_i - sum_{j>diag} U_ij x_j) with diagvalues the
+// factored U_ii.  KRY_SW_ILU: the scalar form (blk 1), dividing by the pivot.
 // maxw: the most blocks the rows of one wave (64/blk consecutive rows) use
 // in this direction (UcfdLevel::maxw; -1 = unknown): sizes the LDS staging.
-UCFD_HIDDEN int kry_bsgs_sweep64(hipStream_t s, int upper, int blk, int base, int cnt,
+UCFD_HIDDEN int kry_bsgs_sweep64(hipStream_t s, int kind, int upper, int blk, int base, int cnt,
                                  const int *order, const int *rowptr, const int *colidx,
                                  const int *diagslots, const double *values,
                                  const double *diagvalues, double *b, int maxw);

@@ -1,12 +1,13 @@
 // Kernels of the Krylov path: BSR / CSR SpMV, block LU-SGS (prepare, lower
-// and upper sweeps in FP64 and in the mixed FP32/FP64 form), BLAS-1/2 with
+// and upper sweeps in FP64 and in the mixed FP32/FP64 form), the ILU(0)
+// family (level-scheduled factorisation, triangular solves), BLAS-1/2 with
 // deterministic reductions, and the fused BiCGSTAB vector updates.
 //
 // FP64 kernels keep the operation order of the reference
-// (src/krylov/matrix/sparsemat.c:57-101, src/krylov/utils/inverse.c:7-71,
-// src/krylov/precon/blusgs.c:28-100) and are compiled without FMA
-// contraction, so SpMV and the FP64 preconditioner apply agree bit for bit
-// with the CPU oracle.  Reductions sum block partials in a fixed order, so
+// (src/krylov/matrix/sparsemat.c:57-101, src/krylov/utils/inverse.c:7-107,
+// src/krylov/precon/blusgs.c:28-100, ilu.c:4-86, bilu.c:5-130, pbilu.c:5-180)
+// and are compiled without FMA contraction, so SpMV, the factorisations and
+// the FP64 preconditioner applies agree bit for bit with the CPU oracle.  Reductions sum block partials in a fixed order, so
 // every dot product is reproducible run to run.
 #include <hip/hip_runtime.h>
 
@@ -208,6 +209,86 @@ __global__ __launch_bounds__(KB) void k_prepare(int bn, const int *__restrict__ 
     for (int q = 0; q < B*B; ++q) dv[(int64_t)i*B*B + q] = a[q];
 }
 
+// ILU(0) factorisation, one level of rows per launch, one thread per row, in
+// place on the values (ilu.c:4-48, bilu.c:5-65, pbilu.c:5-76 at one thread).
+// For each lower slot k of row i, with ck = colidx[k]:
+//   L_ik := A_ik U_kk^-1   (block: lusubmattrans with the factored diagonal
+//                           block of row ck; scalar: division by its pivot)
+//   A_ij -= L_ik A_{ck,j}  for each later slot j of row i whose column row ck
+//                          also holds (the last such slot of row ck: the
+//                          reference's marker array keeps the last write)
+// then the block form factors its diagonal block in place (ludcmp).  dv gets
+// the row's factored diagonal block (scalar: the pivot) for the solves.  A
+// level holds rows that read only rows of earlier levels (complete) or of
+// later ones (untouched), as the reference's row order has them.
+template <int B, bool SCALAR>
+__global__ __launch_bounds__(KB) void k_ilu_factor(int base, int cnt, const int *__restrict__ order,
+                                                   const int *__restrict__ rowptr,
+                                                   const int *__restrict__ colidx,
+                                                   const int *__restrict__ diagslots,
+                                                   double *values, double *__restrict__ dv,
+                                                   int *flag)
+{
+    static_assert(!SCALAR || B == 1, "the scalar form has 1x1 blocks");
+    constexpr int B2 = B*B;
+    const int t = kxcd()*KB + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = order ? order[base + t] : base + t;
+    const int st = rowptr[i], ed = diagslots[i], jed = rowptr[i + 1];
+    bool bad = false;
+    for (int k = st; k < ed; ++k) {
+        const int ck = colidx[k], kk = diagslots[ck], kst = rowptr[ck], ked = rowptr[ck + 1];
+        double aik[B2];
+        if (SCALAR) {
+            const double p = values[kk];
+            bad |= p == 0.0;
+            aik[0] = values[k]/p;
+            values[k] = aik[0];
+        } else {
+#pragma unroll
+            for (int q = 0; q < B2; ++q) aik[q] = values[(int64_t)k*B2 + q];
+            ucfd::lusubmattrans<B>(values + (int64_t)kk*B2, aik);
+#pragma unroll
+            for (int q = 0; q < B2; ++q) values[(int64_t)k*B2 + q] = aik[q];
+        }
+        for (int j = k + 1; j < jed; ++j) {
+            const int col = colidx[j];
+            int iwj = -1;
+            for (int q = kst; q < ked; ++q)
+                if (colidx[q] == col) iwj = q;
+            if (iwj < 0) continue;
+            if (SCALAR) {
+                values[j] -= aik[0]*values[iwj];
+                continue;
+            }
+            const double *u = values + (int64_t)iwj*B2;
+            double *a = values + (int64_t)j*B2;
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int e = 0; e < B; ++e) v += aik[r*B + e]*u[e*B + c];
+                    a[r*B + c] -= v;
+                }
+        }
+    }
+    double d[B2];
+#pragma unroll
+    for (int q = 0; q < B2; ++q) d[q] = values[(int64_t)ed*B2 + q];
+    if (SCALAR) {
+        bad |= d[0] == 0.0;
+    } else {
+        bad |= !ucfd::ludcmp<B>(d);
+#pragma unroll
+        for (int q = 0; q < B2; ++q) values[(int64_t)ed*B2 + q] = d[q];
+    }
+#pragma unroll
+    for (int q = 0; q < B2; ++q) dv[(int64_t)i*B2 + q] = d[q];
+    if (bad) atomicOr(flag, UCFD_KFLAG_PIVOT);
+}
+
 __global__ void k_to32(int64_t n, const double *__restrict__ a, float *__restrict__ b)
 {
     const int64_t st = (int64_t)gridDim.x*KB;
@@ -215,7 +296,18 @@ __global__ void k_to32(int64_t n, const double *__restrict__ a, float *__restric
 }
 
 // ----------------------------------------------------------- block sweeps
-template <int B, bool UPPER>
+//
+// Sweep kinds (template parameter K) of the FP64 kernels:
+//   SW_LUSGS  block LU-SGS (blusgs.c:28-100): lower x_i = D_i^-1 (b_i -
+//             sum_{j<diag} A_ij x_j); upper x_i -= D_i^-1 sum_{j>diag} A_ij x_j
+//   SW_BILU   block ILU(0) solve with the factored blocks (bilu.c:67-130):
+//             lower x_i = b_i - sum_{j<diag} L_ij x_j; upper x_i = U_ii^-1
+//             (x_i - sum_{j>diag} U_ij x_j), U_ii^-1 by LU substitution
+//   SW_ILU    scalar ILU(0) solve (ilu.c:50-86, B = 1): the same with the
+//             products subtracted one by one and a division by U_ii
+constexpr int SW_LUSGS = KRY_SW_LUSGS, SW_BILU = KRY_SW_BILU, SW_ILU = KRY_SW_ILU;
+
+template <int B, bool UPPER, int K>
 __global__ __launch_bounds__(KB) void k_sweep64(int base, int cnt, const int *__restrict__ order,
                                                 const int *__restrict__ rowptr,
                                                 const int *__restrict__ colidx,
@@ -223,12 +315,13 @@ __global__ __launch_bounds__(KB) void k_sweep64(int base, int cnt, const int *__
                                                 const double *__restrict__ values,
                                                 const double *__restrict__ dv, double *b)
 {
+    static_assert(K != SW_ILU || B == 1, "the scalar ILU solve has 1x1 blocks");
     const int t = kxcd()*KB + threadIdx.x;
     if (t >= cnt) return;
     const int i = order ? order[base + t] : base + t;
     const int dd = diagslots[i];
     double arr[B];
-    if (!UPPER) {
+    if (!UPPER || K != SW_LUSGS) {
 #pragma unroll
         for (int k = 0; k < B; ++k) arr[k] = b[(int64_t)i*B + k];
     } else {
@@ -239,6 +332,10 @@ __global__ __launch_bounds__(KB) void k_sweep64(int base, int cnt, const int *__
     const int j1 = UPPER ? rowptr[i + 1] : dd;
     for (int j = j0; j < j1; ++j) {
         const int c = colidx[j];
+        if (K == SW_ILU) {
+            arr[0] -= __builtin_nontemporal_load(values + j)*b[c];
+            continue;
+        }
         const double *blk = values + (int64_t)j*B*B;
         double xc[B];
 #pragma unroll
@@ -248,9 +345,18 @@ __global__ __launch_bounds__(KB) void k_sweep64(int base, int cnt, const int *__
             double v = 0.0;
 #pragma unroll
             for (int q = 0; q < B; ++q) v += __builtin_nontemporal_load(blk + r*B + q)*xc[q];
-            if (UPPER) arr[r] += v;
+            if (UPPER && K == SW_LUSGS) arr[r] += v;
             else arr[r] -= v;
         }
+    }
+    if (K != SW_LUSGS && !UPPER) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) b[(int64_t)i*B + k] = arr[k];
+        return;
+    }
+    if (K == SW_ILU) {
+        b[i] = arr[0]/dv[i];
+        return;
     }
     double lu[B*B];
 #pragma unroll
@@ -259,7 +365,7 @@ __global__ __launch_bounds__(KB) void k_sweep64(int base, int cnt, const int *__
 #pragma unroll
     for (int k = 0; k < B; ++k) {
         const int64_t o = (int64_t)i*B + k;
-        if (UPPER) b[o] -= arr[k];
+        if (UPPER && K == SW_LUSGS) b[o] -= arr[k];
         else b[o] = arr[k];
     }
 }
@@ -324,11 +430,12 @@ __global__ __launch_bounds__(KB) void k_sweep32(int base, int cnt, const int *__
 // the row's lower (lower sweep) or upper (upper sweep) blocks in the order
 // of k_sweep64 / k_sweep32 above; the group then exchanges its B sums with
 // lane shuffles and every lane runs the row's diagonal solve (bit-identical
-// to the thread-per-row kernels).  When the wave's blocks fit the cap words
+// to the thread-per-row kernels).  K selects the FP64 sweep kind (SW_LUSGS
+// or SW_BILU; the mixed form is LU-SGS only).  When the wave's blocks fit the cap words
 // of LDS the launch provides (sized from the level's largest wave) they are
 // staged through LDS (per row, contiguous loads); otherwise the lanes read
 // them from HBM directly.  The rows' LU blocks are always staged.
-template <int B, bool UPPER, bool F32, bool STAGED>
+template <int B, bool UPPER, bool F32, bool STAGED, int K>
 __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0, int n, int off,
                                           const int *__restrict__ colidx,
                                           const typename std::conditional<F32, float, double>::type
@@ -337,6 +444,10 @@ __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0,
 {
     using T = typename std::conditional<F32, float, double>::type;
     constexpr int B2 = B*B, KP = 5;
+    static_assert(K == SW_LUSGS || (K == SW_BILU && !F32), "sweep kind");
+    // accumulate with the sign of the reference: LU-SGS adds the upper
+    // products and applies D^-1 to the sum, the others subtract from x_i
+    constexpr bool ADD = UPPER && K == SW_LUSGS;
     // column x rows of the first KP blocks
     T xp[KP][B];
 #pragma unroll
@@ -349,7 +460,7 @@ __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0,
         }
     T acc = T(0);
     double a = 0.0;
-    if (!F32 && !UPPER && act) a = b[(int64_t)i*B + r];
+    if (!F32 && !ADD && act) a = b[(int64_t)i*B + r];
 #pragma unroll
     for (int k = 0; k < KP; ++k)
         if (k < n) {
@@ -358,7 +469,7 @@ __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0,
 #pragma unroll
             for (int q = 0; q < B; ++q) v += blk[q]*xp[k][q];
             if (F32) acc += v;
-            else if (UPPER) a += (double)v;
+            else if (ADD) a += (double)v;
             else a -= (double)v;
         }
     for (int k = KP; k < n; ++k) {
@@ -371,8 +482,12 @@ __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0,
 #pragma unroll
         for (int q = 0; q < B; ++q) v += blk[q]*xq[q];
         if (F32) acc += v;
-        else if (UPPER) a += (double)v;
+        else if (ADD) a += (double)v;
         else a -= (double)v;
+    }
+    if (K == SW_BILU && !UPPER) {
+        if (act) b[(int64_t)i*B + r] = a;
+        return;
     }
     if (F32) {
         if (UPPER) a = (double)acc;
@@ -396,12 +511,12 @@ __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0,
             x32[o] = (float)arr[r];
         }
     } else {
-        if (UPPER) b[o] -= arr[r];
+        if (ADD) b[o] -= arr[r];
         else b[o] = arr[r];
     }
 }
 
-template <int B, bool UPPER, bool F32>
+template <int B, bool UPPER, bool F32, int K>
 __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
                                                const int *__restrict__ rowptr,
                                                const int *__restrict__ colidx,
@@ -438,7 +553,7 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
     const int off = __shfl(ex, g*B < 64 ? g*B : 0, 64);
     const int nrows = min(G, cnt - t0);
     if (total*B2 > cap) {
-        bsgs_rows<B, UPPER, F32, false>(r, g, act, i, j0, n, 0, colidx, vals,
+        bsgs_rows<B, UPPER, F32, false, K>(r, g, act, i, j0, n, 0, colidx, vals,
                                         dv + (int64_t)i*B2, x32, b);
         return;
     }
@@ -453,12 +568,14 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
             t[gg][k] = e < ng ? __builtin_nontemporal_load(vals + (int64_t)jg*B2 + e) : T(0);
         }
     }
+    // the block ILU lower solve has a unit diagonal: no LU blocks
+    constexpr bool LU = !(K == SW_BILU && !UPPER);
     double tl[NLU];
     const double *lsrc = dv + (int64_t)(base + t0)*B2;
 #pragma unroll
     for (int k = 0; k < NLU; ++k) {
         const int e = k*64 + lane;
-        tl[k] = e < nrows*B2 ? __builtin_nontemporal_load(lsrc + e) : 0.0;
+        tl[k] = LU && e < nrows*B2 ? __builtin_nontemporal_load(lsrc + e) : 0.0;
     }
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
@@ -474,10 +591,10 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
 #pragma unroll
     for (int k = 0; k < NLU; ++k) {
         const int e = k*64 + lane;
-        if (e < nrows*B2) slu[e] = tl[k];
+        if (LU && e < nrows*B2) slu[e] = tl[k];
     }
     ucfd::wave_lds_sync();
-    bsgs_rows<B, UPPER, F32, true>(r, g, act, i, j0, n, off, colidx, sv, slu + g*B2, x32, b);
+    bsgs_rows<B, UPPER, F32, true, K>(r, g, act, i, j0, n, off, colidx, sv, slu + g*B2, x32, b);
 }
 
 // --------------------------------------------------------------- BLAS
@@ -696,6 +813,23 @@ template <int B> struct PrepL {
     }
 };
 
+template <int B> struct IluL {
+    static int run(hipStream_t s, bool scalar, int base, int cnt, const int *ord, const int *rp,
+                   const int *ci, const int *ds, double *v, double *dv, int *flag)
+    {
+        const dim3 g((cnt + KB - 1)/KB);
+        if (scalar) {
+            if (B != 1) return UCFD_ERR_ARG;
+            hipLaunchKernelGGL((k_ilu_factor<1, true>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci,
+                               ds, v, dv, flag);
+        } else {
+            hipLaunchKernelGGL((k_ilu_factor<B, false>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci,
+                               ds, v, dv, flag);
+        }
+        return ok();
+    }
+};
+
 // LDS words for the staged blocks of a wave-staged sweep launch
 template <int B, typename T> int stage_cap(int maxw)
 {
@@ -708,31 +842,51 @@ template <int B, typename T> size_t stage_bytes(int cap)
     return (size_t)(64/B)*B*B*sizeof(double) + (size_t)cap*sizeof(T);
 }
 
-template <int B> struct Sw64L {
-    static int run(hipStream_t s, int upper, int base, int cnt, const int *ord, const int *rp,
+template <int B, int K>
+int sweep64_launch(hipStream_t s, int upper, int base, int cnt, const int *ord, const int *rp,
                    const int *ci, const int *ds, const double *v, const double *dv, double *b,
                    int maxw)
-    {
-        if (B > 1 && !ord) {
+{
+    if constexpr (B > 1) {
+        if (!ord) {
             const dim3 g((cnt + 64/B - 1)/(64/B));
             const int cap = stage_cap<B, double>(maxw);
             const size_t lds = stage_bytes<B, double>(cap);
             if (upper)
-                hipLaunchKernelGGL((k_bsgs_w<B, true, false>), g, dim3(64), lds, s, base, cnt, cap,
-                                   rp, ci, ds, v, dv, nullptr, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, true, false, K>), g, dim3(64), lds, s, base, cnt,
+                                   cap, rp, ci, ds, v, dv, nullptr, b);
             else
-                hipLaunchKernelGGL((k_bsgs_w<B, false, false>), g, dim3(64), lds, s, base, cnt, cap,
-                                   rp, ci, ds, v, dv, nullptr, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, false, false, K>), g, dim3(64), lds, s, base, cnt,
+                                   cap, rp, ci, ds, v, dv, nullptr, b);
             return ok();
         }
-        const dim3 g((cnt + KB - 1)/KB);
-        if (upper)
-            hipLaunchKernelGGL((k_sweep64<B, true>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci, ds, v,
-                               dv, b);
-        else
-            hipLaunchKernelGGL((k_sweep64<B, false>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci, ds,
-                               v, dv, b);
-        return ok();
+    }
+    const dim3 g((cnt + KB - 1)/KB);
+    if (upper)
+        hipLaunchKernelGGL((k_sweep64<B, true, K>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci, ds, v,
+                           dv, b);
+    else
+        hipLaunchKernelGGL((k_sweep64<B, false, K>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci, ds,
+                           v, dv, b);
+    return ok();
+}
+
+template <int B> struct Sw64L {
+    static int run(hipStream_t s, int kind, int upper, int base, int cnt, const int *ord,
+                   const int *rp, const int *ci, const int *ds, const double *v, const double *dv,
+                   double *b, int maxw)
+    {
+        switch (kind) {
+        case SW_LUSGS:
+            return sweep64_launch<B, SW_LUSGS>(s, upper, base, cnt, ord, rp, ci, ds, v, dv, b, maxw);
+        case SW_BILU:
+            return sweep64_launch<B, SW_BILU>(s, upper, base, cnt, ord, rp, ci, ds, v, dv, b, maxw);
+        case SW_ILU:
+            if (B != 1) return UCFD_ERR_ARG;
+            return sweep64_launch<1, SW_ILU>(s, upper, base, cnt, ord, rp, ci, ds, v, dv, b, maxw);
+        default:
+            return UCFD_ERR_ARG;
+        }
     }
 };
 
@@ -746,11 +900,11 @@ template <int B> struct Sw32L {
             const int cap = stage_cap<B, float>(maxw);
             const size_t lds = stage_bytes<B, float>(cap);
             if (upper)
-                hipLaunchKernelGGL((k_bsgs_w<B, true, true>), g, dim3(64), lds, s, base, cnt, cap,
-                                   rp, ci, ds, v, dv, x32, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, true, true, SW_LUSGS>), g, dim3(64), lds, s, base,
+                                   cnt, cap, rp, ci, ds, v, dv, x32, b);
             else
-                hipLaunchKernelGGL((k_bsgs_w<B, false, true>), g, dim3(64), lds, s, base, cnt, cap,
-                                   rp, ci, ds, v, dv, x32, b);
+                hipLaunchKernelGGL((k_bsgs_w<B, false, true, SW_LUSGS>), g, dim3(64), lds, s, base,
+                                   cnt, cap, rp, ci, ds, v, dv, x32, b);
             return ok();
         }
         const dim3 g((cnt + KB - 1)/KB);
@@ -794,13 +948,22 @@ int kry_bsgs_prepare(hipStream_t s, int bn, int blk, int64_t nnz, const int *dia
     return ok();
 }
 
-int kry_bsgs_sweep64(hipStream_t s, int upper, int blk, int base, int cnt, const int *order,
-                     const int *rowptr, const int *colidx, const int *diagslots,
+int kry_ilu_factor(hipStream_t s, int scalar, int blk, int base, int cnt, const int *order,
+                   const int *rowptr, const int *colidx, const int *diagslots, double *values,
+                   double *diagvalues, int *flag)
+{
+    if (cnt <= 0) return UCFD_OK;
+    return by_block<IluL>(blk, s, scalar != 0, base, cnt, order, rowptr, colidx, diagslots, values,
+                          diagvalues, flag);
+}
+
+int kry_bsgs_sweep64(hipStream_t s, int kind, int upper, int blk, int base, int cnt,
+                     const int *order, const int *rowptr, const int *colidx, const int *diagslots,
                      const double *values, const double *diagvalues, double *b, int maxw)
 {
     if (cnt <= 0) return UCFD_OK;
-    return by_block<Sw64L>(blk, s, upper, base, cnt, order, rowptr, colidx, diagslots, values,
-                           diagvalues, b, maxw);
+    return by_block<Sw64L>(blk, s, kind, upper, base, cnt, order, rowptr, colidx, diagslots,
+                           values, diagvalues, b, maxw);
 }
 
 int kry_bsgs_sweep32(hipStream_t s, int upper, int blk, int base, int cnt, const int *order,

@@ -1,7 +1,7 @@
 """Python handles of the Krylov API of liblusgs.so (include/ucfdmat.h,
 ucfdprecon.h, ucfdsolver.h): BSR / CSR matrices, block LU-SGS
-preconditioners (sequential and coloured, FP64 or mixed FP32 apply) and
-GMRES / BiCGSTAB.
+preconditioners (sequential and coloured, FP64 or mixed FP32 apply), the
+ILU(0) family (scalar, block, coloured block) and GMRES / BiCGSTAB.
 
 The handles keep references to the arrays they were built from, because the
 library keeps the pointers (as the reference does, src/krylov/matrix/
@@ -76,9 +76,14 @@ class SpMat:
 
 
 class Precon:
-    """Block LU-SGS preconditioner.  ``icolors`` (colour offsets, length
-    ncolors+1) selects the coloured form (pblusgs.c:107-127), else the
-    sequential one (blusgs.c:111-128); ``fp32`` selects the mixed apply."""
+    """Block LU-SGS (``kind`` None or "blusgs") or ILU(0)-family ("ilu":
+    scalar ILU(0) of a CSR matrix, blk 1, ilu.c:99-117; "bilu": block ILU(0),
+    bilu.c:138-158) preconditioner.  ``icolors`` (colour offsets, length
+    ncolors+1) selects the coloured form (pblusgs.c:107-127,
+    pbilu.c:184-207), else the sequential one (blusgs.c:111-128); ``fp32``
+    selects the mixed apply (block LU-SGS only).  The ILU kinds factor
+    ``values`` in place at prepare, as the reference does: a float64
+    contiguous array (or device tensor) passed in is overwritten."""
 
     def __init__(self, rowptr=None, colidx=None, diagslots=None, values=None, bn=0, blk=5,
                  icolors=None, fp32=False, kind=None):
@@ -95,7 +100,21 @@ class Precon:
             ctypes.byref(h), L.ptr(self.rowptr), L.ptr(self.colidx), L.ptr(self.diagslots),
             L.ptr(self.values)), "UCFDPreconCreatefromArrays")
         self.handle = h
-        if icolors is None:
+        if kind == "ilu":
+            if int(blk) != 1 or icolors is not None:
+                raise ValueError("the scalar ILU(0) takes a CSR matrix (blk 1), no colours")
+            L.check(self._lib.UCFDPreconSetILU(ctypes.byref(h), int(bn)), "UCFDPreconSetILU")
+        elif kind == "bilu" and icolors is None:
+            L.check(self._lib.UCFDPreconSetBILU(ctypes.byref(h), int(bn), int(blk)),
+                    "UCFDPreconSetBILU")
+        elif kind == "bilu":
+            self.icolors = np.ascontiguousarray(icolors, np.int32)
+            L.check(self._lib.UCFDPreconSetPBILU(ctypes.byref(h), int(bn), int(blk),
+                                                 len(self.icolors) - 1, self.icolors.ctypes.data),
+                    "UCFDPreconSetPBILU")
+        elif kind not in (None, "blusgs"):
+            raise ValueError(f"unknown preconditioner kind {kind!r}")
+        elif icolors is None:
             L.check(self._lib.UCFDPreconSetBLUSGS(ctypes.byref(h), int(bn), int(blk)),
                     "UCFDPreconSetBLUSGS")
         else:
@@ -112,8 +131,8 @@ class Precon:
         return cls(kind="none")
 
     @classmethod
-    def from_bsr(cls, A, icolors=None, fp32=False):
-        return cls(A.rowptr, A.colidx, A.diagslots, A.values, A.bn, A.blk, icolors, fp32)
+    def from_bsr(cls, A, icolors=None, fp32=False, kind=None):
+        return cls(A.rowptr, A.colidx, A.diagslots, A.values, A.bn, A.blk, icolors, fp32, kind)
 
     def prepare(self):
         L.check(self._lib.UCFDPreconPrepare(self.handle), "UCFDPreconPrepare")

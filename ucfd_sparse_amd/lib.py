@@ -84,6 +84,7 @@ _SIGS = {
     "UCFDPreconSetILU": [c_void_p, c_int],
     "UCFDPreconSetBILU": [c_void_p, c_int, c_int],
     "UCFDPreconSetPBILU": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "UCFDPreconSetCUDABILU": [c_void_p, c_int, c_int, c_int, c_void_p],
     "UCFDPreconApply": [c_void_p, c_void_p],
     "UCFDPreconSetPrecision": [c_void_p, c_int],
     "UCFDSolverInit": [c_void_p],
