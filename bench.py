@@ -160,7 +160,9 @@ def krylov_workload(args, log):
     LU-SGS with the FP32 apply (FP64 diagonal LU), FP64 Krylov vectors, dot
     products and Givens rotations, rtol 1e-10, 4 seeded N(0,1) right-hand
     sides; all arrays device-resident.  Also times the SpMV and the apply
-    kernels and the all-FP64 preconditioner for comparison."""
+    kernels, the all-FP64 preconditioner and the coloured block ILU(0)
+    (factorisation, apply, and as preconditioner of the same solves) for
+    comparison."""
     import numpy as np
     import torch
 
@@ -195,6 +197,16 @@ def krylov_workload(args, log):
     mat = K.SpMat.from_bsr(Dv)
     pcs = {"fp32": K.Precon.from_bsr(Dv, icolors=lay.off, fp32=True),
            "fp64": K.Precon.from_bsr(Dv, icolors=lay.off)}
+    # block ILU(0) factors its values in place: a copy of the matrix values
+    Di = D()
+    Di.bn, Di.blk, Di.rowptr, Di.colidx, Di.diagslots = Dv.bn, Dv.blk, Dv.rowptr, Dv.colidx, \
+        Dv.diagslots
+    Di.values = Dv.values.clone()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pcs["pbilu"] = K.Precon.from_bsr(Di, icolors=lay.off, kind="bilu")
+    torch.cuda.synchronize()
+    ilu_setup_ms = (time.perf_counter() - t1) * 1e3
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.synchronize()
     L.call("ucfd_set_stream", stream.cuda_stream)
@@ -215,16 +227,24 @@ def krylov_workload(args, log):
     y = torch.zeros_like(x)
     b_spmv, b_ap32 = krylov_bytes(Dv, nnz, n, True)
     _, b_ap64 = krylov_bytes(Dv, nnz, n, False)
+    # the block ILU(0) solve reads no diagonal block in its lower sweep
+    b_ilu = b_ap64 - Dv.bn * Dv.blk * Dv.blk * 8
     kern = {}
     ms = timed(lambda: mat.mult(rhs[0], y), 20)
     kern["spmv_bsr5"] = {"ms": ms, "algorithmic_bytes": b_spmv, "gbs": b_spmv / ms / 1e6}
     w = rhs[0].clone()
-    for name, bb in (("fp32", b_ap32), ("fp64", b_ap64)):
+    for key, name, bb in (("blusgs_apply_fp32", "fp32", b_ap32),
+                          ("blusgs_apply_fp64", "fp64", b_ap64),
+                          ("pbilu_apply_fp64", "pbilu", b_ilu)):
         ms = timed(lambda: pcs[name].apply(w), 10)
-        kern[f"blusgs_apply_{name}"] = {"ms": ms, "algorithmic_bytes": bb, "gbs": bb / ms / 1e6,
-                                        "launches": 2 * lay.ncolors}
+        kern[key] = {"ms": ms, "algorithmic_bytes": bb, "gbs": bb / ms / 1e6,
+                     "launches": 2 * lay.ncolors}
     for k, v in kern.items():
         log(f"{k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s algorithmic")
+    kern["pbilu_prepare"] = {"ms": ilu_setup_ms, "launches": lay.ncolors,
+                             "note": "wall time of UCFDPreconPrepare: structure checks, "
+                                     "schedule and the in-place factorisation"}
+    log(f"pbilu prepare (schedule + factorisation): {ilu_setup_ms:.1f} ms")
 
     # the solves and the residual checks below run on the default stream
     # (library and torch alike)
@@ -232,7 +252,7 @@ def krylov_workload(args, log):
     torch.cuda.synchronize()
     solves = {}
     for kind in ("gmres", "bicgstab"):
-        for pname in ("fp32", "fp64"):
+        for pname in ("fp32", "fp64", "pbilu"):
             s = K.Solver(kind, n, 200 if kind == "gmres" else 2000, args.rtol, m=30, right=True)
             rec = []
             for k, b in enumerate(rhs):
