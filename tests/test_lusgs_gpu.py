@@ -25,6 +25,11 @@ GOLD = os.path.join(REPO, "tests", "golden")
 RTOL = 1e-12
 
 
+def _host(a):
+    """A host copy of a numpy array or a device tensor."""
+    return a.cpu().numpy().copy() if hasattr(a, "cpu") else np.array(a, copy=True)
+
+
 def say(msg):
     print(msg, flush=True)
     sys.stdout.flush()
@@ -82,7 +87,7 @@ def colored_pass_lib(mesh, flow, lcolor, icolor, to_dev=None, two_array=False):
         else:
             L.call("parallel_ns_lower_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
                    fnv, vfn, upts, dub, diag, fsp)
-    dstar = np.array(dub.cpu() if hasattr(dub, "cpu") else dub, copy=True)
+    dstar = _host(dub)
     for c in reversed(range(nc)):
         if two_array:
             L.call("ns_parallel_upper_sweep", int(off[c]), int(off[c + 1]), N, F, nei, ic, lc,
@@ -93,9 +98,7 @@ def colored_pass_lib(mesh, flow, lcolor, icolor, to_dev=None, two_array=False):
     x = rhsb if two_array else dub
     L.call("lusgs_parallel_ns_update", N, upts, x)
 
-    def h(a):
-        return np.array(a.cpu() if hasattr(a, "cpu") else a)
-    return h(diag), dstar, h(x), h(upts)
+    return _host(diag), dstar, _host(x), _host(upts)
 
 
 def torch_dev():
