@@ -68,6 +68,55 @@ def test_stages_bitwise(where, colored, kind):
         assert np.array_equal(g, v), k
 
 
+def split_slots(fs, tj, seed, nsplit, nsame):
+    """The same system with nsplit faces split into two one-position slots
+    (each with a copy of the face's Jacobians, area and normal: boundary-like
+    slots) and nsame faces whose two positions carry the same side."""
+    from ucfd_sparse_amd.mesh import FlowSysMesh
+    rng = np.random.default_rng(seed)
+    slots, sides = fs.slots.copy(), fs.sides.copy()
+    order = np.argsort(slots, kind="stable")
+    pairs = order.reshape(-1, 2)            # every generated slot holds two positions
+    pick = rng.choice(len(pairs), size=nsplit + nsame, replace=False)
+    ns = fs.nfaces
+    new_f = []
+    for j, k in enumerate(pick[:nsplit]):
+        p, q = pairs[k]
+        new_f.append(slots[q])
+        slots[q] = ns + j
+    for k in pick[nsplit:]:
+        p, q = pairs[k]
+        sides[q] = sides[p]
+    new_f = np.array(new_f, np.int64)
+    jm = np.concatenate([fs.jmat, fs.jmat[..., new_f]], axis=-1)
+    tjm = np.concatenate([tj, tj[..., new_f]], axis=-1)
+    fa = np.concatenate([fs.face_area, fs.face_area[new_f]])
+    fn = np.concatenate([fs.face_normal, fs.face_normal[:, new_f]], axis=1)
+    return FlowSysMesh(fs.nlocal, ns + nsplit, fs.rowptr, fs.colidx, slots, sides, fa, fn,
+                       fs.rcp_vol, np.ascontiguousarray(jm)), np.ascontiguousarray(tjm)
+
+
+@pytest.mark.parametrize("colored", [False, True])
+def test_stages_bitwise_with_unpaired_slots(colored):
+    """Prepare takes a position's diagonal contribution from its partner's
+    packed block when the slot holds exactly two positions of opposite sides,
+    and from jmat otherwise: slots with one position and a pair with equal
+    sides exercise the second path."""
+    from ucfd_sparse_amd import flowsys as FS
+    say(f"unpaired and same-side slots, colour-ordered={colored}")
+    fs, tj, el = case(4, 73, colored, 6)
+    fs, tj = split_slots(fs, tj, 74, nsplit=40, nsame=3)
+    ref = O.flowsys_run(fs, tj, el, 6, 5, kind="sa", iterations=2)
+    fsd, tjd, eld = on_device(fs, tj, el)
+    S = FS.FlowSys(fsd, 6, 5, tjmat=tjd)
+    S.set_element(0, eld)
+    got = FS.run(S, kind="sa", iterations=2)
+    S.close()
+    for k, v in ref.items():
+        g = np.asarray(got[k]).reshape(v.shape)
+        assert np.array_equal(g, v), k
+
+
 @pytest.mark.parametrize("nvars,nfvars", [(7, 7), (9, 7)])
 def test_hex_wide_blocks_bitwise(nvars, nfvars):
     """7x7 blocks on a colour-ordered hex mesh: one wave's positions exceed the

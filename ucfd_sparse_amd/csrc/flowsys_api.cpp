@@ -34,7 +34,6 @@ struct FsElem {
 
 struct Packed {               // per-position blocks of one variable group
     double *jp = nullptr, *fv = nullptr;
-    double *dj = nullptr;     // Prepare's work array: per-position diagonal contributions
     float *jp32 = nullptr, *fv32 = nullptr;
     bool ready = false;
 };
@@ -56,6 +55,7 @@ struct _FlowSys {
     float *du32 = nullptr;
     int *rowptr_d = nullptr, *colidx_d = nullptr;
     int *pos_sorted_d = nullptr, *pos_cell_d = nullptr;   // positions by face slot; cell of each
+    int *pos_partner_d = nullptr;   // the other position of each position's face slot, or -1
     // scalar LU-SGS: rank arrays [nvars, nlocal] and the caller's per-slot
     // spectral radii (src/lusgs/flowsys.h:20-24)
     double *lu_u = nullptr, *lu_du = nullptr, *lu_diag = nullptr;
@@ -95,7 +95,6 @@ void free_packed(Packed &P)
 {
     dfree(&P.jp);
     dfree(&P.fv);
-    dfree(&P.dj);
     dfree(&P.jp32);
     dfree(&P.fv32);
     P.ready = false;
@@ -114,6 +113,7 @@ void free_device(FlowSys s)
     dfree(&s->colidx_d);
     dfree(&s->pos_sorted_d);
     dfree(&s->pos_cell_d);
+    dfree(&s->pos_partner_d);
     dfree(&s->lu_u);
     dfree(&s->lu_du);
     dfree(&s->lu_diag);
@@ -157,23 +157,36 @@ int build_structure(FlowSys s, const char *fn)
         (rc = hip(hipMemcpy(s->colidx_d, ci.data(), ci.size()*sizeof(int), hipMemcpyHostToDevice),
                   fn, "hipMemcpy(colidx)")))
         return rc;
-    // positions in ascending face slot (stable) and the cell of each position,
-    // for Prepare's coalesced pass over the face-major Jacobians
+    // positions in ascending face slot (stable), the cell of each position,
+    // for Prepare's coalesced pass over the face-major Jacobians, and each
+    // position's partner: the other position of a slot held by exactly two
     {
-        std::vector<int> cnt((size_t)s->nfaces + 1, 0), sorted((size_t)s->npos), cell((size_t)s->npos);
+        std::vector<int> cnt((size_t)s->nfaces + 1, 0), sorted((size_t)s->npos), cell((size_t)s->npos),
+            partner((size_t)s->npos, -1);
         for (int64_t p = 0; p < s->npos; ++p) cnt[(size_t)sl[p] + 1]++;
         for (int f = 0; f < s->nfaces; ++f) cnt[(size_t)f + 1] += cnt[f];
         for (int64_t p = 0; p < s->npos; ++p) sorted[(size_t)cnt[sl[p]]++] = (int)p;
+        // cnt[f] is now the end of slot f's range
+        for (int f = 0; f < s->nfaces; ++f) {
+            const int e = cnt[f], b = f ? cnt[f - 1] : 0;
+            if (e - b == 2) {
+                partner[(size_t)sorted[b]] = sorted[b + 1];
+                partner[(size_t)sorted[b + 1]] = sorted[b];
+            }
+        }
         for (int i = 0; i < n; ++i)
             for (int p = rp[i]; p < rp[i + 1]; ++p) cell[(size_t)p] = i;
         if ((rc = dalloc(&s->pos_sorted_d, (size_t)s->npos, fn)) ||
-            (rc = dalloc(&s->pos_cell_d, (size_t)s->npos, fn)))
+            (rc = dalloc(&s->pos_cell_d, (size_t)s->npos, fn)) ||
+            (rc = dalloc(&s->pos_partner_d, (size_t)s->npos, fn)))
             return rc;
         if (s->npos &&
             ((rc = hip(hipMemcpy(s->pos_sorted_d, sorted.data(), sorted.size()*sizeof(int),
                                  hipMemcpyHostToDevice), fn, "hipMemcpy(pos_sorted)")) ||
              (rc = hip(hipMemcpy(s->pos_cell_d, cell.data(), cell.size()*sizeof(int),
-                                 hipMemcpyHostToDevice), fn, "hipMemcpy(pos_cell)"))))
+                                 hipMemcpyHostToDevice), fn, "hipMemcpy(pos_cell)")) ||
+             (rc = hip(hipMemcpy(s->pos_partner_d, partner.data(), partner.size()*sizeof(int),
+                                 hipMemcpyHostToDevice), fn, "hipMemcpy(pos_partner)"))))
             return rc;
     }
     std::vector<int> seq(n), jb(n), je(n);
@@ -220,8 +233,7 @@ ucfd_status_t prepare(FlowSys s, bool turb, const char *fn)
     int rc = build_structure(s, fn);
     if (rc) return UCFD_FAILED;
     const size_t d2 = (size_t)nv*nv;
-    if ((rc = dalloc(&P.jp, (size_t)s->npos*d2, fn)) || (rc = dalloc(&P.fv, (size_t)s->npos, fn)) ||
-        (rc = dalloc(&P.dj, (size_t)s->npos*d2, fn)))
+    if ((rc = dalloc(&P.jp, (size_t)s->npos*d2, fn)) || (rc = dalloc(&P.fv, (size_t)s->npos, fn)))
         return UCFD_FAILED;
     if (s->fp32 && ((rc = dalloc(&P.jp32, (size_t)s->npos*d2, fn)) ||
                     (rc = dalloc(&P.fv32, (size_t)s->npos, fn))))
@@ -234,11 +246,11 @@ ucfd_status_t prepare(FlowSys s, bool turb, const char *fn)
     const int ijm = sg.add(jm, sizeof(double)*2*d2*(size_t)s->nfaces, UCFD_IN);
     if ((rc = sg.enter())) return UCFD_FAILED;
     rc = fs_launch_prepare(ucfd_stream(), nv, s->nlocal, s->nfaces, s->npos, s->rowptr_d,
-                           s->pos_sorted_d, s->pos_cell_d, (const int *)sg.dev(isl),
-                           (const signed char *)sg.dev(isd), (const double *)sg.dev(ifa),
-                           (const double *)sg.dev(irv), dg, (const double *)sg.dev(ijm), P.jp,
-                           P.dj, P.fv, s->fp32 ? P.jp32 : nullptr, s->fp32 ? P.fv32 : nullptr,
-                           ucfd_device_flag());
+                           s->pos_sorted_d, s->pos_cell_d, s->pos_partner_d,
+                           (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
+                           (const double *)sg.dev(ifa), (const double *)sg.dev(irv), dg,
+                           (const double *)sg.dev(ijm), P.jp, P.fv, s->fp32 ? P.jp32 : nullptr,
+                           s->fp32 ? P.fv32 : nullptr, ucfd_device_flag());
     // the factorisation is complete and checked for zero pivots on return
     if (!rc) rc = ucfd_sync_and_check(fn);
     rc = sg.leave(rc);

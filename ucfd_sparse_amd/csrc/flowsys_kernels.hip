@@ -131,9 +131,9 @@ __global__ __launch_bounds__(FB) void k_subres(int neles, int nlocal, int nvars,
 // fv = face_area*rcp_vol.  Two passes:
 // (1) Positions in face-slot order (sorted[k], built with the structure):
 // consecutive lanes read consecutive slots of the face-major jmat, so its
-// loads are coalesced; each lane forms its position's neighbour block j and
-// diagonal contribution v*fv, the wave transposes them through LDS and
-// writes whole blocks with consecutive lanes on consecutive words.
+// loads are coalesced; each lane forms its position's neighbour block j, the
+// wave transposes the blocks through LDS and writes whole blocks with
+// consecutive lanes on consecutive words.
 template <int NV>
 __global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
                                                  const int *__restrict__ sorted,
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
                                                  const double *__restrict__ face_area,
                                                  const double *__restrict__ rcp_vol,
                                                  const double *__restrict__ jmat, double *jp,
-                                                 double *dj, double *fvp, float *jp32, float *fv32)
+                                                 double *fvp, float *jp32, float *fv32)
 {
     constexpr int D2 = NV*NV;
     __shared__ double sb[64*D2];
@@ -187,31 +187,44 @@ __global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
         jp[o] = j;
         if (jp32) jp32[o] = (float)j;
     }
-    ucfd::wave_lds_sync();
-    // diagonal contribution v*fv
-#pragma unroll
-    for (int q = 0; q < D2; ++q) sb[lane*D2 + q] = (plus ? a[q] : -b[q])*fv;
-    ucfd::wave_lds_sync();
-    for (int e = lane; e < nact*D2; e += 64) {
-        const int blk = e/D2;
-        dj[(int64_t)sp[blk]*D2 + (e - blk*D2)] = sb[e];
-    }
 }
 
 // (2) Cells, one wave per G = 64/NV cells: lane (g, row) adds row `row` of
-// the cell's contributions to the packed diagonal in position order (the
-// wave's contributions staged through LDS when they fit), the group
-// exchanges the rows with lane shuffles, every lane factors the block, and
-// lane (g, row) writes row `row` of the LU.
+// each position's diagonal contribution v*fv to the packed diagonal in
+// position order, the group exchanges the rows with lane shuffles, every
+// lane factors the block, and lane (g, row) writes row `row` of the LU.
+// v = (side ? A+ : -A-) is the negated neighbour block that pass (1) packed
+// for the partner position, the other side of the same face slot (its block
+// is -A+ when this side is "+", A- otherwise): negation is exact, so v is
+// the reference's value bit for bit.  A position without a partner of the
+// other side (a boundary face) reads its half of jmat.  Each lane issues the
+// loads of up to FPM positions together.
+constexpr int FPM = 4;
+
 template <int NV>
-__global__ __launch_bounds__(64) void k_prep_diag(int nlocal, const int *__restrict__ rowptr,
-                                                  const double *__restrict__ dj, double *diag,
+__device__ __forceinline__ void prep_jmat_row(const double *__restrict__ jmat, int64_t nfaces,
+                                              int64_t s, bool plus, int row, double *v)
+{
+    const int64_t half = (int64_t)NV*NV*nfaces;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        const int64_t q = (int64_t)(row*NV + c)*nfaces + s;
+        v[c] = plus ? jmat[q] : -jmat[q + half];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(64) void k_prep_diag(int nlocal, int64_t nfaces,
+                                                  const int *__restrict__ rowptr,
+                                                  const int *__restrict__ partner,
+                                                  const int *__restrict__ slots,
+                                                  const signed char *__restrict__ sides,
+                                                  const double *__restrict__ jmat,
+                                                  const double *__restrict__ jp,
+                                                  const double *__restrict__ fvp, double *diag,
                                                   int *flag)
 {
     constexpr int D2 = NV*NV, G = 64/NV;
-    constexpr int CP = FSTAGE/(8*D2) < 8*G ? FSTAGE/(8*D2) : 8*G;
-    constexpr int UJ = (CP*D2 + 63)/64;
-    __shared__ double sd[CP*D2];
     const int lane = threadIdx.x;
     const int t0 = ucfd::wave_xcd_index()*G;
     if (t0 >= nlocal) return;
@@ -219,33 +232,42 @@ __global__ __launch_bounds__(64) void k_prep_diag(int nlocal, const int *__restr
     const int nrows = min(G, nlocal - t0);
     const int r = t0 + g;
     const bool act = g < nrows;
-    const int p0w = rowptr[t0], p1w = rowptr[t0 + nrows];
     const int st = act ? rowptr[r] : 0, ed = act ? rowptr[r + 1] : 0;
     double dm[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) dm[c] = act ? diag[(int64_t)r*D2 + row*NV + c] : 0.0;
-    if (p1w - p0w <= CP) {
-        const int nj = (p1w - p0w)*D2;
-        const double *src = dj + (int64_t)p0w*D2;
-        double t[UJ];
+    for (int p0 = st; p0 < ed; p0 += FPM) {
+        const int n = min(FPM, ed - p0);
+        int pp[FPM];
+        bool pl[FPM];
+        double fv[FPM];
 #pragma unroll
-        for (int u = 0; u < UJ; ++u) {
-            const int e = u*64 + lane;
-            t[u] = e < nj ? __builtin_nontemporal_load(src + e) : 0.0;
-        }
+        for (int k = 0; k < FPM; ++k)
+            if (k < n) {
+                pp[k] = partner[p0 + k];
+                pl[k] = sides[p0 + k] == 1;
+                fv[k] = fvp[p0 + k];
+            }
+        double v[FPM][NV];
+        bool other[FPM];
 #pragma unroll
-        for (int u = 0; u < UJ; ++u) {
-            const int e = u*64 + lane;
-            if (e < nj) sd[e] = t[u];
-        }
-        ucfd::wave_lds_sync();
-        for (int p = st; p < ed; ++p)
+        for (int k = 0; k < FPM; ++k)
+            if (k < n) {
+                other[k] = pp[k] >= 0;
+                if (other[k]) {
+                    other[k] = (sides[pp[k]] == 1) != pl[k];
 #pragma unroll
-            for (int c = 0; c < NV; ++c) dm[c] += sd[(p - p0w)*D2 + row*NV + c];
-    } else {
-        for (int p = st; p < ed; ++p)
+                    for (int c = 0; c < NV; ++c) v[k][c] = -jp[(int64_t)pp[k]*D2 + row*NV + c];
+                }
+            }
 #pragma unroll
-            for (int c = 0; c < NV; ++c) dm[c] += dj[(int64_t)p*D2 + row*NV + c];
+        for (int k = 0; k < FPM; ++k)
+            if (k < n) {
+                if (!other[k])
+                    prep_jmat_row<NV>(jmat, nfaces, slots[p0 + k], pl[k], row, v[k]);
+#pragma unroll
+                for (int c = 0; c < NV; ++c) dm[c] += v[k][c]*fv[k];
+            }
     }
     double m[D2];
 #pragma unroll
@@ -254,8 +276,18 @@ __global__ __launch_bounds__(64) void k_prep_diag(int nlocal, const int *__restr
         for (int c = 0; c < NV; ++c) m[rr*NV + c] = __shfl(dm[c], g*NV + rr, 64);
     if (!act) return;
     if (!ucfd::ludcmp<NV>(m) && row == 0) atomicOr(flag, UCFD_KFLAG_PIVOT);
+    // this lane's row, selected with constant indices (an index that depends
+    // on the lane would place m in scratch memory)
+    double o[NV];
 #pragma unroll
-    for (int c = 0; c < NV; ++c) diag[(int64_t)r*D2 + row*NV + c] = m[row*NV + c];
+    for (int c = 0; c < NV; ++c) o[c] = m[c];
+#pragma unroll
+    for (int rr = 1; rr < NV; ++rr)
+        if (row == rr)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) o[c] = m[rr*NV + c];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) diag[(int64_t)r*D2 + row*NV + c] = o[c];
 }
 
 // block Gauss-Seidel row update over every position (blusgs.c:273-373)
@@ -618,18 +650,19 @@ __global__ __launch_bounds__(FB) void k_zero(int64_t n, double *a, double *b, fl
 
 template <int NV> struct PrepL {
     static int run(hipStream_t s, int nlocal, int64_t nfaces, int64_t npos, const int *rowptr,
-                   const int *sorted, const int *pos_cell, const int *slots,
+                   const int *sorted, const int *pos_cell, const int *partner, const int *slots,
                    const signed char *sides, const double *face_area, const double *rcp_vol,
-                   double *diag, const double *jmat, double *jp, double *dj, double *fvp,
-                   float *jp32, float *fv32, int *flag)
+                   double *diag, const double *jmat, double *jp, double *fvp, float *jp32,
+                   float *fv32, int *flag)
     {
         if (npos > 0)
             hipLaunchKernelGGL(k_prep_pos<NV>, dim3((unsigned)((npos + 63)/64)), dim3(64), 0, s,
                                npos, nfaces, sorted, pos_cell, slots, sides, face_area, rcp_vol,
-                               jmat, jp, dj, fvp, jp32, fv32);
+                               jmat, jp, fvp, jp32, fv32);
         constexpr int G = 64/NV;
         hipLaunchKernelGGL(k_prep_diag<NV>, dim3((unsigned)((nlocal + G - 1)/G)), dim3(64), 0, s,
-                           nlocal, rowptr, dj, diag, flag);
+                           nlocal, nfaces, rowptr, partner, slots, sides, jmat, jp, fvp, diag,
+                           flag);
         return ok();
     }
 };
@@ -718,14 +751,15 @@ int fs_launch_subres(hipStream_t s, int neles, int nlocal, int nvars, const int 
 }
 
 int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, int64_t npos,
-                      const int *rowptr, const int *sorted, const int *pos_cell, const int *slots,
-                      const signed char *sides, const double *face_area, const double *rcp_vol,
-                      double *diag, const double *jmat, double *jp, double *dj, double *fvp,
-                      float *jp32, float *fv32, int *flag)
+                      const int *rowptr, const int *sorted, const int *pos_cell,
+                      const int *partner, const int *slots, const signed char *sides,
+                      const double *face_area, const double *rcp_vol, double *diag,
+                      const double *jmat, double *jp, double *fvp, float *jp32, float *fv32,
+                      int *flag)
 {
     if (nlocal <= 0) return UCFD_OK;
-    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, sorted, pos_cell, slots, sides,
-                        face_area, rcp_vol, diag, jmat, jp, dj, fvp, jp32, fv32, flag);
+    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, sorted, pos_cell, partner, slots,
+                        sides, face_area, rcp_vol, diag, jmat, jp, fvp, jp32, fv32, flag);
 }
 
 int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int cnt, const int *order,
