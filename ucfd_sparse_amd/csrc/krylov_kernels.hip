@@ -78,11 +78,16 @@ __global__ __launch_bounds__(KB) void k_spmv_bsr(int bn, const int *__restrict__
 
 // Block sizes 2..7: one wave per G = 64/B block rows (wave_rows.h).  The
 // wave's blocks [rowptr[i0], rowptr[i0+G]) are contiguous; they are staged
-// through LDS in chunks of CB blocks with all of a chunk's loads in flight,
-// the column indices and x rows of each lane's first KP blocks are fetched
-// before the staging loads, and lane (g, r) sums scalar row r of block row g
-// in the reference order (per block: v = sum_q A[r][q] x[q], then arr += v).
-constexpr int KSTAGE = 1600;    // staged doubles per wave (12.8 KB of LDS)
+// through LDS in chunks of CB = KSPMV/B^2 blocks with all of a chunk's loads
+// in flight, the column indices and x rows of each lane's first KP blocks
+// are fetched before the staging loads, and lane (g, r) sums scalar row r of
+// block row g in the reference order (per block: v = sum_q A[r][q] x[q],
+// then arr += v).
+constexpr int KSTAGE = 1600;    // staged doubles per wave of the block sweeps (12.8 KB of LDS)
+// SpMV chunks: 6.4 KB of LDS per wave leaves room for 4 waves per SIMD (the
+// register limit), against 3 with 12.8 KB chunks; measured 5 % faster on
+// 5x5 blocks
+constexpr int KSPMV = 800;
 
 template <int B>
 __global__ __launch_bounds__(64) void k_spmv_bsr_w(int bn, const int *__restrict__ rowptr,
@@ -91,7 +96,7 @@ __global__ __launch_bounds__(64) void k_spmv_bsr_w(int bn, const int *__restrict
                                                    double alpha, const double *__restrict__ x,
                                                    double beta, double *__restrict__ y, int *flag)
 {
-    constexpr int B2 = B*B, G = 64/B, CB = KSTAGE/B2, U = (CB*B2 + 63)/64, KP = 5;
+    constexpr int B2 = B*B, G = 64/B, CB = KSPMV/B2, U = (CB*B2 + 63)/64, KP = 5;
     __shared__ double sv[CB*B2];
     const int lane = threadIdx.x;
     const int i0 = ucfd::wave_xcd_index()*G;
