@@ -56,6 +56,10 @@ struct _FlowSys {
     int *rowptr_d = nullptr, *colidx_d = nullptr;
     int *pos_sorted_d = nullptr, *pos_cell_d = nullptr;   // positions by face slot; cell of each
     int *pos_partner_d = nullptr;   // the other position of each position's face slot, or -1
+    std::vector<int> rowptr_h;      // host copy of the structure's row pointer
+    // per block size nv: the most positions 64/nv consecutive rows of each
+    // level hold (lower, upper schedules), computed at the first sweep
+    std::vector<int> wave_pos[2][8];
     // scalar LU-SGS: rank arrays [nvars, nlocal] and the caller's per-slot
     // spectral radii (src/lusgs/flowsys.h:20-24)
     double *lu_u = nullptr, *lu_du = nullptr, *lu_diag = nullptr;
@@ -142,6 +146,7 @@ int build_structure(FlowSys s, const char *fn)
     for (int i = 0; i < n; ++i)
         if (rp[i + 1] < rp[i]) return ucfd_report(UCFD_ERR_INDEX, fn, "rowptr must be nondecreasing");
     s->npos = rp[n];
+    s->rowptr_h = rp;
     if ((rc = ucfd_fetch_ints(s->colidx, (size_t)s->npos, ci, fn))) return rc;
     if ((rc = ucfd_fetch_ints(s->slots, (size_t)s->npos, sl, fn))) return rc;
     for (int64_t p = 0; p < s->npos; ++p) {
@@ -270,11 +275,30 @@ ucfd_status_t sweep(FlowSys s, bool turb, bool upper, const char *fn)
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     const UcfdSchedule &S = upper ? s->upper : s->lower;
-    for (const UcfdLevel &l : S.lv)
+    std::vector<int> &wp = s->wave_pos[upper ? 1 : 0][nv];
+    if (wp.size() != S.lv.size()) {
+        // staging sizes of the wave kernels: 64/nv consecutive rows per wave
+        const int G = 64/nv;
+        const std::vector<int> &rp = s->rowptr_h;
+        wp.assign(S.lv.size(), -1);
+        for (size_t k = 0; k < S.lv.size(); ++k) {
+            const UcfdLevel &l = S.lv[k];
+            if (l.ordered) continue;
+            int mx = 0;
+            for (int w0 = 0; w0 < l.cnt; w0 += G) {
+                const int a = l.base + w0, b = l.base + std::min(w0 + G, l.cnt);
+                mx = std::max(mx, rp[b] - rp[a]);
+            }
+            wp[k] = mx;
+        }
+    }
+    for (size_t k = 0; k < S.lv.size(); ++k) {
+        const UcfdLevel &l = S.lv[k];
         if ((rc = fs_launch_sweep(ucfd_stream(), nv, f32, l.base, l.cnt, l.ordered ? S.order : nullptr,
                                   var0, s->nvars, s->rowptr_d, s->colidx_d, P.jp, P.fv, P.jp32,
-                                  P.fv32, s->rhs, dg, s->du, f32 ? s->du32 : nullptr)))
+                                  P.fv32, s->rhs, dg, s->du, f32 ? s->du32 : nullptr, wp[k])))
             break;
+    }
     return status(sg.leave(rc));
 }
 

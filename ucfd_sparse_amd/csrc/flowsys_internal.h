@@ -36,12 +36,13 @@ UCFD_HIDDEN int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfa
 // One level of a block Gauss-Seidel sweep: rows base..base+cnt-1 (or
 // order[base..base+cnt)), variables var0..var0+nv-1 of the [nlocal, nvars]
 // arrays.  f32: FP32 products on jp32 / fv32 / du32, FP64 diagonal solve,
-// du and du32 both written.
+// du and du32 both written.  maxw: the most positions that 64/nv consecutive
+// rows of the level hold (-1 = unknown); sizes the LDS of a contiguous level.
 UCFD_HIDDEN int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int cnt,
                                 const int *order, int var0, int nvars, const int *rowptr,
                                 const int *colidx, const double *jp, const double *fvp,
                                 const float *jp32, const float *fv32, const double *rhs,
-                                const double *diag, double *du, float *du32);
+                                const double *diag, double *du, float *du32, int maxw);
 // a = b = 0 (and c = 0 when not NULL), n entries
 UCFD_HIDDEN int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c);
 

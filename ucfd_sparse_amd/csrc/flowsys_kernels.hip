@@ -361,9 +361,11 @@ __global__ __launch_bounds__(FB) void k_sweep(int base, int cnt, const int *__re
 // exchanges its NV values with lane shuffles and every lane runs the cell's
 // diagonal solve.  The wave's packed blocks, fv values and diagonal LU
 // blocks are contiguous (positions of consecutive cells are consecutive) and
-// are staged through LDS with coalesced loads when they fit FSTAGE bytes;
-// otherwise the lanes read them from HBM directly.  Results equal k_sweep's
-// bit for bit.
+// are staged through LDS with coalesced loads when they fit the launch's cap
+// positions (dynamic LDS, sized from the level's largest wave so that LDS
+// does not limit the waves per SIMD more than needed; at most FSTAGE bytes of
+// blocks); otherwise the lanes read them from HBM directly.  Results equal
+// k_sweep's bit for bit.
 
 template <int NV, bool F32, bool STAGED>
 __device__ __forceinline__ void fs_rows(int row, int g, bool act, int r, int st, int ed, int p0w,
@@ -421,8 +423,14 @@ __device__ __forceinline__ void fs_rows(int row, int g, bool act, int r, int st,
     if (F32) du32[o] = (float)xv[row];
 }
 
+// staged positions per wave: FSTAGE bytes of blocks, at most 8 per cell
+template <int NV, typename T> constexpr int fs_stage_max()
+{
+    return FSTAGE/((int)sizeof(T)*NV*NV) < 8*(64/NV) ? FSTAGE/((int)sizeof(T)*NV*NV) : 8*(64/NV);
+}
+
 template <int NV, bool F32>
-__global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int var0, int nvars,
+__global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int cap, int var0, int nvars,
                                                 const int *__restrict__ rowptr,
                                                 const int *__restrict__ colidx,
                                                 const typename std::conditional<F32, float,
@@ -437,12 +445,13 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int var0, int
 {
     using T = typename std::conditional<F32, float, double>::type;
     constexpr int D2 = NV*NV, G = 64/NV;
-    // staged positions per wave: FSTAGE bytes of blocks, at most 8 per cell
-    constexpr int CP = FSTAGE/((int)sizeof(T)*D2) < 8*G ? FSTAGE/((int)sizeof(T)*D2) : 8*G;
+    constexpr int CP = fs_stage_max<NV, T>();
     constexpr int UJ = (CP*D2 + 63)/64, UF = (CP + 63)/64, NLU = (G*D2 + 63)/64;
-    __shared__ T sj[CP*D2];
-    __shared__ T sf[CP];
-    __shared__ double slu[G*D2];
+    // dynamic LDS: the rows' LU blocks, then cap blocks, then cap fv values
+    extern __shared__ __align__(16) unsigned char fdyn[];
+    double *slu = (double *)fdyn;
+    T *sj = (T *)(fdyn + G*D2*sizeof(double));
+    T *sf = sj + (size_t)cap*D2;
     const int lane = threadIdx.x;
     const int t0 = ucfd::wave_xcd_index()*G;
     if (t0 >= cnt) return;
@@ -452,7 +461,7 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int var0, int
     const bool act = g < nrows;
     const int p0w = rowptr[r0], p1w = rowptr[r0 + nrows];
     const int st = act ? rowptr[r] : 0, ed = act ? rowptr[r + 1] : 0;
-    if (p1w - p0w > CP) {
+    if (p1w - p0w > cap) {
         fs_rows<NV, F32, false>(row, g, act, r, st, ed, 0, var0, nvars, colidx, jp, fvp,
                                 diag + (int64_t)r*D2, rhs, du, du32);
         return;
@@ -667,20 +676,32 @@ template <int NV> struct PrepL {
     }
 };
 
+template <int NV, typename T> size_t sweep_lds(int cap)
+{
+    return (size_t)(64/NV)*NV*NV*sizeof(double) + (size_t)cap*(NV*NV + 1)*sizeof(T);
+}
+
 template <int NV> struct SweepL {
     static int run(hipStream_t s, bool f32, int base, int cnt, const int *order, int var0,
                    int nvars, const int *rowptr, const int *colidx, const double *jp,
                    const double *fvp, const float *jp32, const float *fv32, const double *rhs,
-                   const double *diag, double *du, float *du32)
+                   const double *diag, double *du, float *du32, int maxw)
     {
         if (!order) {
             const dim3 g((unsigned)((cnt + 64/NV - 1)/(64/NV)));
-            if (f32)
-                hipLaunchKernelGGL((k_sweep_w<NV, true>), g, dim3(64), 0, s, base, cnt, var0, nvars,
-                                   rowptr, colidx, jp32, fv32, rhs, diag, du, du32);
-            else
-                hipLaunchKernelGGL((k_sweep_w<NV, false>), g, dim3(64), 0, s, base, cnt, var0,
-                                   nvars, rowptr, colidx, jp, fvp, rhs, diag, du, du32);
+            if (f32) {
+                constexpr int CP = fs_stage_max<NV, float>();
+                const int cap = maxw < 0 || maxw > CP ? CP : maxw;
+                const size_t lds = sweep_lds<NV, float>(cap);
+                hipLaunchKernelGGL((k_sweep_w<NV, true>), g, dim3(64), lds, s, base, cnt, cap, var0,
+                                   nvars, rowptr, colidx, jp32, fv32, rhs, diag, du, du32);
+            } else {
+                constexpr int CP = fs_stage_max<NV, double>();
+                const int cap = maxw < 0 || maxw > CP ? CP : maxw;
+                const size_t lds = sweep_lds<NV, double>(cap);
+                hipLaunchKernelGGL((k_sweep_w<NV, false>), g, dim3(64), lds, s, base, cnt, cap,
+                                   var0, nvars, rowptr, colidx, jp, fvp, rhs, diag, du, du32);
+            }
             return ok();
         }
         if (f32)
@@ -765,11 +786,11 @@ int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, int64_t
 int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int cnt, const int *order,
                     int var0, int nvars, const int *rowptr, const int *colidx, const double *jp,
                     const double *fvp, const float *jp32, const float *fv32, const double *rhs,
-                    const double *diag, double *du, float *du32)
+                    const double *diag, double *du, float *du32, int maxw)
 {
     if (cnt <= 0) return UCFD_OK;
     return by_nv<SweepL>(nv, s, f32, base, cnt, order, var0, nvars, rowptr, colidx, jp, fvp, jp32,
-                         fv32, rhs, diag, du, du32);
+                         fv32, rhs, diag, du, du32, maxw);
 }
 
 int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c)
