@@ -359,13 +359,14 @@ __global__ __launch_bounds__(FB) void k_sweep(int base, int cnt, const int *__re
 // lane (g, row) = (lane / NV, lane % NV) accumulates component `row` of
 // cell g over every position in the order of k_sweep above; the group
 // exchanges its NV values with lane shuffles and every lane runs the cell's
-// diagonal solve.  The wave's packed blocks, fv values and diagonal LU
-// blocks are contiguous (positions of consecutive cells are consecutive) and
-// are staged through LDS with coalesced loads when they fit the launch's cap
-// positions (dynamic LDS, sized from the level's largest wave so that LDS
-// does not limit the waves per SIMD more than needed; at most FSTAGE bytes of
-// blocks); otherwise the lanes read them from HBM directly.  Results equal
-// k_sweep's bit for bit.
+// diagonal solve.  The wave's packed blocks and fv values are contiguous
+// (positions of consecutive cells are consecutive) and are staged through
+// LDS with coalesced loads when they fit the launch's cap positions (dynamic
+// LDS, sized from the level's largest wave so that LDS does not limit the
+// waves per SIMD more than needed; at most FSTAGE bytes of blocks);
+// otherwise the lanes read them from HBM directly.  The LU blocks are read
+// from HBM by the row's lanes (staging them would cost LDS, and one wave
+// per SIMD).  Results equal k_sweep's bit for bit.
 
 template <int NV, bool F32, bool STAGED>
 __device__ __forceinline__ void fs_rows(int row, int g, bool act, int r, int st, int ed, int p0w,
@@ -446,11 +447,11 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int cap, int 
     using T = typename std::conditional<F32, float, double>::type;
     constexpr int D2 = NV*NV, G = 64/NV;
     constexpr int CP = fs_stage_max<NV, T>();
-    constexpr int UJ = (CP*D2 + 63)/64, UF = (CP + 63)/64, NLU = (G*D2 + 63)/64;
-    // dynamic LDS: the rows' LU blocks, then cap blocks, then cap fv values
+    constexpr int UJ = (CP*D2 + 63)/64, UF = (CP + 63)/64;
+    // dynamic LDS: cap blocks, then cap fv values (the rows' LU blocks are
+    // read from HBM by each row's lanes)
     extern __shared__ __align__(16) unsigned char fdyn[];
-    double *slu = (double *)fdyn;
-    T *sj = (T *)(fdyn + G*D2*sizeof(double));
+    T *sj = (T *)fdyn;
     T *sf = sj + (size_t)cap*D2;
     const int lane = threadIdx.x;
     const int t0 = ucfd::wave_xcd_index()*G;
@@ -466,12 +467,10 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int cap, int 
                                 diag + (int64_t)r*D2, rhs, du, du32);
         return;
     }
-    const int nj = (p1w - p0w)*D2, nf = p1w - p0w, nl = nrows*D2;
+    const int nj = (p1w - p0w)*D2, nf = p1w - p0w;
     const T *js = jp + (int64_t)p0w*D2;
     const T *fs = fvp + p0w;
-    const double *ls = diag + (int64_t)r0*D2;
     T tj[UJ], tf[UF];
-    double tl[NLU];
 #pragma unroll
     for (int u = 0; u < UJ; ++u) {
         const int e = u*64 + lane;
@@ -483,11 +482,6 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int cap, int 
         tf[u] = e < nf ? __builtin_nontemporal_load(fs + e) : T(0);
     }
 #pragma unroll
-    for (int u = 0; u < NLU; ++u) {
-        const int e = u*64 + lane;
-        tl[u] = e < nl ? __builtin_nontemporal_load(ls + e) : 0.0;
-    }
-#pragma unroll
     for (int u = 0; u < UJ; ++u) {
         const int e = u*64 + lane;
         if (e < nj) sj[e] = tj[u];
@@ -497,14 +491,9 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int cap, int 
         const int e = u*64 + lane;
         if (e < nf) sf[e] = tf[u];
     }
-#pragma unroll
-    for (int u = 0; u < NLU; ++u) {
-        const int e = u*64 + lane;
-        if (e < nl) slu[e] = tl[u];
-    }
     ucfd::wave_lds_sync();
     fs_rows<NV, F32, true>(row, g, act, r, st, ed, p0w, var0, nvars, colidx, sj, sf,
-                           slu + (act ? g*D2 : 0), rhs, du, du32);
+                           diag + (int64_t)(act ? r : r0)*D2, rhs, du, du32);
 }
 
 // ------------------------------------------------------ scalar LU-SGS
@@ -678,7 +667,7 @@ template <int NV> struct PrepL {
 
 template <int NV, typename T> size_t sweep_lds(int cap)
 {
-    return (size_t)(64/NV)*NV*NV*sizeof(double) + (size_t)cap*(NV*NV + 1)*sizeof(T);
+    return (size_t)cap*(NV*NV + 1)*sizeof(T);
 }
 
 template <int NV> struct SweepL {
