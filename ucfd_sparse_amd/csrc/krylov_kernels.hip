@@ -436,10 +436,12 @@ __global__ __launch_bounds__(KB) void k_sweep32(int base, int cnt, const int *__
 // of k_sweep64 / k_sweep32 above; the group then exchanges its B sums with
 // lane shuffles and every lane runs the row's diagonal solve (bit-identical
 // to the thread-per-row kernels).  K selects the FP64 sweep kind (SW_LUSGS
-// or SW_BILU; the mixed form is LU-SGS only).  When the wave's blocks fit the cap words
-// of LDS the launch provides (sized from the level's largest wave) they are
-// staged through LDS (per row, contiguous loads); otherwise the lanes read
-// them from HBM directly.  The rows' LU blocks are always staged.
+// or SW_BILU; the mixed form is LU-SGS only).  When the wave's blocks fit
+// the cap words of LDS the launch provides (sized from the level's largest
+// wave) they are staged through LDS (per row, contiguous loads); otherwise
+// the lanes read them from HBM directly.  The rows' LU blocks are read from
+// HBM by the row's lanes (staging them costs LDS, which limits the waves per
+// SIMD).
 template <int B, bool UPPER, bool F32, bool STAGED, int K>
 __device__ __forceinline__ void bsgs_rows(int r, int g, bool act, int i, int j0, int n, int off,
                                           const int *__restrict__ colidx,
@@ -534,11 +536,10 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
 {
     using T = typename std::conditional<F32, float, double>::type;
     constexpr int B2 = B*B, G = 64/B, PER = (5*B2 + 63)/64;
-    constexpr int NLU = (G*B2 + 63)/64;
-    // dynamic LDS: the rows' LU blocks, then cap staged block words
+    // dynamic LDS: cap staged block words (the rows' LU blocks are read from
+    // HBM by the row's lanes)
     extern __shared__ __align__(16) unsigned char kdyn[];
-    double *slu = (double *)kdyn;
-    T *sv = (T *)(kdyn + G*B2*sizeof(double));
+    T *sv = (T *)kdyn;
     const int lane = threadIdx.x;
     const int t0 = ucfd::wave_xcd_index()*G;
     if (t0 >= cnt) return;
@@ -556,13 +557,12 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
     const int ex = ucfd::wave_excl_scan(lead);
     const int total = __shfl(ex + lead, 63, 64);
     const int off = __shfl(ex, g*B < 64 ? g*B : 0, 64);
-    const int nrows = min(G, cnt - t0);
     if (total*B2 > cap) {
         bsgs_rows<B, UPPER, F32, false, K>(r, g, act, i, j0, n, 0, colidx, vals,
                                         dv + (int64_t)i*B2, x32, b);
         return;
     }
-    // stage: every row's blocks [j0, j0+n) to sv[off*B2 ...], and the rows' LU
+    // stage: every row's blocks [j0, j0+n) to sv[off*B2 ...]
     T t[G][PER];
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
@@ -572,15 +572,6 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
             const int e = k*64 + lane;
             t[gg][k] = e < ng ? __builtin_nontemporal_load(vals + (int64_t)jg*B2 + e) : T(0);
         }
-    }
-    // the block ILU lower solve has a unit diagonal: no LU blocks
-    constexpr bool LU = !(K == SW_BILU && !UPPER);
-    double tl[NLU];
-    const double *lsrc = dv + (int64_t)(base + t0)*B2;
-#pragma unroll
-    for (int k = 0; k < NLU; ++k) {
-        const int e = k*64 + lane;
-        tl[k] = LU && e < nrows*B2 ? __builtin_nontemporal_load(lsrc + e) : 0.0;
     }
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
@@ -593,13 +584,9 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
         }
         for (int e = PER*64 + lane; e < ng; e += 64) sv[og + e] = vals[(int64_t)jg*B2 + e];
     }
-#pragma unroll
-    for (int k = 0; k < NLU; ++k) {
-        const int e = k*64 + lane;
-        if (LU && e < nrows*B2) slu[e] = tl[k];
-    }
     ucfd::wave_lds_sync();
-    bsgs_rows<B, UPPER, F32, true, K>(r, g, act, i, j0, n, off, colidx, sv, slu + g*B2, x32, b);
+    bsgs_rows<B, UPPER, F32, true, K>(r, g, act, i, j0, n, off, colidx, sv,
+                                      dv + (int64_t)(act ? i : base + t0)*B2, x32, b);
 }
 
 // --------------------------------------------------------------- BLAS
@@ -844,7 +831,7 @@ template <int B, typename T> int stage_cap(int maxw)
 
 template <int B, typename T> size_t stage_bytes(int cap)
 {
-    return (size_t)(64/B)*B*B*sizeof(double) + (size_t)cap*sizeof(T);
+    return (size_t)(cap > 0 ? cap : 1)*sizeof(T);
 }
 
 template <int B, int K>
