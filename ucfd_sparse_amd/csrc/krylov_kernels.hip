@@ -7,8 +7,9 @@
 // (src/krylov/matrix/sparsemat.c:57-101, src/krylov/utils/inverse.c:7-107,
 // src/krylov/precon/blusgs.c:28-100, ilu.c:4-86, bilu.c:5-130, pbilu.c:5-180)
 // and are compiled without FMA contraction, so SpMV, the factorisations and
-// the FP64 preconditioner applies agree bit for bit with the CPU oracle.  Reductions sum block partials in a fixed order, so
-// every dot product is reproducible run to run.
+// the FP64 preconditioner applies agree bit for bit with the CPU oracle.
+// Reductions sum block partials in a fixed order, so every dot product is
+// reproducible run to run.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
