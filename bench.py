@@ -561,7 +561,7 @@ def blusgs_workload(args, log):
         "krylov_precon_apply": applies,
         "krylov_precon_apply_fp32_rel_err": apply_err,
         "roofline": {
-            "kernel": "FlowSys NS block sweep (k_sweep<5,false>), one launch per colour",
+            "kernel": "FlowSys NS block sweep (k_sweep_w<5,false>), one launch per colour",
             "bound": "hbm", "achieved": lb / (launch_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": lb / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": None, "algorithmic_bytes_per_launch": lb, "avg_launch_ms": launch_ms},
