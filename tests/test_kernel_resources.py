@@ -33,7 +33,9 @@ def kernel_metadata():
     out = []
     with tempfile.TemporaryDirectory() as d:
         fat = os.path.join(d, "fatbin")
-        subprocess.run(["objcopy", "--dump-section", f".hip_fatbin={fat}", LIB], check=True)
+        # an explicit output file: without one objcopy rewrites its input
+        subprocess.run(["objcopy", "--dump-section", f".hip_fatbin={fat}", LIB,
+                        os.path.join(d, "copy.so")], check=True)
         data = open(fat, "rb").read()
         starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
         for k, s in enumerate(starts):
