@@ -566,7 +566,75 @@ __global__ __launch_bounds__(FB) void k_lprep(int nlocal, int nv0, int nv1, doub
 // base..base+cnt-1 (or order[...]), in place on du.  NS rows (NT = 0) with
 // ns_flux_container, turbulence rows (NT turbulence variables) with
 // rans_flux_container (flux.c:77-90); the flux difference of diff_flux
-// (lusgs.c:143-155) in the reference's order.
+// (lusgs.c:143-155) in the reference's order.  The indices of a cell's first
+// LKP positions are read together, then every gather those positions need,
+// then the face terms are added in position order (one memory round trip for
+// them instead of two per position); further positions follow one by one.
+constexpr int LKP = 4;
+
+template <int ND, int NT>
+struct LFace {
+    static constexpr int NF = ND + 2, DNV = NT > 0 ? NT : ND + 2;
+    double nf[ND], fv, lam, uf[NF], dnb[DNV], ut[NT > 0 ? NT : 1];
+};
+
+template <int ND, int NT>
+__device__ __forceinline__ void lface_load(LFace<ND, NT> &F, int64_t nb, int64_t sl, double side,
+                                           double rv, double kappa, int nfvars, int nlocal,
+                                           int64_t nfaces, const double *__restrict__ lambdaf,
+                                           const double *__restrict__ face_area,
+                                           const double *__restrict__ face_normal,
+                                           const double *__restrict__ u, const double *du)
+{
+    constexpr int NF = LFace<ND, NT>::NF, DNV = LFace<ND, NT>::DNV;
+    const int nv0 = NT > 0 ? nfvars : 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) F.nf[d] = side*face_normal[sl + (int64_t)d*nfaces];
+    F.fv = face_area[sl]*rv;
+    F.lam = kappa*lambdaf[sl];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) F.uf[k] = u[nb + (int64_t)k*nlocal];
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) F.dnb[k] = du[nb + (int64_t)(k + nv0)*nlocal];
+    if (NT > 0) {
+#pragma unroll
+        for (int k = 0; k < DNV; ++k) F.ut[k] = u[nb + (int64_t)(nfvars + k)*nlocal];
+    }
+}
+
+template <int ND, int NT>
+__device__ __forceinline__ void lface_add(const LFace<ND, NT> &F, double *df)
+{
+    constexpr int NF = LFace<ND, NT>::NF, DNV = LFace<ND, NT>::DNV;
+    double dfj[DNV];
+    if (NT == 0) {
+        double uf[NF], dnb[DNV], nf[ND];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) uf[k] = F.uf[k];
+#pragma unroll
+        for (int k = 0; k < DNV; ++k) dnb[k] = F.dnb[k];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) nf[d] = F.nf[d];
+        ucfd::diff_flux<double, ND>(uf, dnb, nf, dfj);
+    } else {
+        // flow rows of du are zero: du + u = u for them
+        double c0 = 0.0, c1 = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) c0 += F.uf[d + 1]*F.nf[d];
+        c0 /= F.uf[0];
+        double up[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) up[k] = 0.0 + F.uf[k];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) c1 += up[d + 1]*F.nf[d];
+        c1 /= up[0];
+#pragma unroll
+        for (int k = 0; k < DNV; ++k) dfj[k] = (F.dnb[k] + F.ut[k])*c1 - F.ut[k]*c0;
+    }
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) df[k] += (dfj[k] - F.lam*F.dnb[k])*F.fv;
+}
+
 template <int ND, int NT, bool UPPER>
 __global__ __launch_bounds__(FB) void k_lsweep(int base, int cnt, const int *__restrict__ order,
                                                int nvars, int nfvars, double kappa,
@@ -581,52 +649,45 @@ __global__ __launch_bounds__(FB) void k_lsweep(int base, int cnt, const int *__r
                                                const double *__restrict__ diag,
                                                const double *__restrict__ u, double *du)
 {
-    constexpr int NF = ND + 2;                 // flow variables
-    constexpr int DNV = NT > 0 ? NT : NF;      // rows this sweep updates
+    constexpr int DNV = LFace<ND, NT>::DNV;    // rows this sweep updates
     const int t = fxcd()*FB + threadIdx.x;
     if (t >= cnt) return;
     const int r = order ? order[base + t] : base + t;
     const int nv0 = NT > 0 ? nfvars : 0;
     const double rv = rcp_vol[r];
+    const int p0 = rowptr[r], p1 = rowptr[r + 1];
     double df[DNV];
 #pragma unroll
     for (int k = 0; k < DNV; ++k) df[k] = 0.0;
-    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+    int64_t nbk[LKP], slk[LKP];
+    double sdk[LKP];
+    bool tk[LKP];
+#pragma unroll
+    for (int k = 0; k < LKP; ++k) {
+        tk[k] = false;
+        if (p0 + k < p1) {
+            nbk[k] = colidx[p0 + k];
+            slk[k] = slots[p0 + k];
+            sdk[k] = (double)sides[p0 + k];
+            tk[k] = UPPER ? nbk[k] > r : nbk[k] < r;
+        }
+    }
+    LFace<ND, NT> F[LKP];
+#pragma unroll
+    for (int k = 0; k < LKP; ++k)
+        if (tk[k])
+            lface_load<ND, NT>(F[k], nbk[k], slk[k], sdk[k], rv, kappa, nfvars, nlocal, nfaces,
+                               lambdaf, face_area, face_normal, u, du);
+#pragma unroll
+    for (int k = 0; k < LKP; ++k)
+        if (tk[k]) lface_add<ND, NT>(F[k], df);
+    for (int p = p0 + LKP; p < p1; ++p) {
         const int64_t nb = colidx[p];
         if (UPPER ? !(nb > r) : !(nb < r)) continue;
-        const int64_t sl = slots[p];
-        const double side = (double)sides[p];
-        double nf[ND];
-#pragma unroll
-        for (int d = 0; d < ND; ++d) nf[d] = side*face_normal[sl + (int64_t)d*nfaces];
-        const double fv = face_area[sl]*rv;
-        const double lam = kappa*lambdaf[sl];
-        double uf[NF], dnb[DNV], dfj[DNV];
-#pragma unroll
-        for (int k = 0; k < NF; ++k) uf[k] = u[nb + (int64_t)k*nlocal];
-#pragma unroll
-        for (int k = 0; k < DNV; ++k) dnb[k] = du[nb + (int64_t)(k + nv0)*nlocal];
-        if (NT == 0) {
-            ucfd::diff_flux<double, ND>(uf, dnb, nf, dfj);
-        } else {
-            // flow rows of du are zero: du + u = u for them
-            double ut[DNV], c0 = 0.0, c1 = 0.0;
-#pragma unroll
-            for (int k = 0; k < DNV; ++k) ut[k] = u[nb + (int64_t)(nfvars + k)*nlocal];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) c0 += uf[d + 1]*nf[d];
-            c0 /= uf[0];
-            double up[NF];
-#pragma unroll
-            for (int k = 0; k < NF; ++k) up[k] = 0.0 + uf[k];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) c1 += up[d + 1]*nf[d];
-            c1 /= up[0];
-#pragma unroll
-            for (int k = 0; k < DNV; ++k) dfj[k] = (dnb[k] + ut[k])*c1 - ut[k]*c0;
-        }
-#pragma unroll
-        for (int k = 0; k < DNV; ++k) df[k] += (dfj[k] - lam*dnb[k])*fv;
+        LFace<ND, NT> G;
+        lface_load<ND, NT>(G, nb, slots[p], (double)sides[p], rv, kappa, nfvars, nlocal, nfaces,
+                           lambdaf, face_area, face_normal, u, du);
+        lface_add<ND, NT>(G, df);
     }
 #pragma unroll
     for (int k = 0; k < DNV; ++k) {
