@@ -112,5 +112,26 @@ def test_headers_declare_the_flowsys_api():
         assert n in names, n
 
 
+def test_empty_legacy_calls_return_before_touching_arrays():
+    """Zero cells, or an empty colour range [n0, n0), is a successful no-op in
+    the reference's loops; here too, before any array is read (NULL arrays
+    allowed), so the calls need no GPU."""
+    for name, args in (
+            ("parallel_pre_lusgs", (0, 4, 1.0, None, None, None, None)),
+            ("serial_pre_lusgs", (0, 6, 1.0, None, None, None, None)),
+            ("parallel_ns_lower_sweep", (3, 3, 10, 4) + (None,) * 9),
+            ("parallel_ns_upper_sweep", (0, 0, 0, 4) + (None,) * 9),
+            ("ns_parallel_lower_sweep", (7, 7, 10, 4) + (None,) * 10),
+            ("ns_parallel_upper_sweep", (7, 7, 10, 4) + (None,) * 10),
+            ("serial_ns_lower_sweep", (0, 4) + (None,) * 7),
+            ("serial_ns_upper_sweep", (0, 4) + (None,) * 7),
+            ("ns_serial_lower_sweep", (0, 4) + (None,) * 8),
+            ("ns_serial_upper_sweep", (0, 4) + (None,) * 8),
+            ("lusgs_parallel_ns_update", (0, None, None)),
+            ("lusgs_serial_ns_update", (0, None, None)),
+            ("serial_ns_lusgs_update", (0, None, None))):
+        assert L.call(name, *args) == 0, name
+
+
 def test_device_count_call_is_safe_without_gpu():
     assert L.device_count() >= 0

@@ -17,6 +17,7 @@ import pytest
 import pyoracle as O
 from refbridge import greedy_coloring_py
 from ucfd_sparse_amd import lib as L
+from ucfd_sparse_amd import layout as LY
 from ucfd_sparse_amd import mesh as M
 
 pytestmark = pytest.mark.gpu
@@ -126,6 +127,47 @@ def test_colored_matches_oracle_and_fixture(name, where):
     assert_close(upts, ou, "upts")
     idx = z["sample"]
     assert_close(dq[:, idx], z["c_dq"], "dQ vs reference fixture")
+
+
+def isolate(mesh, cells):
+    """The mesh with every face of the given cells, and the matching face of
+    each of their neighbours, turned into a boundary self-face."""
+    import dataclasses
+    nei = mesh.nei_ele.copy()
+    F = nei.shape[0]
+    for i in cells:
+        for f in range(F):
+            j = int(nei[f, i])
+            if j != i:
+                nei[:, j][nei[:, j] == i] = j
+            nei[f, i] = i
+    return dataclasses.replace(mesh, nei_ele=nei)
+
+
+@pytest.mark.parametrize("frac", [1.0, 0.15])
+def test_boundary_only_cells(frac):
+    """Cells whose faces are all boundary self-faces take no face term:
+    dQ* = rhsb / diag and dQ = dQ* exactly; the rest of the mesh matches the
+    oracle bit for bit (all cells isolated: one colour)."""
+    say(f"tet mesh with {frac:.0%} of its cells isolated")
+    mesh = M.tet_mesh(3, seed=91, permute=True)
+    flow = M.flow_fields(mesh, seed=91)
+    N = mesh.neles
+    rng = np.random.default_rng(92)
+    cells = np.arange(N) if frac == 1.0 else rng.choice(N, size=int(frac*N), replace=False)
+    m2 = isolate(mesh, [int(c) for c in cells])
+    alone = np.all(m2.nei_ele == np.arange(N)[None, :], axis=0)
+    assert alone[cells].all()
+    nc, lcolor, icolor, off = LY.greedy_coloring(m2.nei_ele)
+    if frac == 1.0:
+        assert nc == 1
+    diag, dstar, dq, upts = colored_pass_lib(m2, flow, lcolor, icolor, torch_dev())
+    od, os_, oq, ou = O.colored_pass(m2.nei_ele, icolor, lcolor, m2.fnorm_vol, m2.vec_fnorm,
+                                     flow.fspr, flow.dt, flow.upts, flow.rhsb)
+    for got, ref in ((diag, od), (dstar, os_), (dq, oq), (upts, ou)):
+        assert np.array_equal(got, ref)
+    assert np.array_equal(dstar[:, alone], flow.rhsb[:5, alone] / diag[alone])
+    assert np.array_equal(dq[:, alone], dstar[:, alone])
 
 
 def test_two_array_form_equals_in_place_form():
