@@ -382,3 +382,71 @@ def test_c_abi_program_matches_oracle():
         assert (it, st) == (ito, sto)
         assert _close_hist(hist, histo, min(int(it) + 1, 30)) <= 1e-6
         assert np.allclose(xs, xo, rtol=1e-6, atol=1e-9 * np.abs(xo).max())
+
+
+def diagonal_bsr(bn, blk, seed):
+    rng = np.random.default_rng(seed)
+    rowptr = np.arange(bn + 1, dtype=np.int32)
+    colidx = np.arange(bn, dtype=np.int32)
+    values = rng.uniform(-0.2, 0.2, size=(bn, blk, blk)) + np.eye(blk) * 3.0
+    return M.BSR(bn, blk, rowptr, colidx, rowptr[:-1].copy(), values)
+
+
+@pytest.mark.parametrize("blk", [1, 5])
+def test_block_diagonal_matrix_closed_forms(K, blk):
+    """Rows without off-diagonal blocks (nothing staged by the wave kernels):
+    SpMV = D x, both preconditioners apply D^-1 exactly (= the oracle's LU
+    solve), and the preconditioned solvers stop at the first check."""
+    say(f"block-diagonal matrix, blk={blk}")
+    A = diagonal_bsr(700, blk, 80 + blk)
+    x = M.normal_fill(A.bn * blk, 81)
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    y = dev(np.zeros(A.bn * blk))
+    mat.mult(dev(x), y)
+    assert np.array_equal(y.cpu().numpy(), O.spmv_bsr(A, x))
+    dinv = O.blusgs_apply(A, O.blusgs_prepare(A), x)
+    for icolors in (None, np.array([0, A.bn], np.int32)):
+        pc = K.Precon.from_bsr(D, icolors=icolors)
+        b = dev(x)
+        pc.apply(b)
+        assert np.array_equal(b.cpu().numpy(), dinv)
+    D2 = dev_bsr(A)
+    pcb = K.Precon.from_bsr(D2, kind="bilu")
+    b = dev(x)
+    pcb.apply(b)
+    f = O.ilu_prepare(A, "bilu")
+    assert np.array_equal(b.cpu().numpy(), O.ilu_apply(A, f, x, "bilu"))
+    for kind in ("gmres", "bicgstab"):
+        s = K.Solver(kind, A.bn * blk, 10, 1e-10, m=5)
+        xs = dev(np.zeros(A.bn * blk))
+        st, it, res = s.solve(K.Precon.from_bsr(D), mat, xs, dev(x))
+        ref = O.gmres(A, x, 5, 10, 1e-10) if kind == "gmres" else O.bicgstab(A, x, 10, 1e-10)
+        say(f"  {kind}: status {st}, iterations {it}, residual {res:.2e}; oracle {ref[2]}, {ref[1]}")
+        assert (it, st) == (ref[1], ref[2])
+        assert np.linalg.norm(x - A.to_scipy() @ xs.cpu().numpy()) <= 1e-9 * np.linalg.norm(x)
+
+
+def test_ragged_rows_bitwise(K):
+    """A matrix whose rows range from diagonal-only to many blocks: SpMV and
+    the sequential and block-ILU applies bit for bit against the oracle."""
+    import sys as _s
+    import os as _o
+    _s.path.insert(0, _o.path.join(_o.path.dirname(_o.path.abspath(__file__)), "golden"))
+    import make_golden
+    say("ragged rows")
+    A = make_golden.random_pattern_bsr(500, 3, 3, 93)
+    deg = np.diff(A.rowptr)
+    assert deg.min() == 1 and deg.max() >= 6
+    x = M.normal_fill(A.bn * 3, 94)
+    D = dev_bsr(A)
+    y = dev(np.zeros(A.bn * 3))
+    K.SpMat.from_bsr(D).mult(dev(x), y)
+    assert np.array_equal(y.cpu().numpy(), O.spmv_bsr(A, x))
+    b = dev(x)
+    K.Precon.from_bsr(D).apply(b)
+    assert np.array_equal(b.cpu().numpy(), O.blusgs_apply(A, O.blusgs_prepare(A), x))
+    D2 = dev_bsr(A)
+    b = dev(x)
+    K.Precon.from_bsr(D2, kind="bilu").apply(b)
+    assert np.array_equal(b.cpu().numpy(), O.ilu_apply(A, O.ilu_prepare(A, "bilu"), x, "bilu"))
