@@ -32,31 +32,65 @@ inline dim3 grid(int64_t n) { return dim3((unsigned)((n + FB - 1)/FB)); }
 
 int ok() { return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP; }
 
-// rank_blusgs_pack (blusgs.c:17-45)
-__global__ __launch_bounds__(FB) void k_pack(int neles, int nlocal, int nvars, int nfvars, double a0,
+// rank_blusgs_pack (blusgs.c:17-45): rank_rhs[r][k] = rhs[k][e] and the
+// packed diagonal block of r = (1/dt + a0) I, r = cell_ids[e].  One wave per
+// 64 elements: the wave reads rhs one variable at a time (consecutive lanes
+// on consecutive elements) into LDS, then writes the rows of rank_rhs and
+// the diagonal blocks as flat ranges, consecutive lanes on consecutive
+// words of each row (one thread per element would scatter every store over
+// up to 64 cache lines).
+constexpr int PACK_MAXV = 16;
+
+__global__ __launch_bounds__(64) void k_pack(int neles, int nlocal, int nvars, int nfvars, double a0,
                                              const int *__restrict__ cell_ids,
                                              const double *__restrict__ rhs,
                                              const double *__restrict__ dt, double *rank_rhs,
                                              double *diag, int *flag)
 {
-    const int e = blockIdx.x*FB + threadIdx.x;
-    if (e >= neles) return;
-    const int r = cell_ids[e];
-    if ((unsigned)r >= (unsigned)nlocal) {
-        atomicOr(flag, UCFD_KFLAG_INDEX);
-        return;
+    __shared__ double srhs[64*PACK_MAXV];
+    const int lane = threadIdx.x;
+    const int e0 = ucfd::wave_xcd_index()*64;
+    if (e0 >= neles) return;
+    const int nact = min(64, neles - e0);
+    const int e = e0 + lane;
+    int r = -1;
+    double dg = 0.0;
+    if (lane < nact) {
+        r = cell_ids[e];
+        if ((unsigned)r >= (unsigned)nlocal) {
+            atomicOr(flag, UCFD_KFLAG_INDEX);
+            r = -1;
+        }
+        dg = 1/dt[e] + a0;
+        for (int k = 0; k < nvars; ++k) srhs[lane*nvars + k] = rhs[e + (int64_t)k*neles];
     }
-    for (int k = 0; k < nvars; ++k) rank_rhs[k + (int64_t)r*nvars] = rhs[e + (int64_t)k*neles];
+    ucfd::wave_lds_sync();
+    // uniform trip counts: every lane takes part in every shuffle
+    for (int f0 = 0; f0 < nact*nvars; f0 += 64) {
+        const int f = f0 + lane;
+        const int el = min(f/nvars, nact - 1), k = f - el*nvars;
+        const int re = __shfl(r, el, 64);
+        if (f < nact*nvars && re >= 0) rank_rhs[k + (int64_t)re*nvars] = srhs[f];
+    }
     const int d2 = nfvars*nfvars;
-    for (int row = 0; row < nfvars; ++row) {
-        double *drow = diag + (int64_t)r*d2 + row*nfvars;
-        for (int col = 0; col < nfvars; ++col) drow[col] = 0.0;
-        drow[row] = 1/dt[e] + a0;
+    for (int f0 = 0; f0 < nact*d2; f0 += 64) {
+        const int f = f0 + lane;
+        const int el = min(f/d2, nact - 1), q = f - el*d2;
+        const int re = __shfl(r, el, 64);
+        const double de = __shfl(dg, el, 64);
+        if (f < nact*d2 && re >= 0) {
+            const int row = q/nfvars, col = q - row*nfvars;
+            diag[(int64_t)re*d2 + q] = row == col ? de : 0.0;
+        }
     }
 }
 
 // rank_tblusgs_pack (blusgs.c:60-94) with sa_src_jacobian / kwsst_src_jacobian
 // (source_jacobian.c:6-18); BETAST = 0.09 (flux.h:8)
+// NT > 0: the block (NT = 1 Spalart-Allmaras, 2 k-omega SST) is formed in
+// registers with the reference's operations in order and stored once; NT = 0
+// is the general form on tdiag in place.
+template <int NT>
 __global__ __launch_bounds__(FB) void k_tpack(int neles, int nlocal, int nvars, int nfvars, double a0,
                                               double factor, int kind,
                                               const int *__restrict__ cell_ids,
@@ -70,6 +104,27 @@ __global__ __launch_bounds__(FB) void k_tpack(int neles, int nlocal, int nvars, 
     const int r = cell_ids[e];
     if ((unsigned)r >= (unsigned)nlocal) {
         atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    if (NT > 0) {
+        constexpr int D2 = NT > 0 ? NT*NT : 1;
+        double A[D2];
+#pragma unroll
+        for (int q = 0; q < D2; ++q) A[q] = 0.0;
+        if (kind == 1) {
+            A[0] += dsrc[e + (int64_t)(nvars - 1)*neles];
+        } else if (kind == 2 && NT >= 2) {
+            const double kk = uptsb[e + (int64_t)(nvars - 2)*neles]/uptsb[e];
+            const double bk = 0.09*kk;
+            A[0] += dsrc[e + (int64_t)(nvars - 2)*neles];
+            A[1] += bk > 0.0 ? bk : 0.0;
+            A[(1 + NT) % D2] = dsrc[e + (int64_t)(nvars - 1)*neles];
+        }
+        const double dd = 1/(dt[e]*factor) + a0;
+#pragma unroll
+        for (int row = 0; row < NT; ++row) A[row + row*NT] += dd;
+#pragma unroll
+        for (int q = 0; q < D2; ++q) tdiag[(int64_t)r*D2 + q] = A[q];
         return;
     }
     const int nt = nvars - nfvars, d2 = nt*nt;
@@ -788,8 +843,9 @@ int fs_launch_pack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars, 
                    double *diag, int *flag)
 {
     if (neles <= 0) return UCFD_OK;
-    hipLaunchKernelGGL(k_pack, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars, a0,
-                       cell_ids, rhs, dt, rank_rhs, diag, flag);
+    if (nvars > PACK_MAXV) return UCFD_ERR_ARG;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)((neles + 63)/64)), dim3(64), 0, s, neles, nlocal,
+                       nvars, nfvars, a0, cell_ids, rhs, dt, rank_rhs, diag, flag);
     return ok();
 }
 
@@ -798,8 +854,16 @@ int fs_launch_tpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
                     const double *dsrc, const double *dt, double *tdiag, int *flag)
 {
     if (neles <= 0) return UCFD_OK;
-    hipLaunchKernelGGL(k_tpack, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars, a0,
-                       factor, kind, cell_ids, uptsb, dsrc, dt, tdiag, flag);
+    const int nt = nvars - nfvars;
+    if (nt == 1)
+        hipLaunchKernelGGL(k_tpack<1>, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
+                           a0, factor, kind, cell_ids, uptsb, dsrc, dt, tdiag, flag);
+    else if (nt == 2)
+        hipLaunchKernelGGL(k_tpack<2>, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
+                           a0, factor, kind, cell_ids, uptsb, dsrc, dt, tdiag, flag);
+    else
+        hipLaunchKernelGGL(k_tpack<0>, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
+                           a0, factor, kind, cell_ids, uptsb, dsrc, dt, tdiag, flag);
     return ok();
 }
 
