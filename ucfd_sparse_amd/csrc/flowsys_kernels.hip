@@ -145,8 +145,8 @@ __global__ __launch_bounds__(FB) void k_tpack(int neles, int nlocal, int nvars, 
 // rank_blusgs_update (blusgs.c:126-143)
 __global__ __launch_bounds__(FB) void k_update(int neles, int nlocal, int nvars,
                                                const int *__restrict__ cell_ids,
-                                               const double *__restrict__ du, double *upts,
-                                               int *flag)
+                                               const double *__restrict__ du,
+                                               double *__restrict__ upts, int *flag)
 {
     const int e = blockIdx.x*FB + threadIdx.x;
     if (e >= neles) return;
@@ -162,8 +162,9 @@ __global__ __launch_bounds__(FB) void k_update(int neles, int nlocal, int nvars,
 __global__ __launch_bounds__(FB) void k_subres(int neles, int nlocal, int nvars,
                                                const int *__restrict__ cell_ids,
                                                const double *__restrict__ vol,
-                                               const double *__restrict__ du, double *dup,
-                                               double *res, int *flag)
+                                               const double *__restrict__ du,
+                                               double *__restrict__ dup, double *__restrict__ res,
+                                               int *flag)
 {
     const int e = blockIdx.x*FB + threadIdx.x;
     if (e >= neles) return;
@@ -561,8 +562,9 @@ __global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, 
                                               const double *__restrict__ upts,
                                               const double *__restrict__ rhs,
                                               const double *__restrict__ dt,
-                                              const double *__restrict__ dsrc, double *u,
-                                              double *du, double *diag, int *flag)
+                                              const double *__restrict__ dsrc,
+                                              double *__restrict__ u, double *__restrict__ du,
+                                              double *__restrict__ diag, int *flag)
 {
     const int e = fxcd()*FB + threadIdx.x;
     if (e >= neles) return;
@@ -584,8 +586,8 @@ __global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, 
 // rank_lusgs_update (lusgs.c:54-70)
 __global__ __launch_bounds__(FB) void k_lupdate(int neles, int nlocal, int nvars,
                                                 const int *__restrict__ cell_ids,
-                                                const double *__restrict__ du, double *upts,
-                                                int *flag)
+                                                const double *__restrict__ du,
+                                                double *__restrict__ upts, int *flag)
 {
     const int e = fxcd()*FB + threadIdx.x;
     if (e >= neles) return;
