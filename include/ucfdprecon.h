@@ -42,8 +42,10 @@ UCFD_EXTERN ucfd_status_t UCFDPreconCreateNone(Precon *precon);
  * library memory (FP64, no pivoting).  ILU family (ilu.c:4-48, bilu.c:5-65,
  * pbilu.c:5-76): the ILU(0) factorisation, in place on the caller's values
  * array, as the reference does (calling Prepare twice factors twice).  A
- * zero pivot returns UCFD_FAILED.  Also validates the structure and builds
- * the sweep schedule; the structure must not change afterwards. */
+ * zero pivot returns UCFD_FAILED.  The first Prepare after a UCFDPreconSet*
+ * call also validates the structure and builds the sweep schedule; later
+ * Prepares (new values, e.g. one per Newton step) reuse them and read only
+ * the values, so the structure must not change afterwards. */
 UCFD_EXTERN ucfd_status_t UCFDPreconPrepare(Precon precon);
 /* precon.c:36-45 */
 UCFD_EXTERN ucfd_status_t UCFDPreconDestroy(Precon *precon);

@@ -450,3 +450,31 @@ def test_ragged_rows_bitwise(K):
     b = dev(x)
     K.Precon.from_bsr(D2, kind="bilu").apply(b)
     assert np.array_equal(b.cpu().numpy(), O.ilu_apply(A, O.ilu_prepare(A, "bilu"), x, "bilu"))
+
+
+def test_prepare_again_with_new_values(K):
+    """A later Prepare reuses the schedule built by the first and reads only
+    the values: after the values change in place, the apply equals the
+    oracle's on the new values (block LU-SGS and coloured block ILU(0))."""
+    say("second Prepare on new values")
+    A, off = system("tet", 4, 95, colored=True)
+    x = M.normal_fill(A.bn * A.blk, 96)
+    D = dev_bsr(A)
+    pc = K.Precon.from_bsr(D, icolors=off)
+    rng = np.random.default_rng(97)
+    A2 = M.BSR(A.bn, A.blk, A.rowptr, A.colidx, A.diagslots,
+               A.values * rng.uniform(0.9, 1.1, size=A.values.shape))
+    D.values.copy_(dev(A2.values))
+    pc.prepare()
+    b = dev(x)
+    pc.apply(b)
+    assert np.array_equal(b.cpu().numpy(), O.blusgs_apply(A2, O.blusgs_prepare(A2), x, icolors=off))
+    Di = dev_bsr(A)
+    pci = K.Precon.from_bsr(Di, icolors=off, kind="bilu")
+    Di.values.copy_(dev(A2.values))
+    pci.prepare()
+    b = dev(x)
+    pci.apply(b)
+    f = O.ilu_prepare(A2, "pbilu")
+    assert np.array_equal(Di.values.cpu().numpy(), f.reshape(A.values.shape))
+    assert np.array_equal(b.cpu().numpy(), O.ilu_apply(A2, f, x, "pbilu", off))

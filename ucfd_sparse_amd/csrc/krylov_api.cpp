@@ -49,6 +49,7 @@ struct _Precon {
     const UCFDInt *icolors;  // PBLUSGS / PBILU: ncolors+1 offsets (caller's array)
     bool fp32;               // mixed apply: FP32 blocks and work vector, FP64 diagonal LU
     bool prepared;
+    bool scheduled;          // structure validated and schedules built (first Prepare)
     int64_t nnz;
     double *diagvalues;      // device, bn*blk*blk: LU-SGS the factored diagonal
                              // blocks, ILU the factored U_ii (scalar: the pivot)
@@ -360,6 +361,7 @@ void pc_free_device(Precon pc)
     ucfd_free_schedule(pc->lower);
     ucfd_free_schedule(pc->upper);
     pc->prepared = false;
+    pc->scheduled = false;
 }
 
 ucfd_status_t set_block(Precon *precon, UCFDInt bn, UCFDInt block, PreconType type, int ncolors,
@@ -794,9 +796,14 @@ UCFD_API ucfd_status_t UCFDPreconPrepare(Precon precon)
     Precon pc = precon;
     int rc = pc_alloc(pc, fn);
     if (rc) return UCFD_FAILED;
-    if ((rc = read_nnz(pc->rowptr, pc->bn, &pc->nnz, fn))) return UCFD_FAILED;
-    if (!pc->colidx || !pc->diagslots || !pc->values) return fail(fn, "NULL preconditioner array");
-    if ((rc = make_schedules(pc, fn))) return UCFD_FAILED;
+    if (!pc->scheduled) {
+        // the structure is validated and scheduled once per preconditioner
+        // type; later Prepares read only the values
+        if ((rc = read_nnz(pc->rowptr, pc->bn, &pc->nnz, fn))) return UCFD_FAILED;
+        if (!pc->colidx || !pc->diagslots || !pc->values) return fail(fn, "NULL preconditioner array");
+        if ((rc = make_schedules(pc, fn))) return UCFD_FAILED;
+        pc->scheduled = true;
+    }
     const size_t b2 = (size_t)pc->blk*pc->blk;
     const int kind = pc_kind(pc);
     if (kind != KRY_SW_LUSGS) {
@@ -820,11 +827,7 @@ UCFD_API ucfd_status_t UCFDPreconPrepare(Precon precon)
         pc->prepared = rc == UCFD_OK;
         return status(rc);
     }
-    if (pc->fp32) {
-        if (pc->vals32) (void)hipFree(pc->vals32);
-        if (pc->x32) (void)hipFree(pc->x32);
-        pc->vals32 = nullptr;
-        pc->x32 = nullptr;
+    if (pc->fp32 && !pc->vals32) {
         const size_t vb = sizeof(float)*(size_t)pc->nnz*b2, xb = sizeof(float)*(size_t)pc->bn*pc->blk;
         if ((rc = hip(hipMalloc((void **)&pc->vals32, vb ? vb : 4), fn, "hipMalloc(vals32)")) ||
             (rc = hip(hipMalloc((void **)&pc->x32, xb ? xb : 4), fn, "hipMalloc(x32)")))
