@@ -109,6 +109,23 @@ int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, co
     return sg.leave(rc);
 }
 
+// Level schedules of the serial sweeps, one per direction, kept for the
+// last neighbour table seen: the key is the array's address, the sizes and a
+// content hash of nei_ele (about 0.1 ms on the device), so a schedule is
+// reused only for the same table and rebuilt whenever an entry changes.
+struct SerialSchedule {
+    const void *nei = nullptr;
+    int neles = -1, nface = -1;
+    bool dev = false;
+    unsigned long long hash = 0;
+    std::vector<int> off;
+    int *order_d = nullptr;     // device copy of the level order
+    size_t cap = 0;
+};
+SerialSchedule g_serial[2];
+std::mutex g_serial_mu;
+unsigned long long *g_hash_d = nullptr;
+
 // Dependency levels of the sequential sweep: lower (nb < i) or upper (nb > i).
 // order[] lists cells level by level, off[l]..off[l+1] is level l.
 void build_levels(int dir, int N, int F, const int *nei, std::vector<int> &order,
@@ -156,24 +173,59 @@ int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
     int rc = ucfd_ensure_device(fn);
     if (rc) return rc;
 
-    // the level schedule is built on the host from nei_ele
-    std::vector<int> hnei;
-    const int *nei_h = nei;
+    // the level schedule is built on the host from nei_ele, or reused
     if (!nei) return ucfd_report(UCFD_ERR_ARG, fn, "NULL nei_ele");
-    if (ucfd_is_device_ptr(nei)) {
-        hnei.resize(F*N);
-        rc = ucfd_check_hip(hipMemcpyAsync(hnei.data(), nei, F*N*4, hipMemcpyDeviceToHost,
-                                           ucfd_stream()), fn, "hipMemcpyAsync(nei_ele)");
+    const bool dev = ucfd_is_device_ptr(nei);
+    unsigned long long h = 0;
+    if (dev) {
+        if (!g_hash_d && (rc = ucfd_check_hip(hipMalloc((void **)&g_hash_d, 64), fn,
+                                              "hipMalloc(hash)")))
+            return rc;
+        rc = ucfd_launch_hash(ucfd_stream(), nei, (int64_t)(F*N), g_hash_d);
+        if (!rc) rc = ucfd_check_hip(hipMemcpyAsync(&h, g_hash_d, sizeof h, hipMemcpyDeviceToHost,
+                                                    ucfd_stream()), fn, "hipMemcpyAsync(hash)");
         if (!rc) rc = ucfd_check_hip(hipStreamSynchronize(ucfd_stream()), fn, "sync");
         if (rc) return rc;
-        nei_h = hnei.data();
+    } else {
+        h = ucfd_hash_host(nei, (int64_t)(F*N));
     }
-    std::vector<int> order, off;
-    build_levels(dir, neles, nface, nei_h, order, off);
+    std::lock_guard<std::mutex> lock(g_serial_mu);
+    SerialSchedule &C = g_serial[dir];
+    if (!(C.nei == nei && C.neles == neles && C.nface == nface && C.dev == dev && C.hash == h)) {
+        std::vector<int> hnei;
+        const int *nei_h = nei;
+        if (dev) {
+            hnei.resize(F*N);
+            rc = ucfd_check_hip(hipMemcpy(hnei.data(), nei, F*N*4, hipMemcpyDeviceToHost), fn,
+                                "hipMemcpy(nei_ele)");
+            if (rc) return rc;
+            nei_h = hnei.data();
+        }
+        std::vector<int> order;
+        C.nei = nullptr;
+        build_levels(dir, neles, nface, nei_h, order, C.off);
+        if (C.cap < N) {
+            if (C.order_d) (void)hipFree(C.order_d);
+            C.order_d = nullptr;
+            C.cap = 0;
+            if ((rc = ucfd_check_hip(hipMalloc((void **)&C.order_d, N*4), fn, "hipMalloc(order)")))
+                return rc;
+            C.cap = N;
+        }
+        if ((rc = ucfd_check_hip(hipMemcpy(C.order_d, order.data(), N*4, hipMemcpyHostToDevice),
+                                 fn, "hipMemcpy(order)")))
+            return rc;
+        C.nei = nei;
+        C.neles = neles;
+        C.nface = nface;
+        C.dev = dev;
+        C.hash = h;
+    }
+    const std::vector<int> &off = C.off;
 
     UcfdStage sg(fn);
     const int a_nei = sg.add(nei, F*N*4, UCFD_IN);
-    const int a_ord = sg.add(order.data(), N*4, UCFD_IN);
+    const int a_ord = sg.add(C.order_d, N*4, UCFD_IN);
     const int a_fnv = sg.add(fnorm_vol, F*N*8, UCFD_IN);
     const int a_vfn = sg.add(vec_fnorm, F*D*N*8, UCFD_IN);
     const int a_u = sg.add(upts, V*N*8, UCFD_IN);
@@ -194,8 +246,6 @@ int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
                                (const double *)sg.dev(a_sp), ucfd_device_flag());
     }
     if (rc) ucfd_report(rc, fn, "kernel launch failed");
-    // the level order is a host array, so it is staged and leave() always
-    // waits for the kernels before this frame releases it
     return sg.leave(rc);
 }
 

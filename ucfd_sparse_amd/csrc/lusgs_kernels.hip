@@ -386,6 +386,44 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
                                 vec_fnorm, upts, in, out, diag, fspr, flag);
 }
 
+// Content hash of an int array: sum over k of splitmix64(k, a[k]) mod 2^64.
+// The sum is order independent, so the block partials can be added with
+// atomics and the result is still deterministic.
+__device__ __forceinline__ unsigned long long hash_mix(unsigned long long k, int v)
+{
+    unsigned long long z = k*0x9E3779B97F4A7C15ull + (unsigned long long)(unsigned)v;
+    z = (z ^ (z >> 30))*0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27))*0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_hash_zero(unsigned long long *out)
+{
+    if (threadIdx.x == 0) out[0] = 0ull;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hash(const int *__restrict__ a, int64_t n,
+                                                 unsigned long long *out)
+{
+    unsigned long long s = 0ull;
+    const int64_t st = (int64_t)gridDim.x*kBlock;
+    for (int64_t k = (int64_t)blockIdx.x*kBlock + threadIdx.x; k < n; k += st)
+        s += hash_mix((unsigned long long)k, a[k]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+int ucfd_launch_hash(hipStream_t s, const int *a, int64_t n, unsigned long long *out)
+{
+    hipLaunchKernelGGL(k_hash_zero, dim3(1), dim3(64), 0, s, out);
+    if (n > 0) {
+        int64_t blocks = (n + kBlock - 1)/kBlock;
+        if (blocks > 2048) blocks = 2048;
+        hipLaunchKernelGGL(k_hash, dim3((unsigned)blocks), dim3(kBlock), 0, s, a, n, out);
+    }
+    return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+}
+
 int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x)
 {
     if (n <= 0) return UCFD_OK;

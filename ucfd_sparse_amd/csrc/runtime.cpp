@@ -253,6 +253,19 @@ int UcfdStage::leave(int status)
     return rc;
 }
 
+unsigned long long ucfd_hash_host(const int *a, int64_t n)
+{
+    unsigned long long s = 0ull;
+    for (int64_t k = 0; k < n; ++k) {
+        unsigned long long z = (unsigned long long)k*0x9E3779B97F4A7C15ull +
+                               (unsigned long long)(unsigned)a[k];
+        z = (z ^ (z >> 30))*0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27))*0x94D049BB133111EBull;
+        s += z ^ (z >> 31);
+    }
+    return s;
+}
+
 // ---------------------------------------------------------------- extensions
 
 UCFD_API int ucfd_set_stream(void *stream)

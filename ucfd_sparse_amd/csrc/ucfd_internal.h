@@ -85,3 +85,8 @@ UCFD_HIDDEN int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int 
                                   int *flag);
 
 UCFD_HIDDEN int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x);
+// *out = sum over k < n of a mix of (k, a[k]) mod 2^64 (device array a; out
+// device memory): a content hash, order independent and deterministic.
+UCFD_HIDDEN int ucfd_launch_hash(hipStream_t s, const int *a, int64_t n, unsigned long long *out);
+// the same hash of a host array
+UCFD_HIDDEN unsigned long long ucfd_hash_host(const int *a, int64_t n);
