@@ -3,8 +3,9 @@ SetPBILU / SetCUDABILU, UCFDPreconPrepare, UCFDPreconApply, UCFDSolve),
 against the CPU oracle (oracle/ilu_oracle.c) on the same seeded inputs and
 against the reference's own outputs (tests/golden/ilu_tet3p.npz).
 
-Bars: the in-place factorisations (level-scheduled, one thread per row) and
-the triangular solves are bit-exact (same operation order, no FMA
+Bars: the in-place factorisations (level-scheduled; block rows shared by
+the lanes of a wave, or one thread per row when a lower slot names its own
+row) and the triangular solves are bit-exact (same operation order, no FMA
 contraction); GMRES / BiCGSTAB preconditioned by them follow the oracle with
 the bars of the other Krylov tests (same iteration counts and status,
 histories within 1e-6, true residual within the tolerance).
@@ -161,6 +162,28 @@ def test_colored_block_ilu_with_coupled_ranges_bitwise(K):
     x = dev(b)
     pc.apply(x)
     assert np.array_equal(x.cpu().numpy(), O.ilu_apply(A, f, b, "pbilu", off))
+
+
+@pytest.mark.parametrize("blk,ncolors", [(1, 0), (3, 0), (5, 0), (4, 3)])
+def test_self_slots_bitwise(K, blk, ncolors):
+    """Lower slots whose column is the row itself: the block factorisation
+    leaves the wave-per-rows kernel for the thread-per-row one (a row's
+    blocks are then read while the row is written), the solves read the
+    row's own partial result; all as the sequential reference does."""
+    A0, off, b = random_case(500, blk, 6, 680 + blk, ncolors=ncolors)
+    A = make_golden.with_self_slots(A0, 4, 681)
+    kind = "ilu" if blk == 1 else ("pbilu" if ncolors else "bilu")
+    say(f"{kind} with self slots bn={A.bn} blk={blk}")
+    D = Dev(A)
+    if blk == 1:
+        pc = K.Precon(D.rowptr, D.colidx, D.diagslots, D.values, A.bn, 1, kind="ilu")
+    else:
+        pc = K.Precon.from_bsr(D, icolors=off, kind="bilu")
+    f = O.ilu_prepare(A, kind)
+    assert np.array_equal(D.values.cpu().numpy(), f.reshape(A.values.shape))
+    x = dev(b)
+    pc.apply(x)
+    assert np.array_equal(x.cpu().numpy(), O.ilu_apply(A, f, b, kind, off))
 
 
 def test_second_prepare_factors_again(K):

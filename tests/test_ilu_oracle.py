@@ -178,3 +178,20 @@ def test_live_reference_on_another_pattern():
     f = O.ilu_prepare(A, "bilu")
     assert np.array_equal(f, vals.reshape(f.shape))
     assert np.array_equal(O.ilu_apply(A, f, b, "bilu"), x)
+
+
+@pytest.mark.skipif(not refkrylov.available(), reason="reference build absent")
+@pytest.mark.parametrize("blk", [1, 3])
+def test_live_reference_with_self_slots(blk):
+    """Lower slots whose column is the row itself (the pivot block is then
+    the row's own unfactored diagonal block, and the updates find the
+    diagonal slot as the last match of that column)."""
+    A = make_golden.with_self_slots(make_golden.random_pattern_bsr(80, blk, 6, 420 + blk), 5,
+                                    421)
+    b = np.random.default_rng(422).normal(size=A.bn * A.blk)
+    kind = "bilu" if blk > 1 else "ilu"
+    vals, x = refkrylov.ilu_prepare_apply(A, kind, b)
+    f = O.ilu_prepare(A, kind)
+    assert np.array_equal(f, vals.reshape(f.shape))
+    if blk > 1:
+        assert np.array_equal(O.ilu_apply(A, f, b, kind), x)

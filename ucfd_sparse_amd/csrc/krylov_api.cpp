@@ -50,6 +50,7 @@ struct _Precon {
     bool fp32;               // mixed apply: FP32 blocks and work vector, FP64 diagonal LU
     bool prepared;
     bool scheduled;          // structure validated and schedules built (first Prepare)
+    bool self_free;          // no lower slot of a row refers to the row itself
     int64_t nnz;
     double *diagvalues;      // device, bn*blk*blk: LU-SGS the factored diagonal
                              // blocks, ILU the factored U_ii (scalar: the pivot)
@@ -273,6 +274,13 @@ int make_schedules(Precon pc, const char *fn)
     for (int i = 0; i < bn; ++i)
         if (ds[i] < rp[i] || ds[i] >= rp[i + 1])
             return ucfd_report(UCFD_ERR_INDEX, fn, "diagslots[i] must lie in rowptr[i] .. rowptr[i+1]-1");
+    pc->self_free = true;
+    for (int i = 0; i < bn && pc->self_free; ++i)
+        for (int j = rp[i]; j < ds[i]; ++j)
+            if (ci[j] == i) {
+                pc->self_free = false;
+                break;
+            }
 
     std::vector<int> lseq, useq;
     std::vector<int> off;
@@ -817,7 +825,8 @@ UCFD_API ucfd_status_t UCFDPreconPrepare(Precon precon)
         if ((rc = sg.enter())) return UCFD_FAILED;
         const PcDev d = pc_dev(sg, pi);
         for (const UcfdLevel &l : pc->lower.lv)
-            if ((rc = kry_ilu_factor(ucfd_stream(), kind == KRY_SW_ILU, pc->blk, l.base, l.cnt,
+            if ((rc = kry_ilu_factor(ucfd_stream(), kind == KRY_SW_ILU, pc->self_free, pc->blk,
+                                     l.base, l.cnt,
                                      l.ordered ? pc->lower.order : nullptr, d.rowptr, d.colidx,
                                      d.diagslots, (double *)d.values, pc->diagvalues,
                                      ucfd_device_flag())))

@@ -33,7 +33,9 @@ UCFD_HIDDEN int kry_bsgs_prepare(hipStream_t s, int bn, int blk, int64_t nnz, co
 // order[base..]): scalar != 0 is ILU(0) of a CSR matrix (blk 1, ilu.c:4-48),
 // else block ILU(0) (bilu.c:5-65).  diagvalues[i] gets row i's factored
 // diagonal block (scalar: its pivot); raises UCFD_KFLAG_PIVOT on a zero pivot.
-UCFD_HIDDEN int kry_ilu_factor(hipStream_t s, int scalar, int blk, int base, int cnt,
+// rows != 0 (block form): no lower slot of a row refers to the row itself,
+// which lets the lanes of a wave share a row's block operations.
+UCFD_HIDDEN int kry_ilu_factor(hipStream_t s, int scalar, int rows, int blk, int base, int cnt,
                                const int *order, const int *rowptr, const int *colidx,
                                const int *diagslots, double *values, double *diagvalues,
                                int *flag);

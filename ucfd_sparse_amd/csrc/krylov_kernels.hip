@@ -295,6 +295,106 @@ __global__ __launch_bounds__(KB) void k_ilu_factor(int base, int cnt, const int 
     if (bad) atomicOr(flag, UCFD_KFLAG_PIVOT);
 }
 
+// The block form on one wave per G = 64/B rows of a level: lane (g, r) owns
+// scalar row r of block row g.  Both block operations of a lower slot act on
+// rows independently: row r of L_ik = row r of A_ik times (L_kk U_kk)^-1
+// (lusubmattrans restricted to one row, the same operations in the same
+// order), and row r of A_ij -= (row r of L_ik) A_kj; so the lanes of a group
+// run the row loop of the thread-per-row kernel side by side, reading the
+// blocks of row ck whole (an earlier or a later level: not written here).
+// The group then gathers its diagonal block with lane shuffles and every
+// lane factors it; lane r writes row r.  Bit-identical to k_ilu_factor.
+// Requires that no lower slot of a row refers to the row itself (the host
+// keeps k_ilu_factor for such a matrix).
+template <int B>
+__global__ __launch_bounds__(64) void k_ilu_factor_w(int base, int cnt,
+                                                     const int *__restrict__ order,
+                                                     const int *__restrict__ rowptr,
+                                                     const int *__restrict__ colidx,
+                                                     const int *__restrict__ diagslots,
+                                                     double *values, double *__restrict__ dv,
+                                                     int *flag)
+{
+    constexpr int B2 = B*B, G = 64/B;
+    const int lane = threadIdx.x;
+    const int t0 = ucfd::wave_xcd_index()*G;
+    if (t0 >= cnt) return;
+    const int g = lane/B, r = lane%B;
+    const bool act = g < G && t0 + g < cnt;
+    int i = 0, st = 0, ed = 0, jed = 0;
+    if (act) {
+        i = order ? order[base + t0 + g] : base + t0 + g;
+        st = rowptr[i];
+        ed = diagslots[i];
+        jed = rowptr[i + 1];
+    }
+    for (int k = st; k < ed; ++k) {
+        const int ck = colidx[k], kk = diagslots[ck], kst = rowptr[ck], ked = rowptr[ck + 1];
+        const double *LU = values + (int64_t)kk*B2;
+        double m[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) m[c] = values[(int64_t)k*B2 + r*B + c];
+        // row r of M (L U)^-1, in the order of ucfd::lusubmattrans
+        m[0] /= LU[0];
+#pragma unroll
+        for (int c = 1; c < B; ++c) {
+            double val = 0.0;
+#pragma unroll
+            for (int q = 0; q < c; ++q) val += m[q]*LU[q*B + c];
+            m[c] = (m[c] - val)/LU[c*B + c];
+        }
+#pragma unroll
+        for (int c = B - 2; c > -1; --c) {
+            double val = 0.0;
+#pragma unroll
+            for (int q = c + 1; q < B; ++q) val += m[q]*LU[q*B + c];
+            m[c] -= val;
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) values[(int64_t)k*B2 + r*B + c] = m[c];
+        for (int j = k + 1; j < jed; ++j) {
+            const int col = colidx[j];
+            int iwj = -1;
+            for (int q = kst; q < ked; ++q)
+                if (colidx[q] == col) iwj = q;
+            if (iwj < 0) continue;
+            const double *u = values + (int64_t)iwj*B2;
+            double *a = values + (int64_t)j*B2 + r*B;
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int e = 0; e < B; ++e) v += m[e]*u[e*B + c];
+                a[c] -= v;
+            }
+        }
+    }
+    // the group's diagonal block (each lane wrote its own row), then LU
+    double dr[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) dr[c] = act ? values[(int64_t)ed*B2 + r*B + c] : 0.0;
+    double d[B2];
+#pragma unroll
+    for (int rr = 0; rr < B; ++rr)
+#pragma unroll
+        for (int c = 0; c < B; ++c) d[rr*B + c] = __shfl(dr[c], g*B + rr, 64);
+    if (!act) return;
+    if (!ucfd::ludcmp<B>(d) && r == 0) atomicOr(flag, UCFD_KFLAG_PIVOT);
+    double o[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) o[c] = d[c];
+#pragma unroll
+    for (int rr = 1; rr < B; ++rr)
+        if (r == rr)
+#pragma unroll
+            for (int c = 0; c < B; ++c) o[c] = d[rr*B + c];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+        values[(int64_t)ed*B2 + r*B + c] = o[c];
+        dv[(int64_t)i*B2 + r*B + c] = o[c];
+    }
+}
+
 __global__ void k_to32(int64_t n, const double *__restrict__ a, float *__restrict__ b)
 {
     const int64_t st = (int64_t)gridDim.x*KB;
@@ -807,14 +907,18 @@ template <int B> struct PrepL {
 };
 
 template <int B> struct IluL {
-    static int run(hipStream_t s, bool scalar, int base, int cnt, const int *ord, const int *rp,
-                   const int *ci, const int *ds, double *v, double *dv, int *flag)
+    static int run(hipStream_t s, bool scalar, bool rows, int base, int cnt, const int *ord,
+                   const int *rp, const int *ci, const int *ds, double *v, double *dv, int *flag)
     {
         const dim3 g((cnt + KB - 1)/KB);
         if (scalar) {
             if (B != 1) return UCFD_ERR_ARG;
             hipLaunchKernelGGL((k_ilu_factor<1, true>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci,
                                ds, v, dv, flag);
+        } else if (B > 1 && rows) {
+            if constexpr (B > 1)
+                hipLaunchKernelGGL((k_ilu_factor_w<B>), dim3((cnt + 64/B - 1)/(64/B)), dim3(64), 0,
+                                   s, base, cnt, ord, rp, ci, ds, v, dv, flag);
         } else {
             hipLaunchKernelGGL((k_ilu_factor<B, false>), g, dim3(KB), 0, s, base, cnt, ord, rp, ci,
                                ds, v, dv, flag);
@@ -941,13 +1045,13 @@ int kry_bsgs_prepare(hipStream_t s, int bn, int blk, int64_t nnz, const int *dia
     return ok();
 }
 
-int kry_ilu_factor(hipStream_t s, int scalar, int blk, int base, int cnt, const int *order,
-                   const int *rowptr, const int *colidx, const int *diagslots, double *values,
-                   double *diagvalues, int *flag)
+int kry_ilu_factor(hipStream_t s, int scalar, int rows, int blk, int base, int cnt,
+                   const int *order, const int *rowptr, const int *colidx, const int *diagslots,
+                   double *values, double *diagvalues, int *flag)
 {
     if (cnt <= 0) return UCFD_OK;
-    return by_block<IluL>(blk, s, scalar != 0, base, cnt, order, rowptr, colidx, diagslots, values,
-                          diagvalues, flag);
+    return by_block<IluL>(blk, s, scalar != 0, rows != 0, base, cnt, order, rowptr, colidx,
+                          diagslots, values, diagvalues, flag);
 }
 
 int kry_bsgs_sweep64(hipStream_t s, int kind, int upper, int blk, int base, int cnt,
