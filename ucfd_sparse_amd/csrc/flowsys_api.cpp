@@ -54,7 +54,9 @@ struct _FlowSys {
     double *rhs = nullptr, *du = nullptr, *dup = nullptr, *diag = nullptr, *tdiag = nullptr;
     float *du32 = nullptr;
     int *rowptr_d = nullptr, *colidx_d = nullptr;
-    int *pos_sorted_d = nullptr, *pos_cell_d = nullptr;   // positions by face slot; cell of each
+    int *pos_sorted_d = nullptr;    // positions in ascending face slot (stable)
+    int *slot_ptr_d = nullptr;      // slot s's positions: pos_sorted_d[slot_ptr_d[s] .. [s+1])
+    int *sorted_cell_d = nullptr;   // the cell of each position of pos_sorted_d
     int *pos_partner_d = nullptr;   // the other position of each position's face slot, or -1
     std::vector<int> rowptr_h;      // host copy of the structure's row pointer
     // per block size nv: the most positions 64/nv consecutive rows of each
@@ -116,7 +118,8 @@ void free_device(FlowSys s)
     dfree(&s->rowptr_d);
     dfree(&s->colidx_d);
     dfree(&s->pos_sorted_d);
-    dfree(&s->pos_cell_d);
+    dfree(&s->slot_ptr_d);
+    dfree(&s->sorted_cell_d);
     dfree(&s->pos_partner_d);
     dfree(&s->lu_u);
     dfree(&s->lu_du);
@@ -162,12 +165,12 @@ int build_structure(FlowSys s, const char *fn)
         (rc = hip(hipMemcpy(s->colidx_d, ci.data(), ci.size()*sizeof(int), hipMemcpyHostToDevice),
                   fn, "hipMemcpy(colidx)")))
         return rc;
-    // positions in ascending face slot (stable), the cell of each position,
-    // for Prepare's coalesced pass over the face-major Jacobians, and each
+    // positions in ascending face slot (stable) with each slot's range, for
+    // Prepare's coalesced pass over the face-major Jacobians, and each
     // position's partner: the other position of a slot held by exactly two
     {
-        std::vector<int> cnt((size_t)s->nfaces + 1, 0), sorted((size_t)s->npos), cell((size_t)s->npos),
-            partner((size_t)s->npos, -1);
+        std::vector<int> cnt((size_t)s->nfaces + 1, 0), sorted((size_t)s->npos),
+            partner((size_t)s->npos, -1), cell((size_t)s->npos), scell((size_t)s->npos);
         for (int64_t p = 0; p < s->npos; ++p) cnt[(size_t)sl[p] + 1]++;
         for (int f = 0; f < s->nfaces; ++f) cnt[(size_t)f + 1] += cnt[f];
         for (int64_t p = 0; p < s->npos; ++p) sorted[(size_t)cnt[sl[p]]++] = (int)p;
@@ -181,15 +184,22 @@ int build_structure(FlowSys s, const char *fn)
         }
         for (int i = 0; i < n; ++i)
             for (int p = rp[i]; p < rp[i + 1]; ++p) cell[(size_t)p] = i;
+        for (int64_t k = 0; k < s->npos; ++k) scell[(size_t)k] = cell[(size_t)sorted[(size_t)k]];
+        // slot ranges: cnt[f] ends slot f's range, so it begins slot f + 1's
+        for (int64_t f = s->nfaces; f > 0; --f) cnt[(size_t)f] = cnt[(size_t)f - 1];
+        cnt[0] = 0;
         if ((rc = dalloc(&s->pos_sorted_d, (size_t)s->npos, fn)) ||
-            (rc = dalloc(&s->pos_cell_d, (size_t)s->npos, fn)) ||
+            (rc = dalloc(&s->slot_ptr_d, (size_t)s->nfaces + 1, fn)) ||
+            (rc = dalloc(&s->sorted_cell_d, (size_t)s->npos, fn)) ||
             (rc = dalloc(&s->pos_partner_d, (size_t)s->npos, fn)))
             return rc;
         if (s->npos &&
             ((rc = hip(hipMemcpy(s->pos_sorted_d, sorted.data(), sorted.size()*sizeof(int),
                                  hipMemcpyHostToDevice), fn, "hipMemcpy(pos_sorted)")) ||
-             (rc = hip(hipMemcpy(s->pos_cell_d, cell.data(), cell.size()*sizeof(int),
-                                 hipMemcpyHostToDevice), fn, "hipMemcpy(pos_cell)")) ||
+             (rc = hip(hipMemcpy(s->slot_ptr_d, cnt.data(), cnt.size()*sizeof(int),
+                                 hipMemcpyHostToDevice), fn, "hipMemcpy(slot_ptr)")) ||
+             (rc = hip(hipMemcpy(s->sorted_cell_d, scell.data(), scell.size()*sizeof(int),
+                                 hipMemcpyHostToDevice), fn, "hipMemcpy(sorted_cell)")) ||
              (rc = hip(hipMemcpy(s->pos_partner_d, partner.data(), partner.size()*sizeof(int),
                                  hipMemcpyHostToDevice), fn, "hipMemcpy(pos_partner)"))))
             return rc;
@@ -251,7 +261,7 @@ ucfd_status_t prepare(FlowSys s, bool turb, const char *fn)
     const int ijm = sg.add(jm, sizeof(double)*2*d2*(size_t)s->nfaces, UCFD_IN);
     if ((rc = sg.enter())) return UCFD_FAILED;
     rc = fs_launch_prepare(ucfd_stream(), nv, s->nlocal, s->nfaces, s->npos, s->rowptr_d,
-                           s->pos_sorted_d, s->pos_cell_d, s->pos_partner_d,
+                           s->slot_ptr_d, s->pos_sorted_d, s->sorted_cell_d, s->pos_partner_d,
                            (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
                            (const double *)sg.dev(ifa), (const double *)sg.dev(irv), dg,
                            (const double *)sg.dev(ijm), P.jp, P.fv, s->fp32 ? P.jp32 : nullptr,

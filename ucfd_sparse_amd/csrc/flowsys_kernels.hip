@@ -185,16 +185,18 @@ __global__ __launch_bounds__(FB) void k_subres(int neles, int nlocal, int nvars,
 // its positions in order, then LU; plus the packing of every position's
 // neighbour block (the sweep's jval, side ? A- : -A+, blusgs.c:297-300) and
 // fv = face_area*rcp_vol.  Two passes:
-// (1) Positions in face-slot order (sorted[k], built with the structure):
-// consecutive lanes read consecutive slots of the face-major jmat, so its
-// loads are coalesced; each lane forms its position's neighbour block j, the
-// wave transposes the blocks through LDS and writes whole blocks with
-// consecutive lanes on consecutive words.
+// (1) Face slots, one lane per slot: consecutive lanes read consecutive
+// slots of the face-major jmat and face_area (coalesced, no index load
+// before them); the slot's positions are sorted[slot_ptr[s] ..
+// slot_ptr[s+1]) and their cells sorted_cell[...] (built with the structure;
+// two positions for an interior face).  Round k forms fv and the neighbour
+// block of every slot's k-th position, the wave transposes the blocks through
+// LDS and writes whole blocks with consecutive lanes on consecutive words.
 template <int NV>
-__global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
+__global__ __launch_bounds__(64) void k_prep_pos(int64_t nfaces,
+                                                 const int *__restrict__ slot_ptr,
                                                  const int *__restrict__ sorted,
-                                                 const int *__restrict__ pos_cell,
-                                                 const int *__restrict__ slots,
+                                                 const int *__restrict__ sorted_cell,
                                                  const signed char *__restrict__ sides,
                                                  const double *__restrict__ face_area,
                                                  const double *__restrict__ rcp_vol,
@@ -205,22 +207,17 @@ __global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
     __shared__ double sb[64*D2];
     __shared__ int sp[64];
     const int lane = threadIdx.x;
-    const int64_t k0 = (int64_t)ucfd::wave_xcd_index()*64;
-    if (k0 >= npos) return;
-    const int64_t k = k0 + lane;
-    const bool act = k < npos;
-    const int nact = (int)min((int64_t)64, npos - k0);
-    int p = 0;
-    int64_t s = 0;
-    bool plus = false;
-    double fv = 0.0;
+    const int64_t s0 = (int64_t)ucfd::wave_xcd_index()*64;
+    if (s0 >= nfaces) return;
+    const int64_t s = s0 + lane;
+    const bool act = s < nfaces;
+    int b0 = 0, nr = 0;
+    double fa = 0.0;
     if (act) {
-        p = sorted[k];
-        s = slots[p];
-        plus = sides[p] == 1;
-        fv = face_area[s]*rcp_vol[pos_cell[p]];
+        b0 = slot_ptr[s];
+        nr = slot_ptr[s + 1] - b0;
+        fa = face_area[s];
     }
-    sp[lane] = p;
     const int64_t half = (int64_t)D2*nfaces;
     double a[D2], b[D2];
 #pragma unroll
@@ -228,20 +225,33 @@ __global__ __launch_bounds__(64) void k_prep_pos(int64_t npos, int64_t nfaces,
         a[q] = act ? jmat[s + (int64_t)q*nfaces] : 0.0;
         b[q] = act ? jmat[s + (int64_t)q*nfaces + half] : 0.0;
     }
-    if (act) {
-        fvp[p] = fv;
-        if (fv32) fv32[p] = (float)fv;
-    }
-    // neighbour coupling (the sweep's jval)
+    int mx = nr;
 #pragma unroll
-    for (int q = 0; q < D2; ++q) sb[lane*D2 + q] = plus ? b[q] : -a[q];
-    ucfd::wave_lds_sync();
-    for (int e = lane; e < nact*D2; e += 64) {
-        const int blk = e/D2;
-        const int64_t o = (int64_t)sp[blk]*D2 + (e - blk*D2);
-        const double j = sb[e];
-        jp[o] = j;
-        if (jp32) jp32[o] = (float)j;
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    for (int k = 0; k < mx; ++k) {
+        int p = -1;
+        bool plus = false;
+        if (k < nr) {
+            p = sorted[b0 + k];
+            plus = sides[p] == 1;
+            const double fv = fa*rcp_vol[sorted_cell[b0 + k]];
+            fvp[p] = fv;
+            if (fv32) fv32[p] = (float)fv;
+        }
+        sp[lane] = p;
+#pragma unroll
+        for (int q = 0; q < D2; ++q) sb[lane*D2 + q] = plus ? b[q] : -a[q];
+        ucfd::wave_lds_sync();
+        for (int e = lane; e < 64*D2; e += 64) {
+            const int blk = e/D2;
+            const int pp = sp[blk];
+            if (pp < 0) continue;
+            const int64_t o = (int64_t)pp*D2 + (e - blk*D2);
+            const double j = sb[e];
+            jp[o] = j;
+            if (jp32) jp32[o] = (float)j;
+        }
+        ucfd::wave_lds_sync();
     }
 }
 
@@ -766,14 +776,15 @@ __global__ __launch_bounds__(FB) void k_zero(int64_t n, double *a, double *b, fl
 
 template <int NV> struct PrepL {
     static int run(hipStream_t s, int nlocal, int64_t nfaces, int64_t npos, const int *rowptr,
-                   const int *sorted, const int *pos_cell, const int *partner, const int *slots,
+                   const int *slot_ptr, const int *sorted, const int *sorted_cell,
+                   const int *partner, const int *slots,
                    const signed char *sides, const double *face_area, const double *rcp_vol,
                    double *diag, const double *jmat, double *jp, double *fvp, float *jp32,
                    float *fv32, int *flag)
     {
-        if (npos > 0)
-            hipLaunchKernelGGL(k_prep_pos<NV>, dim3((unsigned)((npos + 63)/64)), dim3(64), 0, s,
-                               npos, nfaces, sorted, pos_cell, slots, sides, face_area, rcp_vol,
+        if (npos > 0 && nfaces > 0)
+            hipLaunchKernelGGL(k_prep_pos<NV>, dim3((unsigned)((nfaces + 63)/64)), dim3(64), 0, s,
+                               nfaces, slot_ptr, sorted, sorted_cell, sides, face_area, rcp_vol,
                                jmat, jp, fvp, jp32, fv32);
         constexpr int G = 64/NV;
         hipLaunchKernelGGL(k_prep_diag<NV>, dim3((unsigned)((nlocal + G - 1)/G)), dim3(64), 0, s,
@@ -888,14 +899,16 @@ int fs_launch_subres(hipStream_t s, int neles, int nlocal, int nvars, const int 
 }
 
 int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, int64_t npos,
-                      const int *rowptr, const int *sorted, const int *pos_cell,
-                      const int *partner, const int *slots, const signed char *sides,
+                      const int *rowptr, const int *slot_ptr, const int *sorted,
+                      const int *sorted_cell, const int *partner, const int *slots,
+                      const signed char *sides,
                       const double *face_area, const double *rcp_vol, double *diag,
                       const double *jmat, double *jp, double *fvp, float *jp32, float *fv32,
                       int *flag)
 {
     if (nlocal <= 0) return UCFD_OK;
-    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, sorted, pos_cell, partner, slots,
+    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, slot_ptr, sorted, sorted_cell,
+                        partner, slots,
                         sides, face_area, rcp_vol, diag, jmat, jp, fvp, jp32, fv32, flag);
 }
 
