@@ -238,6 +238,29 @@ def test_gmres_left_matches_oracle(K, tet3, precon):
     assert np.allclose(xg, xo, rtol=1e-6, atol=1e-9 * np.abs(xo).max())
 
 
+def test_gmres_long_restart_matches_oracle(K, tet3):
+    """One full cycle of GMRES(40) without a preconditioner: the Arnoldi
+    steps project onto 1 .. 40 basis vectors, so the projection kernels run
+    at every register width (8, 16, 32 columns) and past 32 (separate
+    passes); the solution follows the oracle's (true residual 3.5e-9 of
+    |b| = 29)."""
+    say("GMRES(40), one full cycle")
+    A, b, _ = tet3
+    m, maxiter, tol = 40, 1, 1e-30
+    xo, ito, sto, reso, histo = O.gmres(A, b, m=m, maxiter=maxiter, tol=tol, precon=False)
+    D = dev_bsr(A)
+    mat = K.SpMat.from_bsr(D)
+    s = K.Solver("gmres", A.bn * A.blk, maxiter, tol, m=m)
+    x = dev(np.zeros(A.bn * A.blk))
+    st, it, res = s.solve(K.Precon.none(), mat, x, dev(b))
+    say(f"  gpu it={it} st={st} res={res:.3e}; oracle it={ito} st={sto} res={reso:.3e}")
+    assert (it, st) == (ito, sto) == (maxiter, st)
+    assert _close_hist(s.history(), histo, it) <= 1e-6
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(b - A.to_scipy() @ xg) <= 1e-8
+    assert np.allclose(xg, xo, rtol=1e-6, atol=1e-9 * np.abs(xo).max())
+
+
 @pytest.mark.parametrize("precon", [True, False])
 def test_bicgstab_matches_oracle(K, tet3, precon):
     say(f"BiCGSTAB, precon={precon}")

@@ -54,9 +54,10 @@ struct _FlowSys {
     double *rhs = nullptr, *du = nullptr, *dup = nullptr, *diag = nullptr, *tdiag = nullptr;
     float *du32 = nullptr;
     int *rowptr_d = nullptr, *colidx_d = nullptr;
-    int *pos_sorted_d = nullptr;    // positions in ascending face slot (stable)
-    int *slot_ptr_d = nullptr;      // slot s's positions: pos_sorted_d[slot_ptr_d[s] .. [s+1])
-    int *sorted_cell_d = nullptr;   // the cell of each position of pos_sorted_d
+    // (position, its cell) pairs in ascending face slot (stable); slot s's
+    // are pairs slot_ptr_d[s] .. slot_ptr_d[s+1]-1
+    int *pos_sorted_d = nullptr;
+    int *slot_ptr_d = nullptr;
     int *pos_partner_d = nullptr;   // the other position of each position's face slot, or -1
     std::vector<int> rowptr_h;      // host copy of the structure's row pointer
     // per block size nv: the most positions 64/nv consecutive rows of each
@@ -119,7 +120,6 @@ void free_device(FlowSys s)
     dfree(&s->colidx_d);
     dfree(&s->pos_sorted_d);
     dfree(&s->slot_ptr_d);
-    dfree(&s->sorted_cell_d);
     dfree(&s->pos_partner_d);
     dfree(&s->lu_u);
     dfree(&s->lu_du);
@@ -169,28 +169,33 @@ int build_structure(FlowSys s, const char *fn)
     // Prepare's coalesced pass over the face-major Jacobians, and each
     // position's partner: the other position of a slot held by exactly two
     {
-        std::vector<int> cnt((size_t)s->nfaces + 1, 0), sorted((size_t)s->npos),
-            partner((size_t)s->npos, -1), cell((size_t)s->npos), scell((size_t)s->npos);
+        // sorted[2k], sorted[2k+1]: the k-th position in slot order and its
+        // cell, side by side so the counting sort scatters one line per position
+        std::vector<int> cnt((size_t)s->nfaces + 1, 0), sorted(2*(size_t)s->npos),
+            partner((size_t)s->npos, -1);
         for (int64_t p = 0; p < s->npos; ++p) cnt[(size_t)sl[p] + 1]++;
         for (int f = 0; f < s->nfaces; ++f) cnt[(size_t)f + 1] += cnt[f];
-        for (int64_t p = 0; p < s->npos; ++p) sorted[(size_t)cnt[sl[p]]++] = (int)p;
+        // rows cover positions 0 .. npos-1 in order (rowptr[0] = 0, nondecreasing)
+        for (int i = 0; i < n; ++i)
+            for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                const int k = cnt[sl[p]]++;
+                sorted[2*(size_t)k] = p;
+                sorted[2*(size_t)k + 1] = i;
+            }
         // cnt[f] is now the end of slot f's range
         for (int f = 0; f < s->nfaces; ++f) {
             const int e = cnt[f], b = f ? cnt[f - 1] : 0;
             if (e - b == 2) {
-                partner[(size_t)sorted[b]] = sorted[b + 1];
-                partner[(size_t)sorted[b + 1]] = sorted[b];
+                const int pa = sorted[2*(size_t)b], pb = sorted[2*(size_t)b + 2];
+                partner[(size_t)pa] = pb;
+                partner[(size_t)pb] = pa;
             }
         }
-        for (int i = 0; i < n; ++i)
-            for (int p = rp[i]; p < rp[i + 1]; ++p) cell[(size_t)p] = i;
-        for (int64_t k = 0; k < s->npos; ++k) scell[(size_t)k] = cell[(size_t)sorted[(size_t)k]];
         // slot ranges: cnt[f] ends slot f's range, so it begins slot f + 1's
         for (int64_t f = s->nfaces; f > 0; --f) cnt[(size_t)f] = cnt[(size_t)f - 1];
         cnt[0] = 0;
-        if ((rc = dalloc(&s->pos_sorted_d, (size_t)s->npos, fn)) ||
+        if ((rc = dalloc(&s->pos_sorted_d, 2*(size_t)s->npos, fn)) ||
             (rc = dalloc(&s->slot_ptr_d, (size_t)s->nfaces + 1, fn)) ||
-            (rc = dalloc(&s->sorted_cell_d, (size_t)s->npos, fn)) ||
             (rc = dalloc(&s->pos_partner_d, (size_t)s->npos, fn)))
             return rc;
         if (s->npos &&
@@ -198,8 +203,6 @@ int build_structure(FlowSys s, const char *fn)
                                  hipMemcpyHostToDevice), fn, "hipMemcpy(pos_sorted)")) ||
              (rc = hip(hipMemcpy(s->slot_ptr_d, cnt.data(), cnt.size()*sizeof(int),
                                  hipMemcpyHostToDevice), fn, "hipMemcpy(slot_ptr)")) ||
-             (rc = hip(hipMemcpy(s->sorted_cell_d, scell.data(), scell.size()*sizeof(int),
-                                 hipMemcpyHostToDevice), fn, "hipMemcpy(sorted_cell)")) ||
              (rc = hip(hipMemcpy(s->pos_partner_d, partner.data(), partner.size()*sizeof(int),
                                  hipMemcpyHostToDevice), fn, "hipMemcpy(pos_partner)"))))
             return rc;
@@ -261,7 +264,7 @@ ucfd_status_t prepare(FlowSys s, bool turb, const char *fn)
     const int ijm = sg.add(jm, sizeof(double)*2*d2*(size_t)s->nfaces, UCFD_IN);
     if ((rc = sg.enter())) return UCFD_FAILED;
     rc = fs_launch_prepare(ucfd_stream(), nv, s->nlocal, s->nfaces, s->npos, s->rowptr_d,
-                           s->slot_ptr_d, s->pos_sorted_d, s->sorted_cell_d, s->pos_partner_d,
+                           s->slot_ptr_d, (const int2 *)s->pos_sorted_d, s->pos_partner_d,
                            (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
                            (const double *)sg.dev(ifa), (const double *)sg.dev(irv), dg,
                            (const double *)sg.dev(ijm), P.jp, P.fv, s->fp32 ? P.jp32 : nullptr,

@@ -24,13 +24,13 @@ UCFD_HIDDEN int fs_launch_subres(hipStream_t s, int neles, int nlocal, int nvars
 // diag += sum over positions of the signed face Jacobian times fv, then LU;
 // also writes every position's neighbour block jp[p][nv][nv] and fv (and
 // their FP32 copies when jp32 != NULL).  sorted = the positions in
-// ascending face slot (stable), slot s's being sorted[slot_ptr[s] ..
-// slot_ptr[s+1]), sorted_cell[k] = the cell of position sorted[k], partner =
-// the other position of the same face slot (-1 when the slot has not exactly
+// ascending face slot (stable) with their cells, as (position, cell)
+// pairs, slot s's being sorted[slot_ptr[s] .. slot_ptr[s+1]); partner = the
+// other position of the same face slot (-1 when the slot has not exactly
 // two).  A zero pivot raises UCFD_KFLAG_PIVOT.
 UCFD_HIDDEN int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, int64_t npos,
-                                  const int *rowptr, const int *slot_ptr, const int *sorted,
-                                  const int *sorted_cell, const int *partner, const int *slots, const signed char *sides,
+                                  const int *rowptr, const int *slot_ptr, const int2 *sorted,
+                                  const int *partner, const int *slots, const signed char *sides,
                                   const double *face_area, const double *rcp_vol, double *diag,
                                   const double *jmat, double *jp, double *fvp, float *jp32,
                                   float *fv32, int *flag);

@@ -187,16 +187,14 @@ __global__ __launch_bounds__(FB) void k_subres(int neles, int nlocal, int nvars,
 // fv = face_area*rcp_vol.  Two passes:
 // (1) Face slots, one lane per slot: consecutive lanes read consecutive
 // slots of the face-major jmat and face_area (coalesced, no index load
-// before them); the slot's positions are sorted[slot_ptr[s] ..
-// slot_ptr[s+1]) and their cells sorted_cell[...] (built with the structure;
-// two positions for an interior face).  Round k forms fv and the neighbour
+// before them); the slot's (position, cell) pairs are sorted[slot_ptr[s] ..
+// slot_ptr[s+1]) (built with the structure; two for an interior face).  Round k forms fv and the neighbour
 // block of every slot's k-th position, the wave transposes the blocks through
 // LDS and writes whole blocks with consecutive lanes on consecutive words.
 template <int NV>
 __global__ __launch_bounds__(64) void k_prep_pos(int64_t nfaces,
                                                  const int *__restrict__ slot_ptr,
-                                                 const int *__restrict__ sorted,
-                                                 const int *__restrict__ sorted_cell,
+                                                 const int2 *__restrict__ sorted,
                                                  const signed char *__restrict__ sides,
                                                  const double *__restrict__ face_area,
                                                  const double *__restrict__ rcp_vol,
@@ -232,9 +230,10 @@ __global__ __launch_bounds__(64) void k_prep_pos(int64_t nfaces,
         int p = -1;
         bool plus = false;
         if (k < nr) {
-            p = sorted[b0 + k];
+            const int2 pc = sorted[b0 + k];
+            p = pc.x;
             plus = sides[p] == 1;
-            const double fv = fa*rcp_vol[sorted_cell[b0 + k]];
+            const double fv = fa*rcp_vol[pc.y];
             fvp[p] = fv;
             if (fv32) fv32[p] = (float)fv;
         }
@@ -776,15 +775,15 @@ __global__ __launch_bounds__(FB) void k_zero(int64_t n, double *a, double *b, fl
 
 template <int NV> struct PrepL {
     static int run(hipStream_t s, int nlocal, int64_t nfaces, int64_t npos, const int *rowptr,
-                   const int *slot_ptr, const int *sorted, const int *sorted_cell,
-                   const int *partner, const int *slots,
+                   const int *slot_ptr, const int2 *sorted, const int *partner,
+                   const int *slots,
                    const signed char *sides, const double *face_area, const double *rcp_vol,
                    double *diag, const double *jmat, double *jp, double *fvp, float *jp32,
                    float *fv32, int *flag)
     {
         if (npos > 0 && nfaces > 0)
             hipLaunchKernelGGL(k_prep_pos<NV>, dim3((unsigned)((nfaces + 63)/64)), dim3(64), 0, s,
-                               nfaces, slot_ptr, sorted, sorted_cell, sides, face_area, rcp_vol,
+                               nfaces, slot_ptr, sorted, sides, face_area, rcp_vol,
                                jmat, jp, fvp, jp32, fv32);
         constexpr int G = 64/NV;
         hipLaunchKernelGGL(k_prep_diag<NV>, dim3((unsigned)((nlocal + G - 1)/G)), dim3(64), 0, s,
@@ -899,16 +898,14 @@ int fs_launch_subres(hipStream_t s, int neles, int nlocal, int nvars, const int 
 }
 
 int fs_launch_prepare(hipStream_t s, int nv, int nlocal, int64_t nfaces, int64_t npos,
-                      const int *rowptr, const int *slot_ptr, const int *sorted,
-                      const int *sorted_cell, const int *partner, const int *slots,
-                      const signed char *sides,
+                      const int *rowptr, const int *slot_ptr, const int2 *sorted,
+                      const int *partner, const int *slots, const signed char *sides,
                       const double *face_area, const double *rcp_vol, double *diag,
                       const double *jmat, double *jp, double *fvp, float *jp32, float *fv32,
                       int *flag)
 {
     if (nlocal <= 0) return UCFD_OK;
-    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, slot_ptr, sorted, sorted_cell,
-                        partner, slots,
+    return by_nv<PrepL>(nv, s, nlocal, nfaces, npos, rowptr, slot_ptr, sorted, partner, slots,
                         sides, face_area, rcp_vol, diag, jmat, jp, fvp, jp32, fv32, flag);
 }
 

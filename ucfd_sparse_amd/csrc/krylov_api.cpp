@@ -460,10 +460,10 @@ int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, doubl
                 if ((rc = spmv(s, A, ma, 1.0, vj, 0.0, w)) || (rc = pc_apply(s, pc, mp, w)))
                     return rc;
             }
-            // CGS2 (gmres.c:20-29)
+            // CGS2 (gmres.c:20-29); the first update and the second projection
+            // share one pass over V
             if ((rc = kry_gemvt(s, N, kk, V, N, w, part, db + DB_H)) ||
-                (rc = kry_gemv(s, N, kk, V, N, -1.0, db + DB_H, w, part, nullptr)) ||
-                (rc = kry_gemvt(s, N, kk, V, N, w, part, db + DB_H2)) ||
+                (rc = kry_gemv_gemvt(s, N, kk, V, N, -1.0, db + DB_H, w, part, db + DB_H2)) ||
                 (rc = kry_gemv(s, N, kk, V, N, -1.0, db + DB_H2, w, part, db + DB_NSQ)) ||
                 (rc = kry_scale_by_inv_sqrt(s, N, w, db + DB_NSQ, V + (size_t)(j + 1)*n)) ||
                 (rc = d2h(S, 0, DB_NSQ + 1, s, fn)))

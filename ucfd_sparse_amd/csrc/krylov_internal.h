@@ -81,6 +81,11 @@ UCFD_HIDDEN int kry_gemvt(hipStream_t s, int64_t n, int k, const double *V, int6
 UCFD_HIDDEN int kry_gemv(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv,
                          double alpha, const double *h, double *w, double *part,
                          double *normsq);
+// kry_gemv (no norm) followed by kry_gemvt on the updated w, in one pass over
+// V when k <= 32: w[i] += alpha * sum_j V[j*ldv + i] h[j]; out[j] = V_j . w
+UCFD_HIDDEN int kry_gemv_gemvt(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv,
+                               double alpha, const double *h, double *w, double *part,
+                               double *out);
 // BiCGSTAB fused reduction (bicgstab.c:5-22): out[0..4] = (rt.s, rt.t, t.s, t.t, s.s)
 UCFD_HIDDEN int kry_dot5(hipStream_t s, int64_t n, const double *rt, const double *sv,
                          const double *t, double *part, double *out);

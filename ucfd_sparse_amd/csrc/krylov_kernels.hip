@@ -793,6 +793,73 @@ __global__ __launch_bounds__(KB) void k_gemv(int64_t n, int k, const double *__r
     }
 }
 
+// partial dots of w with all k <= KM columns of V in one pass over w (each
+// column's sum runs over the same elements in the same order as k_gemvt8's)
+template <int KM>
+__global__ __launch_bounds__(KB) void k_gemvt_all(int64_t n, int k, const double *__restrict__ V,
+                                                  int64_t ldv, const double *__restrict__ w,
+                                                  double *part)
+{
+    double acc[KM];
+#pragma unroll
+    for (int q = 0; q < KM; ++q) acc[q] = 0.0;
+    const int64_t st = (int64_t)gridDim.x*KB;
+    for (int64_t i = (int64_t)blockIdx.x*KB + threadIdx.x; i < n; i += st) {
+        // the row of V first, every load issued before the first product
+        double vr[KM];
+#pragma unroll
+        for (int q = 0; q < KM; ++q) vr[q] = q < k ? V[(int64_t)q*ldv + i] : 0.0;
+        const double wi = w[i];
+#pragma unroll
+        for (int q = 0; q < KM; ++q)
+            if (q < k) acc[q] += vr[q]*wi;
+    }
+#pragma unroll
+    for (int q = 0; q < KM; ++q)
+        if (q < k) {
+            const double r = block_reduce(acc[q]);
+            if (threadIdx.x == 0) part[(int64_t)q*KMAXPART + blockIdx.x] = r;
+        }
+}
+
+// The middle of a CGS2 step in one pass over V: w_i += alpha*sum_j V_ji h_j
+// (k_gemv's arithmetic), then partial dots of the new w with the same k <= KM
+// columns (k_gemvt_all's), the row of V held in registers between the two.
+template <int KM>
+__global__ __launch_bounds__(KB) void k_gemv_gemvt(int64_t n, int k,
+                                                   const double *__restrict__ V, int64_t ldv,
+                                                   double alpha, const double *__restrict__ h,
+                                                   double *__restrict__ w, double *part)
+{
+    __shared__ double hs[KM];
+    if (threadIdx.x < k) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    double acc[KM];
+#pragma unroll
+    for (int q = 0; q < KM; ++q) acc[q] = 0.0;
+    const int64_t st = (int64_t)gridDim.x*KB;
+    for (int64_t i = (int64_t)blockIdx.x*KB + threadIdx.x; i < n; i += st) {
+        double vr[KM];
+#pragma unroll
+        for (int q = 0; q < KM; ++q) vr[q] = q < k ? V[(int64_t)q*ldv + i] : 0.0;
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < KM; ++q)
+            if (q < k) sum += vr[q]*hs[q];
+        const double v = alpha*sum + 1.0*w[i];
+        w[i] = v;
+#pragma unroll
+        for (int q = 0; q < KM; ++q)
+            if (q < k) acc[q] += vr[q]*v;
+    }
+#pragma unroll
+    for (int q = 0; q < KM; ++q)
+        if (q < k) {
+            const double r = block_reduce(acc[q]);
+            if (threadIdx.x == 0) part[(int64_t)q*KMAXPART + blockIdx.x] = r;
+        }
+}
+
 __global__ __launch_bounds__(KB) void k_scale_isqrt(int64_t n, const double *__restrict__ src,
                                                     const double *__restrict__ nsq,
                                                     double *__restrict__ dst)
@@ -1097,8 +1164,37 @@ int kry_gemvt(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv, con
 {
     if (k <= 0 || k > 64) return UCFD_ERR_ARG;
     const int nb = nblocks(n);
-    for (int j0 = 0; j0 < k; j0 += 8)
-        hipLaunchKernelGGL(k_gemvt8, dim3(nb), dim3(KB), 0, s, n, k, j0, V, ldv, w, part);
+    if (k <= 8)
+        hipLaunchKernelGGL(k_gemvt_all<8>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, w, part);
+    else if (k <= 16)
+        hipLaunchKernelGGL(k_gemvt_all<16>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, w, part);
+    else if (k <= 32)
+        hipLaunchKernelGGL(k_gemvt_all<32>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, w, part);
+    else
+        for (int j0 = 0; j0 < k; j0 += 8)
+            hipLaunchKernelGGL(k_gemvt8, dim3(nb), dim3(KB), 0, s, n, k, j0, V, ldv, w, part);
+    hipLaunchKernelGGL(k_sum_rows, dim3(k), dim3(1024), 0, s, part, nb, KMAXPART, out);
+    return ok();
+}
+
+int kry_gemv_gemvt(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv, double alpha,
+                   const double *h, double *w, double *part, double *out)
+{
+    if (k <= 0 || k > 64) return UCFD_ERR_ARG;
+    if (k > 32) {
+        int rc = kry_gemv(s, n, k, V, ldv, alpha, h, w, part, nullptr);
+        return rc ? rc : kry_gemvt(s, n, k, V, ldv, w, part, out);
+    }
+    const int nb = nblocks(n);
+    if (k <= 8)
+        hipLaunchKernelGGL(k_gemv_gemvt<8>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w,
+                           part);
+    else if (k <= 16)
+        hipLaunchKernelGGL(k_gemv_gemvt<16>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w,
+                           part);
+    else
+        hipLaunchKernelGGL(k_gemv_gemvt<32>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w,
+                           part);
     hipLaunchKernelGGL(k_sum_rows, dim3(k), dim3(1024), 0, s, part, nb, KMAXPART, out);
     return ok();
 }
