@@ -221,6 +221,32 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
     }
 }
 
+// A colour range where no face can be taken in range mode: the first colour
+// of a lower sweep (n0 == 0: no neighbour id lies below it) and the last of
+// an upper sweep (the range ends at N).  The face sum is empty, so the
+// neighbour indices are not needed; the update keeps k_sweep's expressions
+// with df = 0 (so -0.0 inputs and a zero diagonal give the same bits).
+template <int DIR, int NV>
+__global__ __launch_bounds__(kBlock) void k_sweep_none(int n0, int cnt, int N,
+                                                       const double *in, double *out,
+                                                       const double *__restrict__ diag)
+{
+    const int t = blockIdx.x*kBlock + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = n0 + t;
+    const double dg = ld_stream(diag + i);
+    const double df = 0.0;
+    double xin[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) xin[k] = ld_stream(in + (size_t)k*N + i);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const size_t o = (size_t)k*N + i;
+        if (DIR == 0) __builtin_nontemporal_store((xin[k] - 0.5*df)/dg, out + o);
+        else          __builtin_nontemporal_store(xin[k] - (0.5*df/dg), out + o);
+    }
+}
+
 // Any other face count: one face at a time.
 template <int DIR, int MODE, int ND>
 __global__ __launch_bounds__(kBlock) void k_sweep_any(
@@ -363,6 +389,19 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
     if (dir == 1 && mode == UCFD_SWEEP_COLORED)
         return launch_sweep_d<1, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
                                     vec_fnorm, upts, in, out, diag, fspr, flag);
+    // a range with no neighbour in the sweep direction: nothing to gather
+    if (mode == UCFD_SWEEP_RANGE && (dir == 0 ? n0 == 0 : ne == neles)) {
+        const dim3 g((cnt + kBlock - 1)/kBlock), b(kBlock);
+        if (dir == 0 && ndims == 3)
+            hipLaunchKernelGGL((k_sweep_none<0, 5>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+        else if (dir == 0)
+            hipLaunchKernelGGL((k_sweep_none<0, 4>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+        else if (ndims == 3)
+            hipLaunchKernelGGL((k_sweep_none<1, 5>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+        else
+            hipLaunchKernelGGL((k_sweep_none<1, 4>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+        return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
+    }
     // a whole colour with every neighbour in the sweep direction: the last
     // colour of a lower sweep, the first of an upper sweep (tets and hexes)
     const bool all_taken = mode == UCFD_SWEEP_RANGE && (nface == 4 || nface == 6) &&
