@@ -793,6 +793,38 @@ __global__ __launch_bounds__(KB) void k_gemv(int64_t n, int k, const double *__r
     }
 }
 
+// k_gemv for k <= KM columns with the row of V loaded into registers first
+// (every load issued before the first product; the same sums in the same
+// order)
+template <int KM>
+__global__ __launch_bounds__(KB) void k_gemv_r(int64_t n, int k, const double *__restrict__ V,
+                                               int64_t ldv, double alpha,
+                                               const double *__restrict__ h,
+                                               double *__restrict__ w, double *part)
+{
+    __shared__ double hs[KM];
+    if (threadIdx.x < k) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    double ss = 0.0;
+    const int64_t st = (int64_t)gridDim.x*KB;
+    for (int64_t i = (int64_t)blockIdx.x*KB + threadIdx.x; i < n; i += st) {
+        double vr[KM];
+#pragma unroll
+        for (int q = 0; q < KM; ++q) vr[q] = q < k ? V[(int64_t)q*ldv + i] : 0.0;
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < KM; ++q)
+            if (q < k) sum += vr[q]*hs[q];
+        const double v = alpha*sum + 1.0*w[i];
+        w[i] = v;
+        ss += v*v;
+    }
+    if (part) {
+        ss = block_reduce(ss);
+        if (threadIdx.x == 0) part[blockIdx.x] = ss;
+    }
+}
+
 // partial dots of w with all k <= KM columns of V in one pass over w (each
 // column's sum runs over the same elements in the same order as k_gemvt8's)
 template <int KM>
@@ -1204,8 +1236,15 @@ int kry_gemv(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv, doub
 {
     if (k <= 0 || k > 64) return UCFD_ERR_ARG;
     const int nb = nblocks(n);
-    hipLaunchKernelGGL(k_gemv, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w,
-                       normsq ? part : nullptr);
+    double *pp = normsq ? part : nullptr;
+    if (k <= 8)
+        hipLaunchKernelGGL(k_gemv_r<8>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w, pp);
+    else if (k <= 16)
+        hipLaunchKernelGGL(k_gemv_r<16>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w, pp);
+    else if (k <= 32)
+        hipLaunchKernelGGL(k_gemv_r<32>, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w, pp);
+    else
+        hipLaunchKernelGGL(k_gemv, dim3(nb), dim3(KB), 0, s, n, k, V, ldv, alpha, h, w, pp);
     if (normsq) hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(1024), 0, s, part, nb, KMAXPART, normsq);
     return ok();
 }

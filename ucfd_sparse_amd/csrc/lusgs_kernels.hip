@@ -226,24 +226,37 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
 // an upper sweep (the range ends at N).  The face sum is empty, so the
 // neighbour indices are not needed; the update keeps k_sweep's expressions
 // with df = 0 (so -0.0 inputs and a zero diagonal give the same bits).
-template <int DIR, int NV>
+// CPT cells per thread (t, t + S, ...), every load issued before the first
+// store.
+template <int DIR, int NV, int CPT>
 __global__ __launch_bounds__(kBlock) void k_sweep_none(int n0, int cnt, int N,
                                                        const double *in, double *out,
                                                        const double *__restrict__ diag)
 {
-    const int t = blockIdx.x*kBlock + threadIdx.x;
-    if (t >= cnt) return;
-    const int i = n0 + t;
-    const double dg = ld_stream(diag + i);
+    const int S = gridDim.x*kBlock;
+    const int t0 = blockIdx.x*kBlock + threadIdx.x;
     const double df = 0.0;
-    double xin[NV];
+    double dg[CPT], xin[CPT][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) xin[k] = ld_stream(in + (size_t)k*N + i);
+    for (int c = 0; c < CPT; ++c) {
+        const int t = t0 + c*S;
+        if (t >= cnt) continue;
+        const int i = n0 + t;
+        dg[c] = ld_stream(diag + i);
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const size_t o = (size_t)k*N + i;
-        if (DIR == 0) __builtin_nontemporal_store((xin[k] - 0.5*df)/dg, out + o);
-        else          __builtin_nontemporal_store(xin[k] - (0.5*df/dg), out + o);
+        for (int k = 0; k < NV; ++k) xin[c][k] = ld_stream(in + (size_t)k*N + i);
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int t = t0 + c*S;
+        if (t >= cnt) continue;
+        const int i = n0 + t;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const size_t o = (size_t)k*N + i;
+            if (DIR == 0) __builtin_nontemporal_store((xin[c][k] - 0.5*df)/dg[c], out + o);
+            else          __builtin_nontemporal_store(xin[c][k] - (0.5*df/dg[c]), out + o);
+        }
     }
 }
 
@@ -391,15 +404,16 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
                                     vec_fnorm, upts, in, out, diag, fspr, flag);
     // a range with no neighbour in the sweep direction: nothing to gather
     if (mode == UCFD_SWEEP_RANGE && (dir == 0 ? n0 == 0 : ne == neles)) {
-        const dim3 g((cnt + kBlock - 1)/kBlock), b(kBlock);
+        constexpr int CPT = 2;
+        const dim3 g((cnt + CPT*kBlock - 1)/(CPT*kBlock)), b(kBlock);
         if (dir == 0 && ndims == 3)
-            hipLaunchKernelGGL((k_sweep_none<0, 5>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<0, 5, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
         else if (dir == 0)
-            hipLaunchKernelGGL((k_sweep_none<0, 4>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<0, 4, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
         else if (ndims == 3)
-            hipLaunchKernelGGL((k_sweep_none<1, 5>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<1, 5, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
         else
-            hipLaunchKernelGGL((k_sweep_none<1, 4>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<1, 4, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
         return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
     }
     // a whole colour with every neighbour in the sweep direction: the last
