@@ -758,6 +758,10 @@ def main():
                          "(ucfd_greedy_coloring_ordered, -reduced: then Kempe-chain "
                          "reduction), first-fit in breadth-first order (ucfd_bfs_order) "
                          "or balanced")
+    ap.add_argument("--cell-order", choices=("hex", "tet-type"), default="hex",
+                    help="numbering inside each colour of the headline mesh: along the "
+                         "Hilbert curve of the cubes, a cube's tets together (hex), or "
+                         "tet index first, then the curve (tet-type)")
     ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint", "blusgs"),
                     default="lusgs",
                     help="lusgs: the headline colored LU-SGS line (default); krylov: the "
@@ -814,7 +818,7 @@ def main():
     t0 = time.time()
     mesh = M.tet_mesh(args.m, seed=args.seed)
     flow = M.flow_fields(mesh, seed=args.seed)
-    mesh.key = M.hilbert_keys(mesh, "tet", (args.m,))
+    mesh.key = M.hilbert_keys(mesh, "tet", (args.m,), tet_type_major=args.cell_order == "tet-type")
     lay = colored_layout(mesh, flow, coloring=args.coloring)
     del mesh, flow
     N, F = lay.mesh.neles, lay.mesh.nface
@@ -956,8 +960,9 @@ def main():
         "config": {
             "workload": f"colored LU-SGS FP64 legacy API pass (pre + lower + upper + update), "
                         f"{args.m}^3 hex x 6 Kuhn tets, permuted, {args.coloring} coloured, "
-                        f"colour-contiguous Hilbert-ordered layout",
-            "coloring": args.coloring, "colour_sizes": [int(x) for x in np.diff(lay.off)],
+                        f"colour-contiguous Hilbert-ordered layout ({args.cell_order} order)",
+            "coloring": args.coloring, "cell_order": args.cell_order,
+            "colour_sizes": [int(x) for x in np.diff(lay.off)],
             "neles": N, "nface": F, "ncolors": nc, "seed": args.seed,
             "parallelism": "single GPU",
         },
