@@ -390,6 +390,39 @@ def test_two_dimensional_quads():
     assert_close(a[3], ou, "2-D upts")
 
 
+def test_two_dimensional_declared_layout_bitwise():
+    """NDIMS = 2 through a declared colour-contiguous layout: the range-mode
+    sweeps (the all-taken variant, and the ranges with no face in the sweep
+    direction, which skip the neighbour indices) equal the generic colour
+    test bit for bit, in both array forms, and follow the oracle."""
+    h = M.hex_mesh(9, 7, 1, seed=6)
+    m = M.Mesh(np.ascontiguousarray(h.nei_ele[:4]), np.ascontiguousarray(h.fnorm_vol[:4]),
+               np.ascontiguousarray(h.vec_fnorm[:4, :2]), h.orig_id, h.key)
+    flow = M.flow_fields(m, seed=6)
+    lcolor, _ = greedy_coloring_py(m.nei_ele)
+    m2, f2, lc2, ic2, _ = reorder(m, flow, lcolor)
+    say(f"2-D declared layout: {m2.neles} quads, {int(lc2.max())} colours")
+    L.call("ucfd_set_config", 4, 4, 2)
+    try:
+        cv = torch_dev()
+        lc_d, ic_d = cv(lc2), cv(ic2)
+        generic = colored_pass_lib(m2, f2, lc_d, ic_d, cv)
+        L.call("ucfd_declare_colored_layout", m2.neles, ic_d, lc_d, None)
+        try:
+            fast = colored_pass_lib(m2, f2, lc_d, ic_d, cv)
+            fast2 = colored_pass_lib(m2, f2, lc_d, ic_d, cv, two_array=True)
+        finally:
+            L.call("ucfd_release_colored_layout", ic_d, lc_d)
+    finally:
+        L.call("ucfd_set_config", 5, 5, 3)
+    for x, y, w in zip(generic, fast, fast2):
+        assert np.array_equal(x, y) and np.array_equal(x, w)
+    od, os_, oq, ou = O.colored_pass(m2.nei_ele, ic2, lc2, m2.fnorm_vol, m2.vec_fnorm,
+                                     f2.fspr, f2.dt, f2.upts, f2.rhsb)
+    assert_close(fast[2], oq, "2-D declared-layout dQ")
+    assert_close(fast[3], ou, "2-D declared-layout upts")
+
+
 def test_c_abi_program_matches_oracle():
     exe = os.path.join(REPO, "tests", "abi", "legacy_calls.x")
     if not os.path.exists(exe):
