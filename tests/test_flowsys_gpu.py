@@ -68,25 +68,33 @@ def test_stages_bitwise(where, colored, kind):
         assert np.array_equal(g, v), k
 
 
-def split_slots(fs, tj, seed, nsplit, nsame):
+def split_slots(fs, tj, seed, nsplit, nsame, nmerge=0, nempty=0):
     """The same system with nsplit faces split into two one-position slots
     (each with a copy of the face's Jacobians, area and normal: boundary-like
-    slots) and nsame faces whose two positions carry the same side."""
+    slots), nsame faces whose two positions carry the same side, nmerge
+    faces whose second position moves into another face's slot (a slot of
+    three positions and one of one), and nempty extra slots that no position
+    names."""
     from ucfd_sparse_amd.mesh import FlowSysMesh
     rng = np.random.default_rng(seed)
     slots, sides = fs.slots.copy(), fs.sides.copy()
     order = np.argsort(slots, kind="stable")
     pairs = order.reshape(-1, 2)            # every generated slot holds two positions
-    pick = rng.choice(len(pairs), size=nsplit + nsame, replace=False)
+    pick = rng.choice(len(pairs), size=nsplit + nsame + 2*nmerge, replace=False)
     ns = fs.nfaces
     new_f = []
     for j, k in enumerate(pick[:nsplit]):
         p, q = pairs[k]
         new_f.append(slots[q])
         slots[q] = ns + j
-    for k in pick[nsplit:]:
+    for k in pick[nsplit:nsplit + nsame]:
         p, q = pairs[k]
         sides[q] = sides[p]
+    m = pick[nsplit + nsame:]
+    for k1, k2 in zip(m[0::2], m[1::2]):
+        slots[pairs[k2][1]] = slots[pairs[k1][0]]
+    new_f += [int(f) for f in rng.choice(ns, size=nempty)]
+    nsplit += nempty
     new_f = np.array(new_f, np.int64)
     jm = np.concatenate([fs.jmat, fs.jmat[..., new_f]], axis=-1)
     tjm = np.concatenate([tj, tj[..., new_f]], axis=-1)
@@ -100,12 +108,13 @@ def split_slots(fs, tj, seed, nsplit, nsame):
 def test_stages_bitwise_with_unpaired_slots(colored):
     """Prepare takes a position's diagonal contribution from its partner's
     packed block when the slot holds exactly two positions of opposite sides,
-    and from jmat otherwise: slots with one position and a pair with equal
-    sides exercise the second path."""
+    and from jmat otherwise: slots with one position, a pair with equal
+    sides and slots of three positions exercise the second path; slots that
+    no position names are skipped by the slot-ordered packing."""
     from ucfd_sparse_amd import flowsys as FS
     say(f"unpaired and same-side slots, colour-ordered={colored}")
     fs, tj, el = case(4, 73, colored, 6)
-    fs, tj = split_slots(fs, tj, 74, nsplit=40, nsame=3)
+    fs, tj = split_slots(fs, tj, 74, nsplit=40, nsame=3, nmerge=5, nempty=7)
     ref = O.flowsys_run(fs, tj, el, 6, 5, kind="sa", iterations=2)
     fsd, tjd, eld = on_device(fs, tj, el)
     S = FS.FlowSys(fsd, 6, 5, tjmat=tjd)
