@@ -444,16 +444,19 @@ int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, doubl
             beta = sqrt(hb[nsq_slot]);
         }
         y[0] = beta;
-        if ((rc = kry_scale_by_inv_sqrt(s, N, r, db + nsq_slot, V))) return rc;
+        // right mode: z_0 starts as a copy of v_0 (M^-1 is applied in place)
+        if ((rc = kry_scale_by_inv_sqrt(s, N, r, db + nsq_slot, V, S->right ? S->Z : nullptr)))
+            return rc;
         k = 0;
         bool happy = false;
         for (int j = 0; j < m; ++j) {
             const int kk = j + 1;
             double *vj = V + (size_t)j*n;
             if (S->right) {
-                // flexible form: z_j = M^-1 v_j is kept, x is updated with Z y
+                // flexible form: z_j = M^-1 v_j is kept, x is updated with Z y;
+                // z_j holds a copy of v_j written with it
                 double *zj = S->Z + (size_t)j*n;
-                if ((rc = kry_copy(s, N, vj, zj)) || (rc = pc_apply(s, pc, mp, zj)) ||
+                if ((rc = pc_apply(s, pc, mp, zj)) ||
                     (rc = spmv(s, A, ma, 1.0, zj, 0.0, w)))
                     return rc;
             } else {
@@ -465,7 +468,9 @@ int gmres(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, doubl
             if ((rc = kry_gemvt(s, N, kk, V, N, w, part, db + DB_H)) ||
                 (rc = kry_gemv_gemvt(s, N, kk, V, N, -1.0, db + DB_H, w, part, db + DB_H2)) ||
                 (rc = kry_gemv(s, N, kk, V, N, -1.0, db + DB_H2, w, part, db + DB_NSQ)) ||
-                (rc = kry_scale_by_inv_sqrt(s, N, w, db + DB_NSQ, V + (size_t)(j + 1)*n)) ||
+                (rc = kry_scale_by_inv_sqrt(s, N, w, db + DB_NSQ, V + (size_t)(j + 1)*n,
+                                            S->right && j + 1 < m ? S->Z + (size_t)(j + 1)*n
+                                                                  : nullptr)) ||
                 (rc = d2h(S, 0, DB_NSQ + 1, s, fn)))
                 return rc;
             double *Hcol = H.data() + (size_t)j*ld;

@@ -71,8 +71,10 @@ UCFD_HIDDEN int kry_dot(hipStream_t s, int64_t n, const double *x, const double 
 UCFD_HIDDEN int kry_axpy(hipStream_t s, int64_t n, double a, const double *x, double *y);
 UCFD_HIDDEN int kry_axpby(hipStream_t s, int64_t n, double a, const double *x, double b,
                           double *y);  // y = a x + b y
+// dst = src * (1/sqrt(*normsq)); the same values also into dst2 when given
 UCFD_HIDDEN int kry_scale_by_inv_sqrt(hipStream_t s, int64_t n, const double *src,
-                                      const double *normsq, double *dst);   // dst = src/sqrt(*normsq)
+                                      const double *normsq, double *dst,
+                                      double *dst2 = nullptr);
 // h[j] = sum_i V[j*ldv + i] w[i], j < k (k <= 64): out[j] (device)
 UCFD_HIDDEN int kry_gemvt(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv,
                           const double *w, double *part, double *out);

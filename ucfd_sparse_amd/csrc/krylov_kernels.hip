@@ -894,11 +894,16 @@ __global__ __launch_bounds__(KB) void k_gemv_gemvt(int64_t n, int k,
 
 __global__ __launch_bounds__(KB) void k_scale_isqrt(int64_t n, const double *__restrict__ src,
                                                     const double *__restrict__ nsq,
-                                                    double *__restrict__ dst)
+                                                    double *__restrict__ dst,
+                                                    double *__restrict__ dst2)
 {
     const double inv = 1.0/sqrt(*nsq);
     const int64_t st = (int64_t)gridDim.x*KB;
-    for (int64_t i = (int64_t)blockIdx.x*KB + threadIdx.x; i < n; i += st) dst[i] = inv*src[i];
+    for (int64_t i = (int64_t)blockIdx.x*KB + threadIdx.x; i < n; i += st) {
+        const double v = inv*src[i];
+        dst[i] = v;
+        if (dst2) dst2[i] = v;
+    }
 }
 
 __global__ __launch_bounds__(KB) void k_axpby(int64_t n, double a, const double *__restrict__ x,
@@ -1250,9 +1255,9 @@ int kry_gemv(hipStream_t s, int64_t n, int k, const double *V, int64_t ldv, doub
 }
 
 int kry_scale_by_inv_sqrt(hipStream_t s, int64_t n, const double *src, const double *normsq,
-                          double *dst)
+                          double *dst, double *dst2)
 {
-    hipLaunchKernelGGL(k_scale_isqrt, dim3(nblocks(n)), dim3(KB), 0, s, n, src, normsq, dst);
+    hipLaunchKernelGGL(k_scale_isqrt, dim3(nblocks(n)), dim3(KB), 0, s, n, src, normsq, dst, dst2);
     return ok();
 }
 
