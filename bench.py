@@ -930,7 +930,8 @@ def main():
     tr = read_traffic(os.path.join(REPO, "profiles", "traffic_lower_sweep_tet96.txt"))
     if (tr and "hbm_bytes_per_launch" in tr and tr.get("m") == args.m
             and tr.get("ncolors") == nc and tr.get("seed") == args.seed
-            and tr.get("colour") == kc and args.coloring == "greedy-bfs"):
+            and tr.get("colour") == kc and args.coloring == "greedy-bfs"
+            and args.cell_order == "hex"):
         traffic = tr["hbm_bytes_per_launch"]
 
     mixed = None
