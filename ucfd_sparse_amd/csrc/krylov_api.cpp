@@ -560,8 +560,15 @@ int bicgstab(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, do
             beta = (rho/rhoprev)*(alpha/omega);
             if ((rc = kry_bicg_p(s, N, beta, omega, r, v, p, pt))) return rc;
         }
+        // pi = rt.v stays on the device: s = r - (rho/pi) v is formed there, and
+        // pi comes back with the five dots in one copy (one host wait per
+        // iteration).  A pi breakdown then stops before x and r change.
         if ((rc = pc_apply(s, pc, mp, pt)) || (rc = spmv(s, A, ma, 1.0, pt, 0.0, v)) ||
-            (rc = kry_dot(s, N, rt, v, part, db + DB_NSQ)) || (rc = d2h(S, DB_NSQ, 1, s, fn)))
+            (rc = kry_dot(s, N, rt, v, part, db + DB_NSQ)) ||
+            (rc = kry_bicg_s(s, N, rho, db + DB_NSQ, r, v, sv, shat)) ||
+            (rc = pc_apply(s, pc, mp, shat)) || (rc = spmv(s, A, ma, 1.0, shat, 0.0, t)) ||
+            (rc = kry_dot5(s, N, rt, sv, t, part, db + DB_H)) ||
+            (rc = d2h(S, DB_H, DB_NSQ + 1 - DB_H, s, fn)))
             return rc;
         const double pi = hb[DB_NSQ];
         if (fabs(pi) < 1e-30) {
@@ -569,10 +576,6 @@ int bicgstab(Solver S, Precon pc, SpMat A, const MatDev &ma, const PcDev &mp, do
             break;
         }
         alpha = rho/pi;
-        if ((rc = kry_bicg_s(s, N, alpha, r, v, sv, shat)) || (rc = pc_apply(s, pc, mp, shat)) ||
-            (rc = spmv(s, A, ma, 1.0, shat, 0.0, t)) ||
-            (rc = kry_dot5(s, N, rt, sv, t, part, db + DB_H)) || (rc = d2h(S, DB_H, 5, s, fn)))
-            return rc;
         const double phi = hb[DB_H], rtt = hb[DB_H + 1], ts = hb[DB_H + 2], tt = hb[DB_H + 3],
                      ss = hb[DB_H + 4];
         omega = ts/tt;

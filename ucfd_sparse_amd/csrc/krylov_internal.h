@@ -94,9 +94,9 @@ UCFD_HIDDEN int kry_dot5(hipStream_t s, int64_t n, const double *rt, const doubl
 // p = r + beta*(p - omega*v); pt = p
 UCFD_HIDDEN int kry_bicg_p(hipStream_t s, int64_t n, double beta, double omega, const double *r,
                            const double *v, double *p, double *pt);
-// s = r - alpha*v; shat = s
-UCFD_HIDDEN int kry_bicg_s(hipStream_t s, int64_t n, double alpha, const double *r,
-                           const double *v, double *sv, double *shat);
+// s = r - alpha*v; shat = s, with alpha = rho / *pi (pi on the device)
+UCFD_HIDDEN int kry_bicg_s(hipStream_t s, int64_t n, double rho, const double *pi,
+                           const double *r, const double *v, double *sv, double *shat);
 // b = (float)a
 UCFD_HIDDEN int kry_cast32(hipStream_t s, int64_t n, const double *a, float *b);
 // x += alpha*pt + omega*shat; r = s - omega*t

@@ -934,10 +934,14 @@ __global__ __launch_bounds__(KB) void k_bicg_p(int64_t n, double beta, double om
 }
 
 // s = r - alpha*v (as copy + axpy); shat = s
-__global__ __launch_bounds__(KB) void k_bicg_s(int64_t n, double alpha, const double *__restrict__ r,
+// alpha = rho/pi with pi read on the device (the host forms the same
+// quotient once it has pi)
+__global__ __launch_bounds__(KB) void k_bicg_s(int64_t n, double rho, const double *__restrict__ pi,
+                                               const double *__restrict__ r,
                                                const double *__restrict__ v,
                                                double *__restrict__ sv, double *__restrict__ shat)
 {
+    const double alpha = rho/(*pi);
     const int64_t st = (int64_t)gridDim.x*KB;
     for (int64_t i = (int64_t)blockIdx.x*KB + threadIdx.x; i < n; i += st) {
         const double t = r[i] + (-alpha)*v[i];
@@ -1293,10 +1297,10 @@ int kry_bicg_p(hipStream_t s, int64_t n, double beta, double omega, const double
     return ok();
 }
 
-int kry_bicg_s(hipStream_t s, int64_t n, double alpha, const double *r, const double *v,
-               double *sv, double *shat)
+int kry_bicg_s(hipStream_t s, int64_t n, double rho, const double *pi, const double *r,
+               const double *v, double *sv, double *shat)
 {
-    hipLaunchKernelGGL(k_bicg_s, dim3(nblocks(n)), dim3(KB), 0, s, n, alpha, r, v, sv, shat);
+    hipLaunchKernelGGL(k_bicg_s, dim3(nblocks(n)), dim3(KB), 0, s, n, rho, pi, r, v, sv, shat);
     return ok();
 }
 
