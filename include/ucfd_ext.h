@@ -92,8 +92,15 @@ int ucfd_permute_nei(int neles, int nface, const int *perm, const int *nei, int 
  * exact pointers and an [n0, ne) covering one whole colour then decide
  * "lower" / "upper" from the index range (nb < n0 / nb >= ne), which equals
  * the lcolor test on such a layout and saves the lcolor[nb] gathers.  The
- * arrays must not change while declared: release them first.  Writes the
- * number of colours to *ncolors (nullable). */
+ * arrays must not change while declared: release them first.  Each such
+ * call samples one cell per workgroup against the declaration and fails with
+ * status 5 (UCFD_ERR_STATE) if icolor / lcolor no longer match it (arrays
+ * changed, or freed and the addresses reused).  A colour in which no face
+ * can be taken (the first colour of a lower sweep, the last of an upper
+ * sweep) reads no neighbour index on such a layout, so an out-of-range
+ * nei_ele entry of that colour is reported by the sweep in the other
+ * direction rather than by this call.  Writes the number of colours to
+ * *ncolors (nullable). */
 int ucfd_declare_colored_layout(int neles, const int *icolor, const int *lcolor,
                                 int *ncolors);
 int ucfd_release_colored_layout(const int *icolor, const int *lcolor);

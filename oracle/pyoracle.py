@@ -16,8 +16,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBS = {}
 
 
-def lib(fast: bool = False):
-    name = "liboracle_fast.so" if fast else "liboracle.so"
+def lib(fast: bool = False, build: str | None = None):
+    """The parity build (default), the CPU-baseline build (fast=True or
+    build="fast") or the build with the reference's flags (build="ref")."""
+    name = {None: "liboracle_fast.so" if fast else "liboracle.so", "parity": "liboracle.so",
+            "fast": "liboracle_fast.so", "ref": "liboracle_ref.so"}[build]
     if name not in _LIBS:
         path = os.path.join(HERE, name)
         if not os.path.exists(path):

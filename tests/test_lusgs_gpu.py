@@ -330,6 +330,76 @@ def test_bad_neighbour_index_is_reported_not_faulted():
     test_two_array_form_equals_in_place_form()
 
 
+def _raw_pass_status(m, f, lc_d, ic_d, nei_d, cv):
+    """Statuses of every call of one in-place legacy pass (device arrays)."""
+    lib = L.load()
+    N, F = m.neles, m.nface
+    off = O.color_offsets(lc_d.cpu().numpy())
+    fnv, vfn, fsp, dt, upts = (cv(m.fnorm_vol), cv(m.vec_fnorm), cv(f.fspr), cv(f.dt),
+                               cv(f.upts.copy()))
+    diag, dub = cv(np.zeros(N)), cv(f.rhsb.copy())
+    L.call("parallel_pre_lusgs", N, F, 1.0, fnv, dt, diag, fsp)
+    args = lambda c: (int(off[c]), int(off[c + 1]), N, F, nei_d.data_ptr(),  # noqa: E731
+                      ic_d.data_ptr(), lc_d.data_ptr(), fnv.data_ptr(), vfn.data_ptr(),
+                      upts.data_ptr(), dub.data_ptr(), diag.data_ptr(), fsp.data_ptr())
+    nc = len(off) - 1
+    low = [lib.parallel_ns_lower_sweep(*args(c)) for c in range(nc)]
+    upp = [lib.parallel_ns_upper_sweep(*args(c)) for c in reversed(range(nc))]
+    return low, upp
+
+
+def test_declared_layout_reports_changed_arrays():
+    """A declared layout is found by its pointers; if lcolor changes under
+    the same address (as after a free and reuse), the range-mode calls
+    sample it and fail with status 5 instead of sweeping with the wrong
+    neighbour test.  Re-declaring restores the fast path."""
+    z, mesh, flow = load_case("tet5p")
+    lcolor = z["lcolor"].astype(np.int32)
+    m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
+    cv = torch_dev()
+    lc_d, ic_d, nei_d = cv(lc2), cv(ic2), cv(m2.nei_ele)
+    L.call("ucfd_declare_colored_layout", m2.neles, ic_d, lc_d, None)
+    try:
+        low, upp = _raw_pass_status(m2, f2, lc_d, ic_d, nei_d, cv)
+        assert low == [0] * len(low) and upp == [0] * len(upp)
+        # another colouring written into the same buffer
+        lc_d.copy_(cv(lc2[::-1].copy()))
+        low, upp = _raw_pass_status(m2, f2, lc_d, ic_d, nei_d, cv)
+        say(f"  statuses after the arrays changed: lower {low} upper {upp}")
+        assert 5 in low + upp
+        lc_d.copy_(cv(lc2))
+        L.call("ucfd_declare_colored_layout", m2.neles, ic_d, lc_d, None)
+        low, upp = _raw_pass_status(m2, f2, lc_d, ic_d, nei_d, cv)
+        assert low == [0] * len(low) and upp == [0] * len(upp)
+    finally:
+        L.call("ucfd_release_colored_layout", ic_d, lc_d)
+    test_two_array_form_equals_in_place_form()
+
+
+def test_bad_index_in_face_free_range_is_reported_by_the_other_sweep():
+    """On a declared layout the first colour of the lower sweep takes no face
+    and reads no neighbour index (k_sweep_none): an out-of-range entry of a
+    colour-1 cell is not seen by that call, and the upper sweep of colour 1,
+    which reads every index of its cells, reports it (status 3)."""
+    z, mesh, flow = load_case("tet5p")
+    lcolor = z["lcolor"].astype(np.int32)
+    m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
+    nei = m2.nei_ele.copy()
+    nei[2, 3] = m2.neles + 7                 # cell 3 lies in colour 1 (the first range)
+    assert lc2[3] == 1
+    cv = torch_dev()
+    lc_d, ic_d, nei_d = cv(lc2), cv(ic2), cv(nei)
+    L.call("ucfd_declare_colored_layout", m2.neles, ic_d, lc_d, None)
+    try:
+        low, upp = _raw_pass_status(m2, f2, lc_d, ic_d, nei_d, cv)
+    finally:
+        L.call("ucfd_release_colored_layout", ic_d, lc_d)
+    say(f"  lower statuses {low}, upper statuses (last colour first) {upp}")
+    assert low[0] == 0
+    assert upp[-1] == 3
+    test_two_array_form_equals_in_place_form()
+
+
 def test_async_stream_mode_matches_sync():
     import torch
     z, mesh, flow = load_case("tet5p")
@@ -465,31 +535,75 @@ def test_c_abi_program_matches_oracle():
     assert_close(take(np.float64, 5 * N).reshape(5, N), su, "C caller serial upts")
 
 
-def test_tet96_5m_cells_fp64_matches_oracle():
-    """Full benchmark size (96^3 x 6 tets = 5,308,416 cells), colour-contiguous
-    layout, one iteration against the OpenMP oracle."""
-    import torch
-    say("96^3x6 tets (5.3M cells): generating 5.3M-cell tet mesh")
+@pytest.fixture(scope="module")
+def tet96():
+    say("96^3x6 tets (5.3M cells): generating the mesh, fields and Hilbert keys")
     mesh = M.tet_mesh(96, seed=1)
     flow = M.flow_fields(mesh, seed=1)
-    N = mesh.neles
-    lcolor = np.empty(N, np.int32)
-    icolor = np.empty(N, np.int32)
-    L.load().ucfd_greedy_coloring(N, 4, mesh.nei_ele.ctypes.data, icolor.ctypes.data,
-                                  lcolor.ctypes.data, None)
-    m2, f2, lc2, ic2, perm = reorder(mesh, flow, lcolor)
-    del mesh, flow
-    say("96^3x6 tets (5.3M cells): GPU pass (declared colour-contiguous layout)")
+    mesh.key = M.hilbert_keys(mesh, "tet", (96,))
+    return mesh, flow
+
+
+@pytest.mark.parametrize("coloring", ["greedy", "greedy-bfs"])
+def test_tet96_5m_cells_fp64_matches_oracle(tet96, coloring):
+    """Full benchmark size (96^3 x 6 tets = 5,308,416 cells), permuted, in the
+    layout the benchmark times: coloured by the a'1 first-fit routine
+    ("greedy", 5 colours) or by first-fit in breadth-first order
+    ("greedy-bfs", the benchmark's 2 colours), renumbered colour-contiguously
+    in Hilbert order and declared, so the range-mode kernels run
+    (k_sweep_none for the empty ranges, the all-taken variant for the last
+    lower / first upper colour).  Both array forms; one pass against the
+    OpenMP oracle."""
+    import torch
+    mesh, flow = tet96
+    say(f"96^3x6 tets (5.3M cells), {coloring}: colour-contiguous Hilbert layout")
+    lay = LY.colored_layout(mesh, flow, coloring=coloring)
+    m2, f2, N = lay.mesh, lay.flow, lay.mesh.neles
+    say(f"96^3x6 tets, {coloring}: {lay.ncolors} colours {np.diff(lay.off).tolist()}")
     cv = torch_dev()
-    lc_d, ic_d = cv(lc2), cv(ic2)
+    lc_d, ic_d = cv(lay.lcolor), cv(lay.icolor)
     L.call("ucfd_declare_colored_layout", N, ic_d, lc_d, None)
     try:
+        say(f"96^3x6 tets, {coloring}: GPU pass, in-place C form")
         diag, dstar, dq, upts = colored_pass_lib(m2, f2, lc_d, ic_d, cv)
+        torch.cuda.empty_cache()
+        say(f"96^3x6 tets, {coloring}: GPU pass, two-array form (as benchmarked)")
+        _, dstar2, dq2, upts2 = colored_pass_lib(m2, f2, lc_d, ic_d, cv, two_array=True)
     finally:
         L.call("ucfd_release_colored_layout", ic_d, lc_d)
+    del lc_d, ic_d
     torch.cuda.empty_cache()
-    say("96^3x6 tets (5.3M cells): oracle pass")
-    od, os_, oq, ou = O.colored_pass(m2.nei_ele, ic2, lc2, m2.fnorm_vol, m2.vec_fnorm, f2.fspr,
-                                     f2.dt, f2.upts, f2.rhsb)
-    assert_close(dq, oq, "tet96 dQ")
-    assert_close(upts, ou, "tet96 upts")
+    say(f"96^3x6 tets, {coloring}: oracle pass")
+    od, os_, oq, ou = O.colored_pass(m2.nei_ele, lay.icolor, lay.lcolor, m2.fnorm_vol,
+                                     m2.vec_fnorm, f2.fspr, f2.dt, f2.upts, f2.rhsb)
+    assert_close(diag, od, f"tet96 {coloring} diag")
+    assert_close(dstar, os_, f"tet96 {coloring} dQ*")
+    assert_close(dq, oq, f"tet96 {coloring} dQ")
+    assert_close(upts, ou, f"tet96 {coloring} upts")
+    assert_close(dstar2, os_, f"tet96 {coloring} two-array dQ*")
+    assert_close(dq2, oq, f"tet96 {coloring} two-array dQ")
+    assert_close(upts2, ou, f"tet96 {coloring} two-array upts")
+
+
+def test_pressure_clamp_fixture_on_gpu():
+    """The legacy kernels on cells at the pressure floor: the reference's
+    outputs for a case where PMIN clamps (flux.c:50-56) the states of 18 of
+    32 cells, tests/golden/lusgs_clamp.npz (written by the compiled reference
+    through the renumbering bridge on an unpermuted 4x4x2 hex grid),
+    reproduced by the GPU pass through the C-ABI, host and device arrays."""
+    z = np.load(os.path.join(GOLD, "lusgs_clamp.npz"))
+    mesh = M.hex_mesh(4, 4, 2, seed=3, permute=False)
+    flow = M.Flow(z["upts"], z["fspr"], z["dt"], z["rhsb"])
+    u = z["upts"]
+    p = 0.4 * (u[4] - 0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / u[0])
+    say(f"clamp fixture: {(p < 1e-13).sum()} of {p.size} cells below PMIN")
+    assert (p < 1e-13).sum() > 0
+    lcolor = z["lcolor"].astype(np.int32)
+    icolor = np.argsort(lcolor, kind="stable").astype(np.int32)
+    for to_dev in (None, torch_dev()):
+        diag, dstar, dq, upts = colored_pass_lib(mesh, flow, lcolor, icolor, to_dev)
+        where = "host" if to_dev is None else "device"
+        assert_close(diag, z["c_diag"], f"clamp diag ({where})")
+        assert_close(dstar, z["c_dstar"], f"clamp dQ* ({where})")
+        assert_close(dq, z["c_dq"], f"clamp dQ ({where})")
+        assert_close(upts, z["c_upts1"], f"clamp upts ({where})")

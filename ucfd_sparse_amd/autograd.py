@@ -66,9 +66,23 @@ class ColoredSystem:
                                              self.upts.data_ptr()), "ucfd_adjoint_create")
         self.handle = h
 
+    def check_inputs(self, rhsb, dt, fspr, **outs):
+        """Every array the pass reads or writes: float64, contiguous, on the
+        GPU, at least as large as the kernels index (ValueError otherwise)."""
+        N, F = self.N, self.F
+        if rhsb is not None:
+            L.arr(rhsb, np.float64, 5 * N, "rhsb", device=True)
+        L.arr(dt, np.float64, N, "dt", device=True)
+        L.arr(fspr, np.float64, F * N, "fspr", device=True)
+        sizes = {"dq": 5 * N, "dstar": 5 * N, "diag": N, "w": 5 * N}
+        for k, v in outs.items():
+            if v is not None:
+                L.arr(v, np.float64, sizes[k], k, device=True)
+
     def forward_arrays(self, rhsb, dt, fspr, factor=1.0, dq=None, dstar=None, diag=None):
         """pre + lower + upper on torch's current stream; returns (dq, dstar, diag)."""
         lib, N, F = self.lib, self.N, self.F
+        self.check_inputs(rhsb, dt, fspr, dq=dq, dstar=dstar, diag=diag)
         diag = torch.empty(N, dtype=torch.float64, device=rhsb.device) if diag is None else diag
         dstar = torch.zeros_like(rhsb) if dstar is None else dstar
         dq = torch.zeros_like(rhsb) if dq is None else dq
@@ -97,6 +111,7 @@ class ColoredSystem:
         """pre + lower + upper with the in-place C sweeps on one buffer;
         returns dQ (dQ* is overwritten by the upper sweep)."""
         lib, N, F = self.lib, self.N, self.F
+        self.check_inputs(rhsb, dt, fspr)
         diag = torch.empty(N, dtype=torch.float64, device=rhsb.device)
         dub = rhsb.clone()
         p = dict(nei=self.nei.data_ptr(), ic=self.icolor.data_ptr(), lc=self.lcolor.data_ptr(),
@@ -117,6 +132,7 @@ class ColoredSystem:
 
     def recompute_dstar(self, rhsb, dt, fspr, factor=1.0):
         lib = self.lib
+        self.check_inputs(rhsb, dt, fspr)
         diag = torch.empty(self.N, dtype=torch.float64, device=rhsb.device)
         dstar = torch.zeros_like(rhsb)
         p = dict(nei=self.nei.data_ptr(), ic=self.icolor.data_ptr(), lc=self.lcolor.data_ptr(),
@@ -130,10 +146,11 @@ class ColoredSystem:
     def adjoint(self, dt, fspr, dstar, dq, w, factor=1.0, overwrite_dstar=False):
         """(rhsbar, dtbar, fsprbar) of J = sum(w * dq); with overwrite_dstar
         the rhsb gradient is written into dstar's buffer and returned."""
+        w = w.contiguous()
+        self.check_inputs(None, dt, fspr, dq=dq, dstar=dstar, w=w)
         rb = dstar if overwrite_dstar else torch.zeros_like(dq)
         dtb = torch.empty_like(dt)
         fb = torch.empty_like(fspr)
-        w = w.contiguous()
         with _Stream(self.lib):
             L.check(self.lib.ucfd_lusgs_adjoint(self.handle, float(factor), dt.data_ptr(),
                                                 fspr.data_ptr(), dstar.data_ptr(), dq.data_ptr(),
@@ -161,6 +178,7 @@ class LUSGSPass(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rhsb, dt, fspr, system, recompute=False, factor=1.0):
         rhsb, dt, fspr = rhsb.contiguous(), dt.contiguous(), fspr.contiguous()
+        system.check_inputs(rhsb, dt, fspr)
         ctx.system, ctx.recompute, ctx.factor = system, recompute, factor
         if recompute:
             dq = system.forward_inplace(rhsb, dt, fspr, factor)

@@ -31,22 +31,35 @@ bool bad_dims(int neles, int nface) { return neles < 0 || nface < 1 || nface > 3
 // the index range [off[c], off[c+1]).  A coloured call on such a layout whose
 // [n0, ne) is one whole colour takes the neighbour test "nb < n0" / "nb >= ne"
 // instead of reading lcolor[nb]; that is the same test, without the gather.
+// Layouts are found by their pointers; since an address can be freed and
+// reused for other arrays, every range-mode launch samples one cell per
+// workgroup (icolor[i] == i, lcolor[i] == the colour's declared value) and a
+// mismatch fails the call with UCFD_ERR_STATE instead of sweeping silently
+// with the wrong neighbour test.
 struct DeclaredLayout {
     const int *icolor;
     const int *lcolor;
     int neles;
-    std::vector<int> off;
+    std::vector<int> off;      // colour c = [off[c], off[c+1])
+    std::vector<int> value;    // lcolor value of colour c
 };
 std::mutex g_layout_mu;
 std::vector<DeclaredLayout> g_layouts;
 
-bool whole_colour_of_declared(const int *icolor, const int *lcolor, int neles, int n0, int ne)
+// true if [n0, ne) is one whole colour of a layout declared with these
+// pointers; *lc_value = that colour's lcolor value
+bool whole_colour_of_declared(const int *icolor, const int *lcolor, int neles, int n0, int ne,
+                              int *lc_value)
 {
     std::lock_guard<std::mutex> g(g_layout_mu);
     for (const DeclaredLayout &l : g_layouts) {
         if (l.icolor != icolor || l.lcolor != lcolor || l.neles != neles) continue;
         auto it = std::lower_bound(l.off.begin(), l.off.end(), n0);
-        return it != l.off.end() && *it == n0 && it + 1 != l.off.end() && *(it + 1) == ne;
+        if (it != l.off.end() && *it == n0 && it + 1 != l.off.end() && *(it + 1) == ne) {
+            *lc_value = l.value[it - l.off.begin()];
+            return true;
+        }
+        return false;
     }
     return false;
 }
@@ -94,8 +107,9 @@ int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, co
     const int a_out = sg.add(out, V*N*8, UCFD_INOUT);
     const int a_dg = sg.add(diag, N*8, UCFD_IN);
     const int a_sp = sg.add(fspr, F*N*8, UCFD_IN);
-    const int mode = whole_colour_of_declared(icolor, lcolor, neles, n0, ne) ? UCFD_SWEEP_RANGE
-                                                                             : UCFD_SWEEP_COLORED;
+    int lc_value = 0;
+    const int mode = whole_colour_of_declared(icolor, lcolor, neles, n0, ne, &lc_value)
+                         ? UCFD_SWEEP_RANGE : UCFD_SWEEP_COLORED;
     int rc = sg.enter();
     if (rc) return rc;
     rc = ucfd_launch_sweep(ucfd_stream(), dir, mode, n0, ne, neles, nface, c.ndims,
@@ -104,7 +118,7 @@ int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, co
                            (const double *)sg.dev(a_vfn), (const double *)sg.dev(a_u),
                            (const double *)sg.dev(a_in), (double *)sg.dev(a_out),
                            (const double *)sg.dev(a_dg), (const double *)sg.dev(a_sp),
-                           ucfd_device_flag());
+                           ucfd_device_flag(), lc_value);
     if (rc) ucfd_report(rc, fn, "kernel launch failed");
     return sg.leave(rc);
 }
@@ -173,8 +187,11 @@ int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
     int rc = ucfd_ensure_device(fn);
     if (rc) return rc;
 
-    // the level schedule is built on the host from nei_ele, or reused
+    // the level schedule is built on the host from nei_ele, or reused; one
+    // lock covers the hash buffer, the hash and the schedule cache, so
+    // concurrent serial calls neither allocate nor read the hash twice
     if (!nei) return ucfd_report(UCFD_ERR_ARG, fn, "NULL nei_ele");
+    std::lock_guard<std::mutex> lock(g_serial_mu);
     const bool dev = ucfd_is_device_ptr(nei);
     unsigned long long h = 0;
     if (dev) {
@@ -189,7 +206,6 @@ int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
     } else {
         h = ucfd_hash_host(nei, (int64_t)(F*N));
     }
-    std::lock_guard<std::mutex> lock(g_serial_mu);
     SerialSchedule &C = g_serial[dir];
     if (!(C.nei == nei && C.neles == neles && C.nface == nface && C.dev == dev && C.hash == h)) {
         std::vector<int> hnei;
@@ -243,7 +259,7 @@ int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
                                (const double *)sg.dev(a_fnv), (const double *)sg.dev(a_vfn),
                                (const double *)sg.dev(a_u), (const double *)sg.dev(a_in),
                                (double *)sg.dev(a_out), (const double *)sg.dev(a_dg),
-                               (const double *)sg.dev(a_sp), ucfd_device_flag());
+                               (const double *)sg.dev(a_sp), ucfd_device_flag(), 0);
     }
     if (rc) ucfd_report(rc, fn, "kernel launch failed");
     return sg.leave(rc);
@@ -393,13 +409,16 @@ UCFD_API int ucfd_declare_colored_layout(int neles, const int *icolor, const int
     }
     if (!dev_i) std::copy(icolor, icolor + neles, ic.begin());
     if (!dev_l) std::copy(lcolor, lcolor + neles, lc.begin());
-    DeclaredLayout l{icolor, lcolor, neles, {0}};
+    DeclaredLayout l{icolor, lcolor, neles, {0}, {lc[0]}};
     for (int p = 0; p < neles; ++p) {
         if (ic[p] != p)
             return ucfd_report(UCFD_ERR_ARG, fn, "icolor is not the identity permutation");
         if (p > 0 && lc[p] < lc[p - 1])
             return ucfd_report(UCFD_ERR_ARG, fn, "lcolor is not non-decreasing");
-        if (p > 0 && lc[p] != lc[p - 1]) l.off.push_back(p);
+        if (p > 0 && lc[p] != lc[p - 1]) {
+            l.off.push_back(p);
+            l.value.push_back(lc[p]);
+        }
     }
     l.off.push_back(neles);
     std::lock_guard<std::mutex> g(g_layout_mu);

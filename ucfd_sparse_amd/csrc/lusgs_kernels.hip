@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
     const int *__restrict__ order, const int *__restrict__ lcolor,
     const double *__restrict__ fnv, const double *__restrict__ vfn,
     const double *__restrict__ upts, const double *in, double *out,
-    const double *__restrict__ diag, const double *__restrict__ fspr, int *flag)
+    const double *__restrict__ diag, const double *__restrict__ fspr, int *flag, int lce)
 {
     constexpr int NV = ND + 2;
     // all faces' loads in flight at once for tets; pairs of faces for hexes
@@ -132,6 +132,11 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
     constexpr bool RANGE = MODE >= 2, EAGER = MODE == 3;
     const int t = xcd_block()*kSweepBlock + threadIdx.x;
     if (t >= cnt) return;
+    // range modes: the workgroup's first cell samples the declared layout
+    // (uniform address: scalar loads, consumed after the stores)
+    int sic = 0, slc = 0;
+    const int ib = n0 + xcd_block()*kSweepBlock;
+    if (RANGE) { sic = order[ib]; slc = lcolor[ib]; }
     const int i = RANGE ? n0 + t : ld_stream(order + n0 + t);
     if ((unsigned)i >= (unsigned)N) {
         raise_flag(flag, UCFD_KFLAG_INDEX);
@@ -219,6 +224,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
         if (DIR == 0) __builtin_nontemporal_store((xin[k] - 0.5*df[k])/dg, out + o);
         else          __builtin_nontemporal_store(xin[k] - (0.5*df[k]/dg), out + o);
     }
+    if (RANGE && threadIdx.x == 0 && (sic != ib || slc != lce)) raise_flag(flag, UCFD_KFLAG_LAYOUT);
 }
 
 // A colour range where no face can be taken in range mode: the first colour
@@ -228,13 +234,21 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(
 // with df = 0 (so -0.0 inputs and a zero diagonal give the same bits).
 // CPT cells per thread (t, t + S, ...), every load issued before the first
 // store.
+// The workgroup's first cell samples the declared layout as in k_sweep; the
+// neighbour indices of this range are neither read nor bounds checked (the
+// sweep in the other direction reads and checks them).
 template <int DIR, int NV, int CPT>
 __global__ __launch_bounds__(kBlock) void k_sweep_none(int n0, int cnt, int N,
                                                        const double *in, double *out,
-                                                       const double *__restrict__ diag)
+                                                       const double *__restrict__ diag,
+                                                       const int *__restrict__ icolor,
+                                                       const int *__restrict__ lcolor, int lce,
+                                                       int *flag)
 {
     const int S = gridDim.x*kBlock;
     const int t0 = blockIdx.x*kBlock + threadIdx.x;
+    const int ib = n0 + blockIdx.x*kBlock;
+    const int sic = icolor[ib], slc = lcolor[ib];
     const double df = 0.0;
     double dg[CPT], xin[CPT][NV];
 #pragma unroll
@@ -258,6 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_none(int n0, int cnt, int N,
             else          __builtin_nontemporal_store(xin[c][k] - (0.5*df/dg[c]), out + o);
         }
     }
+    if (threadIdx.x == 0 && (sic != ib || slc != lce)) raise_flag(flag, UCFD_KFLAG_LAYOUT);
 }
 
 // Any other face count: one face at a time.
@@ -267,11 +282,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep_any(
     const int *__restrict__ order, const int *__restrict__ lcolor,
     const double *__restrict__ fnv, const double *__restrict__ vfn,
     const double *__restrict__ upts, const double *in, double *out,
-    const double *__restrict__ diag, const double *__restrict__ fspr, int *flag)
+    const double *__restrict__ diag, const double *__restrict__ fspr, int *flag, int lce)
 {
     constexpr int NV = ND + 2;
     const int t = xcd_block()*kBlock + threadIdx.x;
     if (t >= cnt) return;
+    if (MODE >= 2 && threadIdx.x == 0) {
+        const int ib = n0 + xcd_block()*kBlock;
+        if (order[ib] != ib || lcolor[ib] != lce) raise_flag(flag, UCFD_KFLAG_LAYOUT);
+    }
     const int i = MODE >= 2 ? n0 + t : order[n0 + t];
     if ((unsigned)i >= (unsigned)N) {
         raise_flag(flag, UCFD_KFLAG_INDEX);
@@ -333,22 +352,22 @@ template <int DIR, int MODE, int ND>
 int launch_sweep_f(hipStream_t s, int n0, int cnt, int N, int nface, const int *nei,
                    const int *order, const int *lcolor, const double *fnv, const double *vfn,
                    const double *upts, const double *in, double *out, const double *diag,
-                   const double *fspr, int *flag)
+                   const double *fspr, int *flag, int lce)
 {
     const dim3 grid((cnt + kBlock - 1)/kBlock), block(kBlock);
     const dim3 sgrid((cnt + kSweepBlock - 1)/kSweepBlock), sblock(kSweepBlock);
     switch (nface) {
     case 4:
         hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 4>), sgrid, sblock, 0, s, n0, cnt, N, nface, nei,
-                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag, lce);
         break;
     case 6:
         hipLaunchKernelGGL((k_sweep<DIR, MODE, ND, 6>), sgrid, sblock, 0, s, n0, cnt, N, nface, nei,
-                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+                           order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag, lce);
         break;
     default:
         hipLaunchKernelGGL((k_sweep_any<DIR, MODE, ND>), grid, block, 0, s, n0, cnt, N, nface,
-                           nei, order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag);
+                           nei, order, lcolor, fnv, vfn, upts, in, out, diag, fspr, flag, lce);
         break;
     }
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
@@ -358,13 +377,13 @@ template <int DIR, int MODE>
 int launch_sweep_d(hipStream_t s, int ndims, int n0, int cnt, int N, int nface, const int *nei,
                    const int *order, const int *lcolor, const double *fnv, const double *vfn,
                    const double *upts, const double *in, double *out, const double *diag,
-                   const double *fspr, int *flag)
+                   const double *fspr, int *flag, int lce)
 {
     if (ndims == 3)
         return launch_sweep_f<DIR, MODE, 3>(s, n0, cnt, N, nface, nei, order, lcolor, fnv, vfn,
-                                            upts, in, out, diag, fspr, flag);
+                                            upts, in, out, diag, fspr, flag, lce);
     return launch_sweep_f<DIR, MODE, 2>(s, n0, cnt, N, nface, nei, order, lcolor, fnv, vfn, upts,
-                                        in, out, diag, fspr, flag);
+                                        in, out, diag, fspr, flag, lce);
 }
 
 }  // namespace
@@ -391,29 +410,33 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
                       int ndims, int nfvars, const int *nei, const int *order, const int *lcolor,
                       const double *fnorm_vol, const double *vec_fnorm, const double *upts,
                       const double *in, double *out, const double *diag, const double *fspr,
-                      int *flag)
+                      int *flag, int lc_expect)
 {
     const int cnt = ne - n0;
     if (cnt <= 0) return UCFD_OK;
     (void)nfvars;
     if (dir == 0 && mode == UCFD_SWEEP_COLORED)
         return launch_sweep_d<0, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     if (dir == 1 && mode == UCFD_SWEEP_COLORED)
         return launch_sweep_d<1, 0>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     // a range with no neighbour in the sweep direction: nothing to gather
     if (mode == UCFD_SWEEP_RANGE && (dir == 0 ? n0 == 0 : ne == neles)) {
         constexpr int CPT = 2;
         const dim3 g((cnt + CPT*kBlock - 1)/(CPT*kBlock)), b(kBlock);
         if (dir == 0 && ndims == 3)
-            hipLaunchKernelGGL((k_sweep_none<0, 5, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<0, 5, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag,
+                               order, lcolor, lc_expect, flag);
         else if (dir == 0)
-            hipLaunchKernelGGL((k_sweep_none<0, 4, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<0, 4, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag,
+                               order, lcolor, lc_expect, flag);
         else if (ndims == 3)
-            hipLaunchKernelGGL((k_sweep_none<1, 5, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<1, 5, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag,
+                               order, lcolor, lc_expect, flag);
         else
-            hipLaunchKernelGGL((k_sweep_none<1, 4, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag);
+            hipLaunchKernelGGL((k_sweep_none<1, 4, CPT>), g, b, 0, s, n0, cnt, neles, in, out, diag,
+                               order, lcolor, lc_expect, flag);
         return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
     }
     // a whole colour with every neighbour in the sweep direction: the last
@@ -422,21 +445,21 @@ int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int nele
                            (dir == 0 ? ne == neles : n0 == 0);
     if (dir == 0 && all_taken)
         return launch_sweep_d<0, 3>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     if (dir == 1 && all_taken)
         return launch_sweep_d<1, 3>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     if (dir == 0 && mode == UCFD_SWEEP_RANGE)
         return launch_sweep_d<0, 2>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     if (dir == 1 && mode == UCFD_SWEEP_RANGE)
         return launch_sweep_d<1, 2>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     if (dir == 0)
         return launch_sweep_d<0, 1>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                    vec_fnorm, upts, in, out, diag, fspr, flag);
+                                    vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
     return launch_sweep_d<1, 1>(s, ndims, n0, cnt, neles, nface, nei, order, lcolor, fnorm_vol,
-                                vec_fnorm, upts, in, out, diag, fspr, flag);
+                                vec_fnorm, upts, in, out, diag, fspr, flag, lc_expect);
 }
 
 // Content hash of an int array: sum over k of splitmix64(k, a[k]) mod 2^64.

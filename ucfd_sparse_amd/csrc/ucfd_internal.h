@@ -21,7 +21,7 @@ enum {
 };
 
 // Bits raised by kernels in the device error word.
-enum { UCFD_KFLAG_INDEX = 1, UCFD_KFLAG_PIVOT = 2 };
+enum { UCFD_KFLAG_INDEX = 1, UCFD_KFLAG_PIVOT = 2, UCFD_KFLAG_LAYOUT = 4 };
 
 // Flow-variable configuration of the legacy API (compile-time in the
 // reference: NVARS / NFVARS / NDIMS, Makefile.inc:21-24).
@@ -76,13 +76,15 @@ UCFD_HIDDEN int ucfd_launch_pre(hipStream_t s, int neles, int nface, double fact
 // taken iff lcolor[nb] </> lcolor[i].  mode ORDERED: cells order[n0:ne),
 // neighbour taken iff nb </> i (sequential semantics, level scheduled).
 // mode RANGE: cells n0..ne-1 of a declared colour-contiguous layout,
-// neighbour taken iff nb < n0 / nb >= ne (order and lcolor unused).
+// neighbour taken iff nb < n0 / nb >= ne; order (= icolor) and lcolor are
+// only sampled, one cell per workgroup, against the declared layout
+// (icolor[i] == i, lcolor[i] == lc_expect; UCFD_KFLAG_LAYOUT otherwise).
 UCFD_HIDDEN int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int ne, int neles,
                                   int nface, int ndims, int nfvars, const int *nei,
                                   const int *order, const int *lcolor, const double *fnorm_vol,
                                   const double *vec_fnorm, const double *upts, const double *in,
                                   double *out, const double *diag, const double *fspr,
-                                  int *flag);
+                                  int *flag, int lc_expect);
 
 UCFD_HIDDEN int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x);
 // *out = sum over k < n of a mix of (k, a[k]) mod 2^64 (device array a; out

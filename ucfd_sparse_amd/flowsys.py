@@ -17,10 +17,24 @@ from . import lib as L
 RHS, DU, DUP, DIAG, TDIAG = 0, 1, 2, 3, 4
 
 
-def _conv(a, dtype):
-    if hasattr(a, "data_ptr"):
-        return a
-    return np.ascontiguousarray(a, dtype)
+def _npos(rowptr, nlocal):
+    """Number of (cell, face) positions, rowptr[nlocal] (one device read for a
+    tensor)."""
+    return int(rowptr[nlocal])
+
+
+def _structure(fs, ndims):
+    """The face-CSR arrays of UCFDFlowSysCreate, validated (dtype, size)."""
+    nl, nf = int(fs.nlocal), int(fs.nfaces)
+    k = {"rowptr": L.arr(fs.rowptr, np.int32, nl + 1, "rowptr")}
+    npos = _npos(k["rowptr"], nl)
+    k["colidx"] = L.arr(fs.colidx, np.int32, npos, "colidx")
+    k["slots"] = L.arr(fs.slots, np.int32, npos, "slots")
+    k["sides"] = L.arr(fs.sides, np.int8, npos, "sides")
+    k["face_area"] = L.arr(fs.face_area, np.float64, nf, "face_area")
+    k["face_normal"] = L.arr(fs.face_normal, np.float64, int(ndims) * nf, "face_normal")
+    k["rcp_vol"] = L.arr(fs.rcp_vol, np.float64, nl, "rcp_vol")
+    return k
 
 
 class FlowSys:
@@ -28,13 +42,8 @@ class FlowSys:
         """fs: ucfd_sparse_amd.mesh.FlowSysMesh (or any object with the same
         fields, numpy or torch)."""
         self._lib = L.load()
-        self.keep = {}
+        self.keep = _structure(fs, ndims)
         k = self.keep
-        k["rowptr"], k["colidx"] = _conv(fs.rowptr, np.int32), _conv(fs.colidx, np.int32)
-        k["slots"], k["sides"] = _conv(fs.slots, np.int32), _conv(fs.sides, np.int8)
-        k["face_area"] = _conv(fs.face_area, np.float64)
-        k["face_normal"] = _conv(fs.face_normal, np.float64)
-        k["rcp_vol"] = _conv(fs.rcp_vol, np.float64)
         self.nlocal, self.nvars, self.nfvars = int(fs.nlocal), int(nvars), int(nfvars)
         h = ctypes.c_void_p()
         L.check(self._lib.UCFDFlowSysCreate(
@@ -43,8 +52,11 @@ class FlowSys:
             L.ptr(k["sides"]), L.ptr(k["face_area"]), L.ptr(k["face_normal"]),
             L.ptr(k["rcp_vol"])), "UCFDFlowSysCreate")
         self.handle = h
-        k["jmat"] = _conv(fs.jmat if jmat is None else jmat, np.float64)
-        k["tjmat"] = None if tjmat is None else _conv(tjmat, np.float64)
+        nf, nfv, ntv = int(fs.nfaces), self.nfvars, self.nvars - self.nfvars
+        k["jmat"] = L.arr(fs.jmat if jmat is None else jmat, np.float64, 2 * nfv * nfv * nf,
+                          "jmat")
+        k["tjmat"] = None if tjmat is None else L.arr(tjmat, np.float64, 2 * ntv * ntv * nf,
+                                                      "tjmat")
         L.check(self._lib.UCFDFlowSysSetBLUSGS(ctypes.byref(h), L.ptr(k["jmat"]),
                                                L.ptr(k["tjmat"])), "UCFDFlowSysSetBLUSGS")
         if fp32:
@@ -53,14 +65,20 @@ class FlowSys:
     def set_element(self, eidx, el, nface=4):
         k = self.keep
         names = ("cell_ids", "uptsb", "rhs", "dt", "dsrc", "vol", "resid_out")
+        ids = L.arr(el["cell_ids"], np.int32, None, "cell_ids")
+        ne = len(ids)
+        # rows of nvars element values ([nvars][neles]) or one value per element
+        size = {"cell_ids": ne, "dt": ne, "vol": ne}
         for n in names:
             if n == "resid_out" and n not in el:
                 k[(eidx, n)] = np.zeros_like(np.asarray(el["uptsb"])) \
                     if not hasattr(el["uptsb"], "data_ptr") else el["uptsb"].new_zeros(
                         el["uptsb"].shape)
-                continue
-            k[(eidx, n)] = _conv(el[n], np.int32 if n == "cell_ids" else np.float64)
-        ne = len(k[(eidx, "cell_ids")])
+            else:
+                # numpy arrays are kept as converted copies: read written
+                # ones (uptsb, resid_out) back with element_array()
+                k[(eidx, n)] = L.arr(el[n], np.int32 if n == "cell_ids" else np.float64,
+                                     size.get(n, self.nvars * ne), n)
         L.check(self._lib.UCFDFlowSysSetElement(
             ctypes.byref(self.handle), int(eidx), ne, int(nface),
             *[L.ptr(k[(eidx, n)]) for n in names]), "UCFDFlowSysSetElement")
@@ -75,7 +93,7 @@ class FlowSys:
         n = {RHS: self.nlocal * self.nvars, DU: self.nlocal * self.nvars,
              DUP: self.nlocal * self.nvars, DIAG: self.nlocal * self.nfvars ** 2,
              TDIAG: self.nlocal * (self.nvars - self.nfvars) ** 2}[which]
-        out = np.zeros(n) if out is None else out
+        out = np.zeros(n) if out is None else L.arr(out, np.float64, n, "out", out=True)
         L.check(self._lib.UCFDBLUSGS_Export(self.handle, int(which), L.ptr(out)),
                 "UCFDBLUSGS_Export")
         return out
@@ -99,13 +117,8 @@ class FlowSysLUSGS(FlowSys):
 
     def __init__(self, fs, nvars, nfvars, fspr, tfspr=None, ndims=3, nelem=1):
         self._lib = L.load()
-        self.keep = {}
+        self.keep = _structure(fs, ndims)
         k = self.keep
-        k["rowptr"], k["colidx"] = _conv(fs.rowptr, np.int32), _conv(fs.colidx, np.int32)
-        k["slots"], k["sides"] = _conv(fs.slots, np.int32), _conv(fs.sides, np.int8)
-        k["face_area"] = _conv(fs.face_area, np.float64)
-        k["face_normal"] = _conv(fs.face_normal, np.float64)
-        k["rcp_vol"] = _conv(fs.rcp_vol, np.float64)
         self.nlocal, self.nvars, self.nfvars = int(fs.nlocal), int(nvars), int(nfvars)
         h = ctypes.c_void_p()
         L.check(self._lib.UCFDFlowSysCreate(
@@ -114,13 +127,15 @@ class FlowSysLUSGS(FlowSys):
             L.ptr(k["sides"]), L.ptr(k["face_area"]), L.ptr(k["face_normal"]),
             L.ptr(k["rcp_vol"])), "UCFDFlowSysCreate")
         self.handle = h
-        k["fspr"] = _conv(fspr, np.float64)
-        k["tfspr"] = None if tfspr is None else _conv(tfspr, np.float64)
+        nf = int(fs.nfaces)
+        k["fspr"] = L.arr(fspr, np.float64, nf, "fspr")
+        k["tfspr"] = None if tfspr is None else L.arr(tfspr, np.float64, nf, "tfspr")
         L.check(self._lib.UCFDFlowSysSetLUSGS(ctypes.byref(h), L.ptr(k["fspr"]),
                                               L.ptr(k["tfspr"])), "UCFDFlowSysSetLUSGS")
 
     def export(self, which, out=None):
-        out = np.zeros(self.nlocal * self.nvars) if out is None else out
+        n = self.nlocal * self.nvars
+        out = np.zeros(n) if out is None else L.arr(out, np.float64, n, "out", out=True)
         L.check(self._lib.UCFDLUSGS_Export(self.handle, int(which), L.ptr(out)),
                 "UCFDLUSGS_Export")
         return out

@@ -21,16 +21,17 @@ CONVERGED, HAPPYBREAKDOWN, REACH_ITERMAX, RHOBREAKDOWN, PIBREAKDOWN = 0, 1, 2, 3
 INITIALIZED = -1
 
 
-def _i32(a):
-    if hasattr(a, "data_ptr"):
-        return a
-    return np.ascontiguousarray(a, np.int32)
+def _i32(a, n=None, name="index array"):
+    return L.arr(a, np.int32, n, name)
 
 
-def _f64(a):
-    if hasattr(a, "data_ptr"):
-        return a
-    return np.ascontiguousarray(a, np.float64)
+def _f64(a, n=None, name="value array"):
+    return L.arr(a, np.float64, n, name)
+
+
+def _nnz(rowptr, bn):
+    """Stored blocks, rowptr[bn] (one device read for a tensor)."""
+    return int(rowptr[bn])
 
 
 class SpMat:
@@ -38,7 +39,11 @@ class SpMat:
 
     def __init__(self, rowptr, colidx, values, bn, blk=None):
         self._lib = L.load()
-        self.rowptr, self.colidx, self.values = _i32(rowptr), _i32(colidx), _f64(values)
+        b2 = 1 if blk is None else int(blk) ** 2
+        self.rowptr = _i32(rowptr, int(bn) + 1, "rowptr")
+        nnz = _nnz(self.rowptr, int(bn))
+        self.colidx = _i32(colidx, nnz, "colidx")
+        self.values = _f64(values, nnz * b2, "values")
         h = ctypes.c_void_p()
         if blk is None:
             L.check(self._lib.UCFDMatCreateCSR(ctypes.byref(h), int(bn), L.ptr(self.rowptr),
@@ -59,6 +64,8 @@ class SpMat:
         return cls(A.rowptr, A.colidx, A.values, A.bn, A.blk)
 
     def mult(self, x, y, alpha=1.0, beta=0.0):
+        x = _f64(x, self.n, "x")
+        y = L.arr(y, np.float64, self.n, "y", out=True)
         L.check(self._lib.UCFDMatMult(float(alpha), self.handle, L.ptr(x), float(beta), L.ptr(y)),
                 "UCFDMatMult")
         return y
@@ -94,8 +101,12 @@ class Precon:
             L.check(self._lib.UCFDPreconCreateNone(ctypes.byref(h)), "UCFDPreconCreateNone")
             self.handle = h
             return
-        self.rowptr, self.colidx = _i32(rowptr), _i32(colidx)
-        self.diagslots, self.values = _i32(diagslots), _f64(values)
+        b2 = int(blk) ** 2
+        self.rowptr = _i32(rowptr, int(bn) + 1, "rowptr")
+        nnz = _nnz(self.rowptr, int(bn))
+        self.colidx = _i32(colidx, nnz, "colidx")
+        self.diagslots = _i32(diagslots, int(bn), "diagslots")
+        self.values = _f64(values, nnz * b2, "values")
         L.check(self._lib.UCFDPreconCreatefromArrays(
             ctypes.byref(h), L.ptr(self.rowptr), L.ptr(self.colidx), L.ptr(self.diagslots),
             L.ptr(self.values)), "UCFDPreconCreatefromArrays")
@@ -138,6 +149,7 @@ class Precon:
         L.check(self._lib.UCFDPreconPrepare(self.handle), "UCFDPreconPrepare")
 
     def apply(self, b):
+        b = L.arr(b, np.float64, self.n, "b", out=True)
         L.check(self._lib.UCFDPreconApply(self.handle, L.ptr(b)), "UCFDPreconApply")
         return b
 
@@ -177,6 +189,8 @@ class Solver:
         self.trace = trace
 
     def solve(self, pc, A, x, b):
+        x = L.arr(x, np.float64, self.n, "x", out=True)
+        b = _f64(b, self.n, "b")
         L.check(self._lib.UCFDSolve(self.handle, pc.handle, A.handle, L.ptr(x), L.ptr(b)),
                 "UCFDSolve")
         return self.result()

@@ -14,6 +14,8 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblusgs.so")
 _LIB = None
@@ -202,6 +204,45 @@ def ptr(a):
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
     return a.ctypes.data
+
+
+def arr(a, dtype, n=None, name="array", out=False, device=False):
+    """Validate one array argument before its address reaches the library.
+
+    dtype: the numpy dtype the entry point reads (float64, float32, int32,
+    int8); n: the number of elements it touches (None: unchecked).  numpy
+    inputs are converted to a C-contiguous array of dtype; numpy outputs
+    (out=True) must already be one, since the results would otherwise land
+    in a copy.  torch tensors must have exactly that dtype, be contiguous and
+    hold at least n elements; device=True also requires a CUDA tensor.  Raw
+    integer addresses pass unchecked.  Raises ValueError on a mismatch, so a
+    wrong dtype or size never becomes an out-of-bounds device access."""
+    if a is None or isinstance(a, int):
+        return a
+    want = np.dtype(dtype)
+    if hasattr(a, "data_ptr"):
+        import torch
+        tdt = getattr(torch, want.name)
+        if a.dtype != tdt:
+            raise ValueError(f"{name}: dtype {a.dtype}, expected {tdt}")
+        if not a.is_contiguous():
+            raise ValueError(f"{name}: tensor must be contiguous")
+        if device and not a.is_cuda:
+            raise ValueError(f"{name}: must be a CUDA tensor")
+        if n is not None and a.numel() < n:
+            raise ValueError(f"{name}: {a.numel()} elements, at least {n} needed")
+        return a
+    if out:
+        if not isinstance(a, np.ndarray) or a.dtype != want or not a.flags.c_contiguous:
+            raise ValueError(f"{name}: output must be a C-contiguous {want} numpy array")
+        b = a
+    else:
+        b = np.ascontiguousarray(a, want)
+    if device:
+        raise ValueError(f"{name}: must be a CUDA tensor")
+    if n is not None and b.size < n:
+        raise ValueError(f"{name}: {b.size} elements, at least {n} needed")
+    return b
 
 
 class UCFDError(RuntimeError):

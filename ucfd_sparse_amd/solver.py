@@ -19,13 +19,20 @@ class LUSGSSolver:
     def __init__(self, neles, nface, color_off, nei_ele, fnorm_vol, vec_fnorm, fspr, dt, upts):
         self._lib = L.load()
         off = np.ascontiguousarray(color_off, np.int32)
+        N, F = int(neles), int(nface)
+        f64 = np.float64
+        a = dict(nei=L.arr(nei_ele, np.int32, F * N, "nei_ele"),
+                 fnv=L.arr(fnorm_vol, f64, F * N, "fnorm_vol"),
+                 vfn=L.arr(vec_fnorm, f64, 3 * F * N, "vec_fnorm"),
+                 fsp=L.arr(fspr, f64, F * N, "fspr"), dt=L.arr(dt, f64, N, "dt"),
+                 upts=L.arr(upts, f64, 5 * N, "upts"))
         h = ctypes.c_void_p()
         L.check(self._lib.ucfd_lusgs_solver_create(
-            ctypes.byref(h), int(neles), int(nface), len(off) - 1, off.ctypes.data,
-            L.ptr(nei_ele), L.ptr(fnorm_vol), L.ptr(vec_fnorm), L.ptr(fspr), L.ptr(dt),
-            L.ptr(upts)), "ucfd_lusgs_solver_create")
+            ctypes.byref(h), N, F, len(off) - 1, off.ctypes.data,
+            *[L.ptr(a[k]) for k in ("nei", "fnv", "vfn", "fsp", "dt", "upts")]),
+            "ucfd_lusgs_solver_create")
         self.handle = h
-        self.neles = int(neles)
+        self.neles = N
 
     @classmethod
     def from_layout(cls, lay):
@@ -34,12 +41,17 @@ class LUSGSSolver:
                    np.ascontiguousarray(f.upts[:5]))
 
     def residual(self, b, dq, r=None):
+        n = 5 * self.neles
+        b, dq = L.arr(b, np.float64, n, "b"), L.arr(dq, np.float64, n, "dq")
+        r = L.arr(r, np.float64, n, "r", out=True)
         nrm = ctypes.c_double()
         L.check(self._lib.ucfd_lusgs_residual(self.handle, L.ptr(b), L.ptr(dq), L.ptr(r),
                                               ctypes.byref(nrm)), "ucfd_lusgs_residual")
         return nrm.value
 
     def solve(self, b, dq, mode="fp64", rtol=1e-10, maxiter=200):
+        n = 5 * self.neles
+        b, dq = L.arr(b, np.float64, n, "b"), L.arr(dq, np.float64, n, "dq", out=True)
         st = L.SolveStats()
         m = L.UCFD_SOLVE_FP64 if mode == "fp64" else L.UCFD_SOLVE_MIXED
         L.check(self._lib.ucfd_lusgs_solve(self.handle, m, L.ptr(b), L.ptr(dq), float(rtol),
