@@ -137,6 +137,28 @@ int ucfd_lusgs_solver_destroy(ucfd_lusgs_solver *solver);
 /* r = b - A(dq) in FP64 (r nullable); *rnorm = ||r||_2 (nullable) */
 int ucfd_lusgs_residual(ucfd_lusgs_solver *solver, const double *b, const double *dq,
                         double *r, double *rnorm);
+/* No-allocation variant of ucfd_lusgs_residual.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same r = b - A(dq) (r nullable) and *rnorm = ||r||_2 (nullable), bit for bit,
+ * but without a solver handle: the operator is formed from the legacy arrays
+ * (nei_ele, fnorm_vol, vec_fnorm, fspr, dt, upts [>= 5 rows]) of a
+ * colour-contiguous mesh (colour c = cells color_off[c] .. color_off[c+1]-1;
+ * color_off is host memory).  Every array must be device (or managed)
+ * memory.  The call allocates nothing: its scratch (block partial sums, norm,
+ * error word) is a fixed library array of 1 MiB per GPU, held until the
+ * process ends, so the colour ranges may hold at most 131072 workgroups of
+ * 128 cells (sum over colours of ceil(cells/128); about 16.7M cells) and
+ * larger meshes are refused with status 1.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (several host threads or streams)
+ * race and must be serialised by the caller.  Runs on the library stream and
+ * returns after it finished (the norm is read back).  Status 3 for an index
+ * outside [0, neles), 1 for a face joining two cells of one colour. */
+int ucfd_lusgs_residual_noalloc(int neles, int nface, int ncolors, const int *color_off,
+                                const int *nei_ele, const double *fnorm_vol,
+                                const double *vec_fnorm, const double *fspr, const double *dt,
+                                const double *upts, const double *b, const double *dq, double *r,
+                                double *rnorm);
 /* dq (output, [5][neles]) with ||b - A(dq)|| <= rtol ||b||; maxiter bounds the
  * number of LU-SGS passes; stats nullable */
 int ucfd_lusgs_solve(ucfd_lusgs_solver *solver, int mode, const double *b, double *dq,

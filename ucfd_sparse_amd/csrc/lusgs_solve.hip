@@ -367,6 +367,94 @@ __global__ __launch_bounds__(1024) void k_sum_parts(const double *part, int n, d
     }
 }
 
+// ---- r = b - A(dq) read straight from the legacy arrays (no packed face
+// records): the face record and the diagonal are formed in registers with
+// the expressions of k_pack, so r and the block partials equal k_residual64's
+// bit for bit.  Cells of colour range [n0, n0+cnt); a neighbour index out of
+// range or inside the cell's own colour raises a bit in flag (that face is
+// skipped).
+template <int FC>
+__global__ __launch_bounds__(kSB) void k_residual_raw(
+    int n0, int cnt, int N, const int *__restrict__ nei, const double *__restrict__ fnv,
+    const double *__restrict__ vfn, const double *__restrict__ fspr,
+    const double *__restrict__ dt, const double *__restrict__ q, const double *__restrict__ b,
+    const double *__restrict__ dq, double *__restrict__ r, double *part, int pofs, int *flag)
+{
+    const int blk = xcd_remap();
+    const int t = blk*kSB + threadIdx.x;
+    const int i = n0 + t;
+    double ss = 0.0;
+    if (t < cnt) {
+        int nbv[FC];
+        double fv[FC], lam[FC], nrm[FC][3];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const size_t o = (size_t)f*N + i;
+            nbv[f] = ldg_nt(nei + o);
+            fv[f] = ldg_nt(fnv + o);
+            lam[f] = ldg_nt(fspr + o);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nrm[f][d] = ldg_nt(vfn + (size_t)(3*f + d)*N + i);
+        }
+        const double dti = ldg_nt(dt + i);
+        double bi[NV], dqi[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const size_t o = (size_t)k*N + i;
+            bi[k] = ldg_nt(b + o);
+            dqi[k] = dq[o];
+        }
+        bool in_f[FC];
+        double U[FC][NV], d[FC][NV];
+        int bad = 0;
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const int nb = nbv[f];
+            if ((unsigned)nb >= (unsigned)N) {
+                bad |= UCFD_KFLAG_INDEX;
+                in_f[f] = false;
+            } else if (nb != i && nb >= n0 && nb < n0 + cnt) {
+                bad |= UCFD_KFLAG_LAYOUT;
+                in_f[f] = false;
+            } else {
+                in_f[f] = nb != i;
+            }
+            if (!in_f[f]) continue;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                U[f][k] = q[(size_t)k*N + nb];
+                d[f][k] = dq[(size_t)k*N + nb];
+            }
+        }
+        if (bad) atomicOr(flag, bad);
+        double sd = 0.0;
+#pragma unroll
+        for (int f = 0; f < FC; ++f) sd += 0.5*lam[f]*fv[f];
+        const double dg = 1.0/dti + sd;
+        double res[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) res[k] = 0.0;
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            if (!in_f[f]) continue;
+            const Face4<double> g{fv[f]*nrm[f][0], fv[f]*nrm[f][1], fv[f]*nrm[f][2],
+                                  fv[f]*lam[f]};
+            double o[NV];
+            face_term<double>(U[f], d[f], g, o);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) res[k] += o[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const size_t o = (size_t)k*N + i;
+            const double rv = bi[k] - (dg*dqi[k] + 0.5*res[k]);
+            ss += rv*rv;
+            if (r) r[o] = rv;
+        }
+    }
+    block_sum<kSB>(ss, part, pofs + blk);
+}
+
 __global__ void k_add32(int64_t n, double *__restrict__ y, const float *__restrict__ x)
 {
     const int64_t st = (int64_t)gridDim.x*kB;
@@ -742,4 +830,124 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     info.converged = bnorm == 0.0 || rn <= rtol*bnorm;
     if (stats) *stats = info;
     return sg.leave(rc);
+}
+
+// ===================================================== no-allocation residual
+//
+// ucfd_lusgs_residual_noalloc: the residual of ucfd_lusgs_residual without a
+// solver handle and without any allocation, for callers that cannot allocate
+// device memory.  Its only scratch (the block partial sums, the norm and an
+// error word) is the file-scope array below: one copy per GPU, alive until
+// the process ends, shared by every call on that GPU (overlapping calls on
+// one GPU race).  Host code never passes the array's name to a kernel: its
+// device address is looked up once per device with hipGetSymbolAddress.
+
+namespace {
+
+constexpr int kNoAllocParts = 131072;   // block partials: 1 MiB
+struct NoAllocScratch {
+    double part[kNoAllocParts];
+    double norm[8];
+    int flag[8];
+};
+static __device__ __align__(256) NoAllocScratch scratch;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+NoAllocScratch *g_scratch[kMaxDevices] = {};
+
+__global__ void k_clear_flag(int *flag)
+{
+    if (threadIdx.x == 0) flag[0] = 0;
+}
+
+// device address of this device's scratch (looked up on the first call)
+int scratch_ptr(NoAllocScratch **out, const char *fn)
+{
+    int dev = -1;
+    int rc = ucfd_check_hip(hipGetDevice(&dev), fn, "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices)
+        return ucfd_report(UCFD_ERR_HIP, fn, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        rc = ucfd_check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), fn,
+                            "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        g_scratch[dev] = (NoAllocScratch *)p;
+    }
+    *out = g_scratch[dev];
+    return UCFD_OK;
+}
+
+}  // namespace
+
+UCFD_API int ucfd_lusgs_residual_noalloc(int neles, int nface, int ncolors, const int *color_off,
+                                         const int *nei_ele, const double *fnorm_vol,
+                                         const double *vec_fnorm, const double *fspr,
+                                         const double *dt, const double *upts, const double *b,
+                                         const double *dq, double *r, double *rnorm)
+{
+    const char *fn = "ucfd_lusgs_residual_noalloc";
+    if (neles <= 0 || (nface != 4 && nface != 6) || ncolors <= 0 || !color_off || !nei_ele ||
+        !fnorm_vol || !vec_fnorm || !fspr || !dt || !upts || !b || !dq)
+        return ucfd_report(UCFD_ERR_ARG, fn, "bad arguments (nface must be 4 or 6, 3-D Euler)");
+    if (color_off[0] != 0 || color_off[ncolors] != neles)
+        return ucfd_report(UCFD_ERR_ARG, fn, "color_off must run from 0 to neles");
+    long long parts = 0;
+    for (int c = 0; c < ncolors; ++c) {
+        if (color_off[c + 1] < color_off[c])
+            return ucfd_report(UCFD_ERR_ARG, fn, "color_off must be non-decreasing");
+        parts += (color_off[c + 1] - color_off[c] + kSB - 1)/kSB;
+    }
+    if (parts > kNoAllocParts)
+        return ucfd_report(UCFD_ERR_ARG, fn, "mesh too large for the fixed scratch: the colour "
+                                             "ranges need more than 131072 workgroups of 128 "
+                                             "cells (about 16.7M cells)");
+    if (ucfd_config().ndims != 3)
+        return ucfd_report(UCFD_ERR_ARG, fn, "3-D Euler only (ndims = 3)");
+    const void *arrays[] = {nei_ele, fnorm_vol, vec_fnorm, fspr, dt, upts, b, dq, r};
+    for (const void *a : arrays)
+        if (a && !ucfd_is_device_ptr(a))
+            return ucfd_report(UCFD_ERR_ARG, fn, "arrays must be device (or managed) memory: "
+                                                 "host arrays would need a staging allocation");
+    NoAllocScratch *S = nullptr;
+    int rc = scratch_ptr(&S, fn);
+    if (rc) return rc;
+    const hipStream_t st = ucfd_stream();
+    int *flag = S->flag;
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, flag);
+    int pofs = 0;
+    for (int c = 0; c < ncolors; ++c) {
+        const int n0 = color_off[c], cnt = color_off[c + 1] - n0;
+        if (cnt <= 0) continue;
+        const dim3 g((cnt + kSB - 1)/kSB);
+        if (nface == 4)
+            hipLaunchKernelGGL((k_residual_raw<4>), g, dim3(kSB), 0, st, n0, cnt, neles, nei_ele,
+                               fnorm_vol, vec_fnorm, fspr, dt, upts, b, dq, r, S->part, pofs,
+                               flag);
+        else
+            hipLaunchKernelGGL((k_residual_raw<6>), g, dim3(kSB), 0, st, n0, cnt, neles, nei_ele,
+                               fnorm_vol, vec_fnorm, fspr, dt, upts, b, dq, r, S->part, pofs,
+                               flag);
+        pofs += (int)g.x;
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, S->part, pofs, S->norm,
+                       (double *)nullptr);
+    if ((rc = ucfd_check_hip(hipGetLastError(), fn, "kernel launch"))) return rc;
+    double nrm2 = 0.0;
+    int f = 0;
+    if ((rc = ucfd_check_hip(hipMemcpyAsync(&nrm2, S->norm, sizeof nrm2, hipMemcpyDeviceToHost,
+                                            st), fn, "hipMemcpyAsync(norm)")) ||
+        (rc = ucfd_check_hip(hipMemcpyAsync(&f, flag, sizeof f, hipMemcpyDeviceToHost, st), fn,
+                             "hipMemcpyAsync(flag)")) ||
+        (rc = ucfd_check_hip(hipStreamSynchronize(st), fn, "hipStreamSynchronize")))
+        return rc;
+    if (f & UCFD_KFLAG_INDEX)
+        return ucfd_report(UCFD_ERR_INDEX, fn, "nei_ele holds an entry outside [0, neles)");
+    if (f & UCFD_KFLAG_LAYOUT)
+        return ucfd_report(UCFD_ERR_ARG, fn, "a face joins two cells of one colour");
+    if (rnorm) *rnorm = sqrt(nrm2);
+    return UCFD_OK;
 }

@@ -63,6 +63,7 @@ _SIGS = {
     "ucfd_lusgs_solver_create": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 7,
     "ucfd_lusgs_solver_destroy": [c_void_p],
     "ucfd_lusgs_residual": [c_void_p] * 5,
+    "ucfd_lusgs_residual_noalloc": [c_int, c_int, c_int] + [c_void_p] * 11,
     "ucfd_lusgs_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_void_p],
     "ucfd_adjoint_create": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 5,
     "ucfd_adjoint_destroy": [c_void_p],
@@ -196,13 +197,18 @@ def device_count() -> int:
 
 
 def ptr(a):
-    """Raw address of a numpy array, torch tensor or integer address."""
+    """Raw address of a numpy array, torch tensor, ctypes object (or
+    ctypes.byref of one) or integer address."""
     if a is None:
         return None
     if isinstance(a, int):
         return a
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
+    if isinstance(a, (ctypes._SimpleCData, ctypes.Structure, ctypes.Array)):
+        return ctypes.addressof(a)
+    if hasattr(a, "_obj"):                      # ctypes.byref(obj)
+        return ctypes.addressof(a._obj)
     return a.ctypes.data
 
 
