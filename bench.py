@@ -1,22 +1,26 @@
 """Benchmark of the colored LU-SGS hot path on one MI355X.
 
-Workload (the configuration the metric is quoted on): colored LU-SGS, 3-D
-Euler (nvars = 5), FP64, on a 96^3 hex grid split into 6 Kuhn tets per hex
-(5,308,416 cells, nface = 4), cell order randomly permuted from the seed,
-normals / areas / volumes from jittered vertex coordinates, fields as in
-ucfd_sparse_amd.mesh.flow_fields.  The mesh is greedy-coloured and stored
-colour-contiguously (Hilbert order inside a colour) and the layout is
-declared to the library (ucfd_declare_colored_layout).  The colouring is
-first-fit greedy visiting the cells along the Hilbert curve, with the highest
-colour then removed by Kempe-chain swaps (4 colours): most cells fall in two
-dense colours whose neighbour gathers stay cache-local, and no tiny colour
-costs a latency-bound launch.  One step = one
-legacy-API pass: parallel_pre_lusgs, ns_parallel_lower_sweep for every
-colour, ns_parallel_upper_sweep for every colour in reverse, and
-lusgs_parallel_ns_update - all on device-resident arrays, enqueued on one
-stream.  The right-hand side is never overwritten (the two-array API takes
-the lower sweep's input and the upper sweep's output as separate arrays), so
-no restore copy sits in the timed region.
+Headline workload (the configuration the metric is quoted on): colored
+LU-SGS, 3-D Euler (nvars = 5), FP64, on a 96^3 hex grid split into 6 Kuhn
+tets per hex (5,308,416 cells, nface = 4), cell order randomly permuted from
+the seed, normals / areas / volumes from jittered vertex coordinates, fields
+as in ucfd_sparse_amd.mesh.flow_fields.  The mesh is coloured by first-fit in
+breadth-first order (2 colours: the Kuhn face graph is bipartite), stored
+colour-contiguously in Hilbert order and declared to the library
+(ucfd_declare_colored_layout).  One step = one legacy-API pass:
+parallel_pre_lusgs, ns_parallel_lower_sweep for every colour,
+ns_parallel_upper_sweep for every colour in reverse and
+lusgs_parallel_ns_update, all on device-resident arrays on one stream.  The
+two-array API keeps the right-hand side intact, so no restore copy sits in
+the timed region.
+
+Beside the headline the line carries: the dominant kernel's roofline, the
+pass on two other colourings (the first-fit routine of the library and a
+4-colour layout), the FP64 and mixed FP32/FP64 defect-correction solves to a
+1e-10 residual drop with per-kernel bandwidth, the CPU baseline (the oracle's
+OpenMP pass, two builds) with the GPU pass checked against the oracle, and
+summaries of the block LU-SGS (96^3 x 6), Krylov (70^3 x 6) and adjoint
+(55^3 x 6) workloads.  --quick keeps only the headline pass.
 
 Prints one JSON line.  value = cell-iterations/s of the FP64 path.
 """
@@ -27,6 +31,13 @@ import json
 import os
 import sys
 import time
+
+# OpenMP placement of the CPU baseline: set before any OpenMP runtime loads
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_PROC_BIND", "close")
+# the CPUs this process may use, read before an OpenMP runtime binds the
+# initial thread to its first place
+_AFFINITY = sorted(os.sched_getaffinity(0))
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
@@ -45,7 +56,8 @@ def algorithmic_bytes(F, V=5, D=3, s=8):
 
 
 def read_traffic(path):
-    """key = value lines written from the rocprofv3 --pmc runs (profiles/)."""
+    """key = value lines written from the rocprofv3 --pmc runs (profiles/):
+    numbers as floats, anything else as text."""
     out = {}
     if not os.path.exists(path):
         return None
@@ -56,52 +68,127 @@ def read_traffic(path):
             try:
                 out[k] = float(v)
             except ValueError:
-                pass
+                out[k] = v
     return out
 
 
-def cpu_baseline(lay, seconds, nvars=5):
-    """The oracle's OpenMP coloured pass (liboracle_fast.so) on the host
-    cores, same renumbered arrays; a bounded number of iterations."""
+def cpu_info():
+    """CPU model, logical CPUs this process may use and the physical cores
+    among them (sysfs topology)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cpus = _AFFINITY
+    cores = set()
+    for c in cpus:
+        try:
+            pkg = open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id").read()
+            core = open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id").read()
+            cores.add((pkg.strip(), core.strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores)}
+
+
+def cpu_baseline(lay, seconds, gpu_pass=None, nvars=5):
+    """The CPU baseline and the parity check of the measured GPU pass.
+
+    Parity: one pass of the oracle's parity build (oracle/liboracle.so: -O2,
+    no fast-math, no FMA contraction, bit-identical to the reference's own
+    code) from the same initial state as gpu_pass (diag, dQ*, dQ, upts of one
+    GPU pass), compared element by element.
+    Baseline: the oracle's OpenMP coloured pass on the same renumbered arrays,
+    pre + lower + upper + update per iteration, built twice: -O3
+    -march=x86-64-v3 -fopenmp (liboracle_fast.so) and with the reference's
+    own flags -O3 -ffast-math -fopenmp (liboracle_ref.so, Makefile.inc:11-16);
+    3 warm-up iterations, then at least 10 timed iterations (about seconds/2
+    per build), median reported, rhsb restored before each iteration."""
     import numpy as np
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as O
-    lib = O.lib(fast=True)
     m, f = lay.mesh, lay.flow
     N, F = m.neles, m.nface
+    parity = None
+    if gpu_pass is not None:
+        od, os_, oq, ou = O.colored_pass(m.nei_ele, lay.icolor, lay.lcolor, m.fnorm_vol,
+                                         m.vec_fnorm, f.fspr, f.dt, f.upts, f.rhsb)
+        errs = {}
+        bitwise = True
+        for name, g, o in (("diag", gpu_pass[0], od), ("dstar", gpu_pass[1], os_),
+                           ("dq", gpu_pass[2], oq), ("upts", gpu_pass[3], ou)):
+            errs[name] = float(np.abs(g - o).max() / max(np.abs(o).max(), 1e-300))
+            bitwise = bitwise and bool(np.array_equal(g, o))
+        parity = {"max_rel_err": max(errs.values()), "per_array": errs, "bitwise": bitwise,
+                  "checked": f"one pass of the timed configuration ({N} cells, "
+                             f"{lay.ncolors} colours) against the oracle's parity build"}
     off = np.ascontiguousarray(lay.off, np.int32)
-    upts = f.upts.copy()
-    dub = f.rhsb.copy()
-    diag = np.empty(N)
-    args = lambda: (N, F, 3, 5, nvars, lay.ncolors, off.ctypes.data,  # noqa: E731
-                    m.nei_ele.ctypes.data, lay.icolor.ctypes.data, lay.lcolor.ctypes.data,
-                    m.fnorm_vol.ctypes.data, m.vec_fnorm.ctypes.data, upts.ctypes.data,
-                    dub.ctypes.data, diag.ctypes.data, f.fspr.ctypes.data, f.dt.ctypes.data)
-    dub[...] = f.rhsb
-    lib.orc_colored_iteration(*args())          # warm-up
-    n, t = 0, 0.0
-    while t < seconds or n < 2:
-        dub[...] = f.rhsb
-        t0 = time.perf_counter()
-        lib.orc_colored_iteration(*args())
-        t += time.perf_counter() - t0
-        n += 1
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    return {"value": n * N / t, "unit": "cell-iter/s", "cores": threads, "kind": "port",
-            "sample": f"{n} coloured FP64 iterations of the full {N}-cell workload, "
-                      f"oracle/lusgs_oracle.c built -O3 -march=x86-64-v3 -fopenmp, "
-                      f"{threads} OpenMP threads, {t:.1f} s"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(_AFFINITY)
+    builds = {}
+    for key, build, flags in (("fast", "fast", "-O3 -march=x86-64-v3 -fopenmp"),
+                              ("reference_flags", "ref", "-O3 -ffast-math -fopenmp")):
+        lib = O.lib(build=build)
+        upts = f.upts.copy()
+        dub = f.rhsb.copy()
+        diag = np.empty(N)
+        args = (N, F, 3, 5, nvars, lay.ncolors, off.ctypes.data, m.nei_ele.ctypes.data,
+                lay.icolor.ctypes.data, lay.lcolor.ctypes.data, m.fnorm_vol.ctypes.data,
+                m.vec_fnorm.ctypes.data, upts.ctypes.data, dub.ctypes.data, diag.ctypes.data,
+                f.fspr.ctypes.data, f.dt.ctypes.data)
+        for _ in range(3):
+            dub[...] = f.rhsb
+            lib.orc_colored_iteration(*args)
+        times = []
+        while sum(times) < seconds / 2 or len(times) < 10:
+            dub[...] = f.rhsb
+            t0 = time.perf_counter()
+            lib.orc_colored_iteration(*args)
+            times.append(time.perf_counter() - t0)
+        med = sorted(times)[len(times) // 2]
+        builds[key] = {"cell_iters_per_s": N / med, "median_ms": med * 1e3,
+                       "iterations": len(times), "flags": flags,
+                       "gbs_algorithmic": 888 * N / med / 1e9}
+    info = cpu_info()
+    fast = builds["fast"]
+    return {"value": fast["cell_iters_per_s"], "unit": "cell-iter/s", "cores": threads,
+            "kind": "port",
+            "sample": f"median of {fast['iterations']} coloured FP64 iterations (pre + lower + "
+                      f"upper + update) of the full {N}-cell workload, oracle/lusgs_oracle.c "
+                      f"built {fast['flags']}, {threads} OpenMP threads "
+                      f"(OMP_PLACES={os.environ.get('OMP_PLACES')}, "
+                      f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')})",
+            "cpu_model": info["model"], "logical_cpus": info["logical_cpus"],
+            "physical_cores": info["physical_cores"], "builds": builds}, parity
+
+
+# Algorithmic bytes per cell of one pass of the defect-correction kernels
+# (every array a kernel touches counted once per pass; neighbour reads of the
+# array the kernel writes counted as one more read, as for the legacy sweeps;
+# s = bytes per real, packed face records of 4 reals per face).
+def solver_kernel_bytes(F, s):
+    lower = 4 * F + s * (4 * F + 26)      # nei, geo, diag, rhs, e, qb, dstar read + write
+    upper = 4 * F + s * (4 * F + 31)      # nei, geo, diag, qb, dstar, delta r+w, e r+w
+    return lower, upper
 
 
 def precision_forms(lay, T, rtol, repeats, log):
     """Time-to-rtol of the FP64 and mixed FP32/FP64 defect-correction forms on
-    the same device-resident system; error of the mixed result vs FP64."""
+    the same device-resident system; error of the mixed result vs FP64; HIP
+    event times of one pass of each solver kernel (FP64 and FP32) with their
+    algorithmic bandwidth."""
+    import ctypes
+
     import torch
 
+    from ucfd_sparse_amd import lib as L
     from ucfd_sparse_amd.solver import LUSGSSolver
     m = lay.mesh
-    s = LUSGSSolver(m.neles, m.nface, lay.off, T["nei"], T["fnv"], T["vfn"], T["fsp"], T["dt"],
-                    T["upts"])
+    N, F = m.neles, m.nface
+    s = LUSGSSolver(N, F, lay.off, T["nei"], T["fnv"], T["vfn"], T["fsp"], T["dt"], T["upts"])
     b = T["rhs"]
     out = {}
     res = {}
@@ -120,11 +207,29 @@ def precision_forms(lay, T, rtol, repeats, log):
         out[mode] = {"ms": times[len(times) // 2], "passes": st.passes, "outer": st.outer,
                      "rel_residual": st.rel_residual, "converged": bool(st.converged)}
         log(f"{mode}: {out[mode]}")
+    ms = (ctypes.c_double * 5)()
+    L.check(L.load().ucfd_lusgs_solver_time_kernels(s.handle, b.data_ptr(), 10, ms),
+            "ucfd_lusgs_solver_time_kernels")
     s.close()
     a, bm = res["fp64"], res["mixed"]
     err = float((bm - a).abs().max() / a.abs().max())
-    N = m.neles
     k64 = out["fp64"]["passes"]
+    kern = {}
+    for key, idx, sz in (("lower_res_fp64", 0, 8), ("upper_upd_fp64", 1, 8),
+                         ("lower_res_fp32", 2, 4), ("upper_upd_fp32", 3, 4)):
+        lb, ub = solver_kernel_bytes(F, sz)
+        nb = (lb if key.startswith("lower") else ub) * N
+        gbs = nb / (ms[idx] * 1e-3) / 1e9
+        kern[key] = {"ms_per_pass": ms[idx], "algorithmic_bytes": nb, "gbs": gbs,
+                     "frac": gbs / HBM_PEAK_GBS, "launches": lay.ncolors}
+    rb = (4 * F + 8 * (4 * F + 1 + 15)) * N        # nei, geo, diag, q, b, dq
+    kern["residual_fp64"] = {"ms_per_pass": ms[4], "algorithmic_bytes": rb,
+                             "gbs": rb / (ms[4] * 1e-3) / 1e9,
+                             "frac": rb / (ms[4] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    p64 = ms[0] + ms[1]
+    p32 = ms[2] + ms[3]
+    log("solver kernels: " + ", ".join(f"{k} {v['ms_per_pass']:.3f} ms {v['gbs']:.0f} GB/s"
+                                       for k, v in kern.items()))
     return {
         "rtol": rtol,
         "fp64": out["fp64"],
@@ -133,8 +238,12 @@ def precision_forms(lay, T, rtol, repeats, log):
         "fp64_equivalent_cell_iters_per_s": {"fp64": N * k64 / (out["fp64"]["ms"] * 1e-3),
                                              "mixed": N * k64 / (out["mixed"]["ms"] * 1e-3)},
         "mixed_max_rel_err_vs_fp64": err,
+        "kernels": kern,
+        "pass_ms": {"fp64": p64, "fp32": p32, "fp32_speedup": p64 / p32},
         "note": "defect correction A(dq) = b with one coloured LU-SGS pass as preconditioner; "
-                "mixed = FP32 sweeps inside an FP64 outer residual loop, dq accumulated in FP64",
+                "mixed = FP32 sweeps inside an FP64 outer residual loop, dq accumulated in FP64; "
+                "kernel bytes per pass: lower 4F + s(4F+26), upper 4F + s(4F+31) per cell "
+                "(s = 8 FP64, 4 FP32)",
     }
 
 
@@ -309,7 +418,7 @@ def krylov_workload(args, log):
         "kernels": kern,
         "solves": solves,
     }
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def flowsys_bytes(npos, N, nv, fp32):
@@ -581,7 +690,7 @@ def blusgs_workload(args, log):
             "unit": "GB/s", "frac": lb / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": None, "algorithmic_bytes_per_launch": lb, "avg_launch_ms": launch_ms},
     }
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def adjoint_workload(args, log):
@@ -736,218 +845,288 @@ def adjoint_workload(args, log):
                                       v["gradient_signed_raw"] <= 1e-6
                                       for v in fd_err.values())},
     }
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--m", type=int, default=96, help="hex cells per axis (6 m^3 tets)")
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
-    ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--coloring", choices=("greedy", "greedy-key", "greedy-key-reduced",
-                                           "greedy-bfs", "balanced"),
-                    default="greedy-bfs",
-                    help="colouring of the headline mesh: first-fit in cell id "
-                         "(ucfd_greedy_coloring), first-fit along the Hilbert curve "
-                         "(ucfd_greedy_coloring_ordered, -reduced: then Kempe-chain "
-                         "reduction), first-fit in breadth-first order (ucfd_bfs_order) "
-                         "or balanced")
-    ap.add_argument("--cell-order", choices=("hex", "tet-type"), default="hex",
-                    help="numbering inside each colour of the headline mesh: along the "
-                         "Hilbert curve of the cubes, a cube's tets together (hex), or "
-                         "tet index first, then the curve (tet-type)")
-    ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint", "blusgs"),
-                    default="lusgs",
-                    help="lusgs: the headline colored LU-SGS line (default); krylov: the "
-                         "GMRES/BiCGSTAB workload on a 70^3 x 6 tet block system; adjoint: "
-                         "the differentiation form on a 55^3 x 6 tet mesh; blusgs: block "
-                         "LU-SGS (FlowSys NS + SA sweeps, Krylov-path apply) on the 96^3 x 6 "
-                         "tet mesh, FP64 vs FP32 sweeps")
-    ap.add_argument("--adjoint-m", type=int, default=55)
-    ap.add_argument("--adjoint-reps", type=int, default=10)
-    ap.add_argument("--adjoint-dirs", type=int, default=64)
-    ap.add_argument("--krylov-m", type=int, default=70)
-    ap.add_argument("--krylov-rhs", type=int, default=4)
-    ap.add_argument("--calibrate-copy-mb", type=float, default=0.0,
-                    help="only run a plain elementwise copy kernel (out = in + 0) over this "
-                         "many MiB, 5 times: the reference reading for the PMC byte counters")
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print("bench.py measures the colored LU-SGS path on one GPU only; multi-GPU runs "
-              "(domain decomposition with halo exchange) are not part of this build. "
-              "Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+class LegacyPass:
+    """One legacy-API pass on a colour-contiguous layout, device arrays,
+    enqueued on its own stream."""
 
-    import numpy as np
-    import torch
+    def __init__(self, lay):
+        import numpy as np
+        import torch
 
-    if args.calibrate_copy_mb > 0:
-        n = int(args.calibrate_copy_mb * 2**20) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            torch.add(a, 0.0, out=b)
+        from ucfd_sparse_amd import lib as L
+        self.L, self.lib = L, L.load()
+        self.lay = lay
+        dev = torch.device("cuda:0")
+
+        def up(a):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        m, f = lay.mesh, lay.flow
+        self.N, self.F, self.nc = m.neles, m.nface, lay.ncolors
+        self.T = dict(nei=up(m.nei_ele), ic=up(lay.icolor), lc=up(lay.lcolor),
+                      fnv=up(m.fnorm_vol), vfn=up(m.vec_fnorm), fsp=up(f.fspr), dt=up(f.dt),
+                      upts=up(f.upts), rhs=up(f.rhsb))
+        self.upts0 = up(f.upts)
+        self.T["dstar"] = torch.zeros_like(self.T["rhs"])
+        self.T["dq"] = torch.zeros_like(self.T["rhs"])
+        self.T["diag"] = torch.zeros(self.N, dtype=torch.float64, device=dev)
+        self.P = {k: v.data_ptr() for k, v in self.T.items()}
+        self.off = [int(x) for x in lay.off]
+        L.call("ucfd_set_config", 5, 5, 3)
+        L.call("ucfd_declare_colored_layout", self.N, self.T["ic"], self.T["lc"], None)
+        self.stream = torch.cuda.Stream(device=dev)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_kernel_bytes_read": 8 * n, "copy_kernel_bytes_written": 8 * n}))
-        return
 
-    from ucfd_sparse_amd import lib as L
-    from ucfd_sparse_amd import mesh as M
-    from ucfd_sparse_amd.layout import colored_layout
-
-    def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-    if args.workload == "krylov":
-        krylov_workload(args, log)
-        return
-    if args.workload == "adjoint":
-        adjoint_workload(args, log)
-        return
-    if args.workload == "blusgs":
-        blusgs_workload(args, log)
-        return
-
-    t0 = time.time()
-    mesh = M.tet_mesh(args.m, seed=args.seed)
-    flow = M.flow_fields(mesh, seed=args.seed)
-    mesh.key = M.hilbert_keys(mesh, "tet", (args.m,), tet_type_major=args.cell_order == "tet-type")
-    lay = colored_layout(mesh, flow, coloring=args.coloring)
-    del mesh, flow
-    N, F = lay.mesh.neles, lay.mesh.nface
-    nc = lay.ncolors
-    log(f"mesh {N} cells, {nc} colours, setup {time.time() - t0:.1f} s")
-
-    dev = torch.device("cuda:0")
-
-    def up(a):
-        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    m, f = lay.mesh, lay.flow
-    nei, ic, lc = up(m.nei_ele), up(lay.icolor), up(lay.lcolor)
-    fnv, vfn, fsp, dt = up(m.fnorm_vol), up(m.vec_fnorm), up(f.fspr), up(f.dt)
-    upts, rhs = up(f.upts), up(f.rhsb)
-    dstar = torch.zeros_like(rhs)
-    dq = torch.zeros_like(rhs)
-    diag = torch.zeros(N, dtype=torch.float64, device=dev)
-    P = {k: v.data_ptr() for k, v in dict(nei=nei, ic=ic, lc=lc, fnv=fnv, vfn=vfn, fsp=fsp,
-                                          dt=dt, upts=upts, rhs=rhs, dstar=dstar, dq=dq,
-                                          diag=diag).items()}
-    off = [int(x) for x in lay.off]
-    lib = L.load()
-    L.call("ucfd_set_config", 5, 5, 3)
-    L.call("ucfd_declare_colored_layout", N, ic, lc, None)
-    stream = torch.cuda.Stream(device=dev)
-    torch.cuda.synchronize()
-    L.call("ucfd_set_stream", stream.cuda_stream)
-
-    def step(ev=None, cev=None):
+    def step(self, ev=None, cev=None):
+        lib, P, N, F, off, nc = self.lib, self.P, self.N, self.F, self.off, self.nc
         if ev:
-            ev[0].record(stream)
+            ev[0].record(self.stream)
         lib.parallel_pre_lusgs(N, F, 1.0, P["fnv"], P["dt"], P["diag"], P["fsp"])
         if ev:
-            ev[1].record(stream)
+            ev[1].record(self.stream)
         for c in range(nc):
             lib.ns_parallel_lower_sweep(off[c], off[c + 1], N, F, P["nei"], P["ic"], P["lc"],
                                         P["fnv"], P["vfn"], P["upts"], P["rhs"], P["dstar"],
                                         P["diag"], P["fsp"])
             if cev:
-                cev[0][c].record(stream)
+                cev[0][c].record(self.stream)
         if ev:
-            ev[2].record(stream)
+            ev[2].record(self.stream)
         for c in reversed(range(nc)):
             lib.ns_parallel_upper_sweep(off[c], off[c + 1], N, F, P["nei"], P["ic"], P["lc"],
                                         P["fnv"], P["vfn"], P["upts"], P["dq"], P["dstar"],
                                         P["diag"], P["fsp"])
             if cev:
-                cev[1][c].record(stream)
+                cev[1][c].record(self.stream)
         if ev:
-            ev[3].record(stream)
+            ev[3].record(self.stream)
         lib.lusgs_parallel_ns_update(N, P["upts"], P["dq"])
         if ev:
-            ev[4].record(stream)
+            ev[4].record(self.stream)
 
-    for _ in range(args.warmup):
-        step()
-    L.call("ucfd_synchronize")
-    torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    t_start.record(stream)
-    for k in range(args.steps):
-        step(evs[k])
-    t_end.record(stream)
-    L.call("ucfd_synchronize")
-    torch.cuda.synchronize()
-    # per-colour launch times from separate instrumented steps (events
-    # between every launch), so the timed steps above carry no extra markers
-    reps = 3
-    cevs = [[[torch.cuda.Event(enable_timing=True) for _ in range(nc)] for _ in range(2)]
-            for _ in range(reps)]
-    pevs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(reps)]
-    for r in range(reps):
-        step(pevs[r], cevs[r])
-    L.call("ucfd_synchronize")
-    torch.cuda.synchronize()
-    L.call("ucfd_set_stream", None)
-    lower_c = [sum((pevs[r][1] if c == 0 else cevs[r][0][c - 1]).elapsed_time(cevs[r][0][c])
-                   for r in range(reps)) / reps for c in range(nc)]
-    upper_c = [sum((pevs[r][2] if c == nc - 1 else cevs[r][1][c + 1]).elapsed_time(cevs[r][1][c])
-                   for r in range(reps)) / reps for c in range(nc)]
-    total_ms = t_start.elapsed_time(t_end)
-    ms_step = total_ms / args.steps
-    pre_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
-    low_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
-    upp_ms = sum(e[2].elapsed_time(e[3]) for e in evs) / args.steps
-    upd_ms = sum(e[3].elapsed_time(e[4]) for e in evs) / args.steps
-    log(f"per step {ms_step:.3f} ms: pre {pre_ms:.3f} lower {low_ms:.3f} upper {upp_ms:.3f} "
-        f"update {upd_ms:.3f}")
+    def run(self, fn):
+        """fn() with the library on this pass's stream; waits and checks."""
+        import torch
+        self.L.call("ucfd_set_stream", self.stream.cuda_stream)
+        try:
+            r = fn()
+            self.L.call("ucfd_synchronize")
+        finally:
+            self.L.call("ucfd_set_stream", None)
+        torch.cuda.synchronize()
+        return r
+
+    def time(self, steps, warmup):
+        """ms per step over `steps` timed steps (events on the pass stream)."""
+        import torch
+
+        def go():
+            for _ in range(warmup):
+                self.step()
+            self.L.call("ucfd_synchronize")
+            evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record(self.stream)
+            for k in range(steps):
+                self.step(evs[k])
+            t1.record(self.stream)
+            return evs, t0, t1
+        evs, t0, t1 = self.run(go)
+        total = t0.elapsed_time(t1)
+        phases = [sum(e[j].elapsed_time(e[j + 1]) for e in evs) / steps for j in range(4)]
+        return total / steps, dict(zip(("pre", "lower", "upper", "update"), phases))
+
+    def launch_times(self, reps=3):
+        """Per-colour launch times from separate instrumented steps."""
+        import torch
+        nc = self.nc
+        cevs = [[[torch.cuda.Event(enable_timing=True) for _ in range(nc)] for _ in range(2)]
+                for _ in range(reps)]
+        pevs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(reps)]
+
+        def go():
+            for r in range(reps):
+                self.step(pevs[r], cevs[r])
+        self.run(go)
+        lower = [sum((pevs[r][1] if c == 0 else cevs[r][0][c - 1]).elapsed_time(cevs[r][0][c])
+                     for r in range(reps)) / reps for c in range(nc)]
+        upper = [sum((pevs[r][2] if c == nc - 1 else cevs[r][1][c + 1]).elapsed_time(
+            cevs[r][1][c]) for r in range(reps)) / reps for c in range(nc)]
+        return lower, upper
+
+    def fresh_pass(self):
+        """One pass from the initial state into fresh outputs; host copies of
+        (diag, dQ*, dQ, upts) for the parity check."""
+        import torch
+        T = self.T
+        T["upts"].copy_(self.upts0)
+        for k in ("dstar", "dq", "diag"):
+            T[k].zero_()
+        # the copies ran on torch's stream, the pass runs on its own
+        torch.cuda.synchronize()
+        self.run(self.step)
+        return tuple(T[k].cpu().numpy() for k in ("diag", "dstar", "dq", "upts"))
+
+    def close(self):
+        self.L.call("ucfd_release_colored_layout", self.T["ic"], self.T["lc"])
+        self.T.clear()
+
+
+def layout_rate(mesh, flow, coloring, steps, warmup, log):
+    """The same pass on another colouring of the same mesh."""
+    import numpy as np
+    import torch
+
+    from ucfd_sparse_amd.layout import colored_layout
+    t0 = time.time()
+    lay = colored_layout(mesh, flow, coloring=coloring)
+    lp = LegacyPass(lay)
+    ms, phases = lp.time(steps, warmup)
+    lower, upper = lp.launch_times()
+    lp.close()
+    del lp
+    torch.cuda.empty_cache()
+    N = lay.mesh.neles
+    log(f"layout {coloring}: {lay.ncolors} colours, {ms:.3f} ms per pass "
+        f"({N / ms / 1e6:.2f} G cell-iter/s), setup {time.time() - t0:.1f} s")
+    return {"ncolors": lay.ncolors, "colour_sizes": [int(x) for x in np.diff(lay.off)],
+            "ms_per_step": ms, "cell_iters_per_s": N / (ms * 1e-3),
+            "hbm_gbs_algorithmic": 888 * N / (ms * 1e-3) / 1e9,
+            "phase_ms": phases, "sweep_launch_ms": {"lower": lower, "upper": upper}}
+
+
+def krylov_summary(r):
+    k = r["kernels"]
+    out = {"config": r["config"]["workload"], "bn": r["config"]["bn"]}
+    for key in ("spmv_bsr5", "blusgs_apply_fp32", "blusgs_apply_fp64", "pbilu_apply_fp64"):
+        out[key] = {"ms": k[key]["ms"], "gbs": k[key]["gbs"],
+                    "frac": k[key]["gbs"] / HBM_PEAK_GBS}
+    for key, v in r["solves"].items():
+        out[key] = {"ms_mean": v["ms_mean"],
+                    "iterations": [x["iterations"] for x in v["per_rhs"]],
+                    "status": [x["status"] for x in v["per_rhs"]],
+                    "max_true_rel_residual": max(x["true_rel_residual"] for x in v["per_rhs"])}
+    return out
+
+
+def blusgs_summary(r):
+    f64, f32 = r["flowsys"]["fp64"], r["flowsys"]["fp32"]
+    sc = r["flowsys_scalar_lusgs"]
+    return {"config": r["config"]["workload"],
+            "ns_cell_iters_per_s": {"fp64": f64["ns_cell_iters_per_s"],
+                                    "fp32": f32["ns_cell_iters_per_s"]},
+            "ns_sweep_gbs": {"fp64": f64["ns_sweep_gbs"], "fp32": f32["ns_sweep_gbs"]},
+            "ns_sweep_frac_fp64": f64["ns_sweep_gbs"] / HBM_PEAK_GBS,
+            "fp32_speedup_ns_sweeps": r["fp32_speedup_ns_sweeps"],
+            "fp32_speedup_rans_sweeps": r["fp32_speedup_rans_sweeps"],
+            "fp32_rel_err_vs_fp64": r["fp32_rel_err_vs_fp64"],
+            "prepare_ms": {"pack": f64["pack_ms"], "ns": f64["ns_prepare_ms"],
+                           "rans": f64["rans_prepare_ms"]},
+            "scalar_lusgs": {"ns_lower_upper_ms": sc["ns_lower_upper_ms"],
+                             "ns_sweep_gbs": sc["ns_sweep_gbs"],
+                             "ns_sweep_frac": sc["ns_sweep_gbs"] / HBM_PEAK_GBS,
+                             "rans_lower_upper_ms": sc["rans_lower_upper_ms"]},
+            "krylov_precon_apply": {k: {"ms": v["ms"], "gbs": v["gbs"],
+                                        "frac": v["gbs"] / HBM_PEAK_GBS}
+                                    for k, v in r["krylov_precon_apply"].items()},
+            "roofline": r["roofline"]}
+
+
+def adjoint_summary(r):
+    mib = 2.0 ** 20
+    return {"config": r["config"]["workload"], "forward_ms": r["forward_ms"],
+            "adjoint_ms": r["adjoint_ms"],
+            "stored": {"ms": r["modes"]["stored"]["ms_forward_backward"],
+                       "peak_mib": r["modes"]["stored"]["peak_bytes_forward_backward"] / mib,
+                       "held_mib": r["modes"]["stored"]["bytes_held_forward_to_backward"] / mib},
+            "recompute": {"ms": r["modes"]["recompute"]["ms_forward_backward"],
+                          "peak_mib": r["modes"]["recompute"]["peak_bytes_forward_backward"] / mib,
+                          "held_mib": r["modes"]["recompute"]["bytes_held_forward_to_backward"]
+                          / mib},
+            "recompute_gradients_identical": r["recompute_gradients_identical"],
+            "fd_directions_per_input": r["fd_check"]["directions_per_input"],
+            "fd_worst": r["fd_check"]["worst"], "fd_pass_1e-6": r["fd_check"]["pass_1e-6"]}
+
+
+def lusgs_headline(args, log):
+    """The headline pass (timed), its roofline, the other colourings, the
+    precision forms and the CPU baseline with the parity check."""
+    import numpy as np
+    import torch
+
+    from ucfd_sparse_amd import mesh as M
+    from ucfd_sparse_amd.layout import colored_layout
+
+    t0 = time.time()
+    mesh = M.tet_mesh(args.m, seed=args.seed)
+    flow = M.flow_fields(mesh, seed=args.seed)
+    mesh.key = M.hilbert_keys(mesh, "tet", (args.m,),
+                              tet_type_major=args.cell_order == "tet-type")
+    lay = colored_layout(mesh, flow, coloring=args.coloring)
+    N, F = lay.mesh.neles, lay.mesh.nface
+    nc = lay.ncolors
+    log(f"mesh {N} cells, {nc} colours, setup {time.time() - t0:.1f} s")
+
+    lp = LegacyPass(lay)
+    ms_step, phases = lp.time(args.steps, args.warmup)
+    lower_c, upper_c = lp.launch_times()
+    log(f"per step {ms_step:.3f} ms: " + " ".join(f"{k} {v:.3f}" for k, v in phases.items()))
     log("launch ms per colour: lower " + " ".join(f"{x:.3f}" for x in lower_c)
         + " | upper " + " ".join(f"{x:.3f}" for x in upper_c))
-
     b_pre, b_sweep, b_upd = algorithmic_bytes(F)
     bytes_cell = b_pre + 2 * b_sweep + b_upd
-    cell_iters = N / (ms_step * 1e-3)
     gbs = bytes_cell * N / (ms_step * 1e-3) / 1e9
-    # dominant kernel: the longest lower-sweep launch.  On the 2-colour layout
-    # that is the last colour, every face taken; the first colour's launch
-    # takes no face and reads 104 of the 344 algorithmic bytes per cell, so
-    # it is not averaged in.
+    # dominant kernel: the longest lower-sweep launch (on the 2-colour layout
+    # the last colour, every face taken; the first colour's launch takes no
+    # face and reads 88 of the 344 algorithmic bytes per cell)
     kc = max(range(nc), key=lambda c: lower_c[c])
+    off = lp.off
     heavy_cells = off[kc + 1] - off[kc]
     lower_launch_ms = lower_c[kc]
     lower_bytes_launch = b_sweep * heavy_cells
     kname = f"k_sweep<0,{3 if (kc == nc - 1 and F in (4, 6)) else 2},3,{F}>"
     achieved = lower_bytes_launch / (lower_launch_ms * 1e-3) / 1e9
-    traffic = None
-    tr = read_traffic(os.path.join(REPO, "profiles", "traffic_lower_sweep_tet96.txt"))
-    if (tr and "hbm_bytes_per_launch" in tr and tr.get("m") == args.m
-            and tr.get("ncolors") == nc and tr.get("seed") == args.seed
-            and tr.get("colour") == kc and args.coloring == "greedy-bfs"
+    tpath = os.path.join("profiles", "traffic_lusgs_pass_tet96.txt")
+    tr = read_traffic(os.path.join(REPO, tpath))
+    traffic = pass_traffic = None
+    traffic_source = "not measured for this configuration"
+    if (tr and tr.get("m") == args.m and tr.get("ncolors") == nc and tr.get("seed") == args.seed
+            and tr.get("colour") == kc and tr.get("coloring") == args.coloring
             and args.cell_order == "hex"):
         traffic = tr["hbm_bytes_per_launch"]
-
+        pass_traffic = tr["pass_hbm_bytes"]
+        traffic_source = (f"{tpath}: calibrated FETCH_SIZE + WRITE_SIZE of this configuration "
+                          f"from separate rocprofv3 --pmc runs (not measured in this run)")
+    log("parity: one pass from the initial state ...")
+    gpu_pass = lp.fresh_pass()
+    T = {k: lp.T[k] for k in ("nei", "fnv", "vfn", "fsp", "dt", "rhs")}
+    T["upts"] = lp.upts0
     mixed = None
     if not args.no_mixed:
         log("precision forms (FP64 vs mixed) ...")
-        T = dict(nei=nei, fnv=fnv, vfn=vfn, fsp=fsp, dt=dt, upts=upts, rhs=rhs)
         mixed = precision_forms(lay, T, args.rtol, 3, log)
+    lp.close()
+    del lp, T
+    torch.cuda.empty_cache()
 
-    cpu = None
+    rates = {}
+    if not args.quick:
+        for col in ("greedy", "greedy-key-reduced"):
+            rates[col] = layout_rate(mesh, flow, col, args.steps, args.warmup, log)
+    del mesh, flow
+
+    cpu = parity = None
     if not args.no_cpu:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(lay, args.cpu_seconds)
+        log("cpu baseline and parity check ...")
+        cpu, parity = cpu_baseline(lay, args.cpu_seconds, gpu_pass)
+        log(f"parity: max rel err {parity['max_rel_err']:.3e}, bitwise {parity['bitwise']}")
 
     rec = {
         "metric": METRIC,
-        "value": cell_iters,
+        "value": N / (ms_step * 1e-3),
         "unit": "cell-iter/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -969,8 +1148,10 @@ def main():
         },
         "hbm_gbs_algorithmic": gbs,
         "bytes_per_cell_iter": bytes_cell,
-        "phase_ms": {"pre": pre_ms, "lower": low_ms, "upper": upp_ms, "update": upd_ms},
+        "phase_ms": phases,
         "sweep_launch_ms": {"lower": lower_c, "upper": upper_c},
+        "parity_max_rel_err": None if parity is None else parity["max_rel_err"],
+        "parity": parity,
         "roofline": {
             "kernel": f"lower sweep of colour {kc} ({kname}, {heavy_cells} cells)",
             "bound": "hbm",
@@ -979,17 +1160,112 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "hbm_gbs_from_traffic": (traffic / (lower_launch_ms * 1e-3) / 1e9
                                      if traffic else None),
             "algorithmic_bytes_per_launch": lower_bytes_launch,
             "avg_launch_ms": lower_launch_ms,
             "pass_algorithmic_gbs": gbs,
             "pass_frac": gbs / HBM_PEAK_GBS,
+            "pass_traffic_bytes": pass_traffic,
+            "pass_traffic_gbs": (pass_traffic / (ms_step * 1e-3) / 1e9 if pass_traffic
+                                 else None),
+            "pass_traffic_frac": (pass_traffic / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                  if pass_traffic else None),
         },
+        "other_colourings": rates,
         "cpu_baseline": cpu,
         "precision_forms": mixed,
     }
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--m", type=int, default=96, help="hex cells per axis (6 m^3 tets)")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU baseline time budget (split over the two builds)")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline and the parity check")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form comparison")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline pass only: no other colourings, no block LU-SGS / Krylov / "
+                         "adjoint summaries")
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--coloring", choices=("greedy", "greedy-key", "greedy-key-reduced",
+                                           "greedy-bfs", "balanced"),
+                    default="greedy-bfs",
+                    help="colouring of the headline mesh: first-fit in cell id "
+                         "(ucfd_greedy_coloring), first-fit along the Hilbert curve "
+                         "(ucfd_greedy_coloring_ordered, -reduced: then Kempe-chain "
+                         "reduction), first-fit in breadth-first order (ucfd_bfs_order) "
+                         "or balanced")
+    ap.add_argument("--cell-order", choices=("hex", "tet-type"), default="hex",
+                    help="numbering inside each colour of the headline mesh: along the "
+                         "Hilbert curve of the cubes, a cube's tets together (hex), or "
+                         "tet index first, then the curve (tet-type)")
+    ap.add_argument("--workload", choices=("lusgs", "krylov", "adjoint", "blusgs"),
+                    default="lusgs",
+                    help="lusgs: the headline line (default, with the other workloads' "
+                         "summaries unless --quick); krylov / adjoint / blusgs: that "
+                         "workload's own full line")
+    ap.add_argument("--adjoint-m", type=int, default=55)
+    ap.add_argument("--adjoint-reps", type=int, default=10)
+    ap.add_argument("--adjoint-dirs", type=int, default=64)
+    ap.add_argument("--krylov-m", type=int, default=70)
+    ap.add_argument("--krylov-rhs", type=int, default=4)
+    ap.add_argument("--calibrate-copy-mb", type=float, default=0.0,
+                    help="only run a plain elementwise copy kernel (out = in + 0) over this "
+                         "many MiB, 5 times: the reference reading for the PMC byte counters")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures the colored LU-SGS path on one GPU only; multi-GPU runs "
+              "(domain decomposition with halo exchange) are not part of this build. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if args.calibrate_copy_mb > 0:
+        n = int(args.calibrate_copy_mb * 2**20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.add(a, 0.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel_bytes_read": 8 * n, "copy_kernel_bytes_written": 8 * n}))
+        return
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    side = {"krylov": krylov_workload, "adjoint": adjoint_workload, "blusgs": blusgs_workload}
+    if args.workload in side:
+        print(json.dumps(side[args.workload](args, log)), flush=True)
+        return
+
+    t_start = time.time()
+    rec = lusgs_headline(args, log)
+    if not args.quick:
+        summarize = {"blusgs": blusgs_summary, "krylov": krylov_summary,
+                     "adjoint": adjoint_summary}
+        for name in ("blusgs", "krylov", "adjoint"):
+            t0 = time.time()
+            log(f"{name} workload ...")
+            rec[name] = summarize[name](side[name](args, log))
+            torch.cuda.empty_cache()
+            log(f"{name} done in {time.time() - t0:.1f} s")
+    rec["bench_wall_s"] = time.time() - t_start
     print(json.dumps(rec), flush=True)
+    if rec["parity_max_rel_err"] is not None and rec["parity_max_rel_err"] > 1e-12:
+        print(f"parity check failed: max rel err {rec['parity_max_rel_err']:.3e} > 1e-12",
+              file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

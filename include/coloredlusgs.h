@@ -62,6 +62,35 @@ int ns_parallel_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
                             double *vec_fnorm, double *upts, double *rhsb,
                             double *dub, double *diag, double *fspr);
 
+/* RANS turbulence rows (k = NFVARS .. NVARS-1, NVARS - NFVARS in {1, 2}),
+ * the {rans}_{parallel}_{lower,upper}_sweep family of doc/1_introduction.md:
+ * 19-22,40-41, which no reference file implements or calls.  Signatures: the
+ * NS ones above (decided here) with tdiag[i] the turbulence diagonal (from
+ * parallel_pre_lusgs with the turbulence spectral radius and factor =
+ * turb_factor) and tfspr[f*neles + i] the turbulence spectral radius; upts
+ * holds all NVARS rows (the neighbour's density and momentum give the
+ * contravariant velocity).  Per taken face:
+ *   df_k += ((dub_nb,k + u_nb,k)*theta - u_nb,k*theta - tfspr_f*dub_nb,k)*fnorm_vol_f,
+ *   theta = (sum_d m_nb,d n_f,d)/rho_nb
+ * (src/lusgs/lusgs.c:274-321 with rans_flux_container, flux.c:77-90); update
+ * and the in-place / two-array conventions as for the NS rows. */
+int parallel_rans_lower_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                              int *icolor, int *lcolor, double *fnorm_vol,
+                              double *vec_fnorm, double *upts, double *dub,
+                              double *tdiag, double *tfspr);
+int parallel_rans_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                              int *icolor, int *lcolor, double *fnorm_vol,
+                              double *vec_fnorm, double *upts, double *dub,
+                              double *tdiag, double *tfspr);
+int rans_parallel_lower_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                              int *icolor, int *lcolor, double *fnorm_vol,
+                              double *vec_fnorm, double *upts, double *rhsb,
+                              double *dub, double *tdiag, double *tfspr);
+int rans_parallel_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                              int *icolor, int *lcolor, double *fnorm_vol,
+                              double *vec_fnorm, double *upts, double *rhsb,
+                              double *dub, double *tdiag, double *tfspr);
+
 /* upts[k,i] += x[k,i] for k < NVARS (x = dub in C, rhsb in Python).
  * replaces: lusgs_parallel_ns_update, called at examples/omp3d.c:189 and
  * examples/omp3d.py:94; arithmetic of src/lusgs/lusgs.c:62-68.

@@ -43,6 +43,24 @@ int ns_serial_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
                           double *vec_fnorm, double *upts, double *rhsb,
                           double *dub, double *diag, double *fspr);
 
+/* RANS turbulence rows, serial (sequential-sweep) form: the
+ * {rans}_{serial}_{lower,upper}_sweep family of doc/1_introduction.md:19-22,
+ * 40-41 (no reference implementation or call site).  Signatures and
+ * arithmetic as the coloured RANS sweeps of coloredlusgs.h, neighbour lower
+ * iff nb < i (upper iff nb > i), as for the NS rows above. */
+int serial_rans_lower_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                            double *vec_fnorm, double *upts, double *dub,
+                            double *tdiag, double *tfspr);
+int serial_rans_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                            double *vec_fnorm, double *upts, double *dub,
+                            double *tdiag, double *tfspr);
+int rans_serial_lower_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                            double *vec_fnorm, double *upts, double *rhsb,
+                            double *dub, double *tdiag, double *tfspr);
+int rans_serial_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                            double *vec_fnorm, double *upts, double *rhsb,
+                            double *dub, double *tdiag, double *tfspr);
+
 /* upts[k,i] += x[k,i] for k < NVARS.
  * replaces: serial_ns_lusgs_update (examples/mpi3d.c:173) and
  * lusgs_serial_ns_update (examples/mpi3d.py:21,26,81). */

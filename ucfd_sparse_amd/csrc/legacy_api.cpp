@@ -84,18 +84,41 @@ int do_pre(const char *fn, int neles, int nface, double factor, const double *fn
     return sg.leave(rc);
 }
 
+// Turbulence rows of the RANS sweeps: NVARS - NFVARS, 1 (Spalart-Allmaras)
+// or 2 (k-omega); 0 when the configuration has none.
+int rans_rows(const char *fn, int nface)
+{
+    const UcfdConfig &c = ucfd_config();
+    const int nt = c.nvars - c.nfvars;
+    if (nt < 1 || nt > 2) {
+        ucfd_report(UCFD_ERR_ARG, fn, "RANS sweeps need NVARS - NFVARS turbulence rows in "
+                                      "{1, 2} (ucfd_set_config)");
+        return 0;
+    }
+    if (nface != 3 && nface != 4 && nface != 6) {
+        ucfd_report(UCFD_ERR_ARG, fn, "RANS sweeps support nface 3, 4 or 6");
+        return 0;
+    }
+    return nt;
+}
+
 // One coloured sweep over icolor[n0:ne).  lower: out = (in - 0.5 df)/diag;
-// upper: out = in - 0.5 df/diag.  in == out is the in-place C form.
+// upper: out = in - 0.5 df/diag.  in == out is the in-place C form.  rans:
+// the turbulence rows (diag / fspr are then the turbulence diagonal and
+// spectral radius), else the NFVARS flow rows.
 int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, const int *nei,
                const int *icolor, const int *lcolor, const double *fnorm_vol,
                const double *vec_fnorm, const double *upts, const double *in, double *out,
-               const double *diag, const double *fspr)
+               const double *diag, const double *fspr, bool rans = false)
 {
     if (bad_dims(neles, nface) || n0 < 0 || ne < n0 || ne > neles)
         return ucfd_report(UCFD_ERR_ARG, fn, "bad n0/ne/neles/nface");
+    const int nt = rans ? rans_rows(fn, nface) : 0;
+    if (rans && !nt) return UCFD_ERR_ARG;
     if (ne == n0) return UCFD_OK;
     const UcfdConfig &c = ucfd_config();
-    const size_t N = (size_t)neles, F = (size_t)nface, D = (size_t)c.ndims, V = (size_t)c.nfvars;
+    const size_t N = (size_t)neles, F = (size_t)nface, D = (size_t)c.ndims,
+                 V = rans ? (size_t)c.nvars : (size_t)c.nfvars;
     UcfdStage sg(fn);
     const int a_nei = sg.add(nei, F*N*4, UCFD_IN);
     const int a_ic = sg.add(icolor, N*4, UCFD_IN);
@@ -112,13 +135,22 @@ int do_colored(const char *fn, int dir, int n0, int ne, int neles, int nface, co
                          ? UCFD_SWEEP_RANGE : UCFD_SWEEP_COLORED;
     int rc = sg.enter();
     if (rc) return rc;
-    rc = ucfd_launch_sweep(ucfd_stream(), dir, mode, n0, ne, neles, nface, c.ndims,
-                           c.nfvars, (const int *)sg.dev(a_nei), (const int *)sg.dev(a_ic),
-                           (const int *)sg.dev(a_lc), (const double *)sg.dev(a_fnv),
-                           (const double *)sg.dev(a_vfn), (const double *)sg.dev(a_u),
-                           (const double *)sg.dev(a_in), (double *)sg.dev(a_out),
-                           (const double *)sg.dev(a_dg), (const double *)sg.dev(a_sp),
-                           ucfd_device_flag(), lc_value);
+    if (rans)
+        rc = ucfd_launch_rans_sweep(ucfd_stream(), dir, mode, n0, ne, neles, nface, c.ndims, nt,
+                                    (const int *)sg.dev(a_nei), (const int *)sg.dev(a_ic),
+                                    (const int *)sg.dev(a_lc), (const double *)sg.dev(a_fnv),
+                                    (const double *)sg.dev(a_vfn), (const double *)sg.dev(a_u),
+                                    (const double *)sg.dev(a_in), (double *)sg.dev(a_out),
+                                    (const double *)sg.dev(a_dg), (const double *)sg.dev(a_sp),
+                                    ucfd_device_flag(), lc_value);
+    else
+        rc = ucfd_launch_sweep(ucfd_stream(), dir, mode, n0, ne, neles, nface, c.ndims,
+                               c.nfvars, (const int *)sg.dev(a_nei), (const int *)sg.dev(a_ic),
+                               (const int *)sg.dev(a_lc), (const double *)sg.dev(a_fnv),
+                               (const double *)sg.dev(a_vfn), (const double *)sg.dev(a_u),
+                               (const double *)sg.dev(a_in), (double *)sg.dev(a_out),
+                               (const double *)sg.dev(a_dg), (const double *)sg.dev(a_sp),
+                               ucfd_device_flag(), lc_value);
     if (rc) ucfd_report(rc, fn, "kernel launch failed");
     return sg.leave(rc);
 }
@@ -178,12 +210,16 @@ void build_levels(int dir, int N, int F, const int *nei, std::vector<int> &order
 
 int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
               const double *fnorm_vol, const double *vec_fnorm, const double *upts,
-              const double *in, double *out, const double *diag, const double *fspr)
+              const double *in, double *out, const double *diag, const double *fspr,
+              bool rans = false)
 {
     if (bad_dims(neles, nface)) return ucfd_report(UCFD_ERR_ARG, fn, "bad neles/nface");
+    const int nt = rans ? rans_rows(fn, nface) : 0;
+    if (rans && !nt) return UCFD_ERR_ARG;
     if (neles == 0) return UCFD_OK;
     const UcfdConfig &c = ucfd_config();
-    const size_t N = (size_t)neles, F = (size_t)nface, D = (size_t)c.ndims, V = (size_t)c.nfvars;
+    const size_t N = (size_t)neles, F = (size_t)nface, D = (size_t)c.ndims,
+                 V = rans ? (size_t)c.nvars : (size_t)c.nfvars;
     int rc = ucfd_ensure_device(fn);
     if (rc) return rc;
 
@@ -253,13 +289,23 @@ int do_serial(const char *fn, int dir, int neles, int nface, const int *nei,
     if (rc) return rc;
     const int nlev = (int)off.size() - 1;
     for (int l = 0; l < nlev && !rc; ++l) {
-        rc = ucfd_launch_sweep(ucfd_stream(), dir, UCFD_SWEEP_ORDERED, off[l], off[l + 1], neles,
-                               nface, c.ndims, c.nfvars, (const int *)sg.dev(a_nei),
-                               (const int *)sg.dev(a_ord), nullptr,
-                               (const double *)sg.dev(a_fnv), (const double *)sg.dev(a_vfn),
-                               (const double *)sg.dev(a_u), (const double *)sg.dev(a_in),
-                               (double *)sg.dev(a_out), (const double *)sg.dev(a_dg),
-                               (const double *)sg.dev(a_sp), ucfd_device_flag(), 0);
+        if (rans)
+            rc = ucfd_launch_rans_sweep(ucfd_stream(), dir, UCFD_SWEEP_ORDERED, off[l], off[l + 1],
+                                        neles, nface, c.ndims, nt, (const int *)sg.dev(a_nei),
+                                        (const int *)sg.dev(a_ord), nullptr,
+                                        (const double *)sg.dev(a_fnv),
+                                        (const double *)sg.dev(a_vfn),
+                                        (const double *)sg.dev(a_u), (const double *)sg.dev(a_in),
+                                        (double *)sg.dev(a_out), (const double *)sg.dev(a_dg),
+                                        (const double *)sg.dev(a_sp), ucfd_device_flag(), 0);
+        else
+            rc = ucfd_launch_sweep(ucfd_stream(), dir, UCFD_SWEEP_ORDERED, off[l], off[l + 1],
+                                   neles, nface, c.ndims, c.nfvars, (const int *)sg.dev(a_nei),
+                                   (const int *)sg.dev(a_ord), nullptr,
+                                   (const double *)sg.dev(a_fnv), (const double *)sg.dev(a_vfn),
+                                   (const double *)sg.dev(a_u), (const double *)sg.dev(a_in),
+                                   (double *)sg.dev(a_out), (const double *)sg.dev(a_dg),
+                                   (const double *)sg.dev(a_sp), ucfd_device_flag(), 0);
     }
     if (rc) ucfd_report(rc, fn, "kernel launch failed");
     return sg.leave(rc);
@@ -385,6 +431,84 @@ UCFD_API int serial_ns_lusgs_update(int neles, double *upts, double *dub)
 UCFD_API int lusgs_serial_ns_update(int neles, double *upts, double *rhsb)
 {
     return do_update("lusgs_serial_ns_update", neles, upts, rhsb);
+}
+
+// ---------------------------------------------------------------- RANS rows
+// The {rans}_{serial,parallel}_{lower,upper}_sweep family named by
+// doc/1_introduction.md:19-22,40-41 (no call site in the reference): the NS
+// argument lists with diag / fspr taken as the turbulence diagonal and
+// spectral radius (the diagonal from {serial,parallel}_pre_lusgs with the
+// turbulence spectral radius and factor = turb_factor, as lusgs.c:23-36,
+// 127-137 form the turbulence rows of diag), acting on the turbulence rows
+// NFVARS .. NVARS-1 of dub / rhsb with the neighbour state upts; arithmetic
+// of lusgs.c:274-321 with rans_flux_container (flux.c:77-90).
+
+UCFD_API int parallel_rans_lower_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                       int *icolor, int *lcolor, double *fnorm_vol,
+                                       double *vec_fnorm, double *upts, double *dub, double *tdiag,
+                                       double *tfspr)
+{
+    return do_colored("parallel_rans_lower_sweep", 0, n0, ne, neles, nface, nei_ele, icolor,
+                      lcolor, fnorm_vol, vec_fnorm, upts, dub, dub, tdiag, tfspr, true);
+}
+
+UCFD_API int parallel_rans_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                       int *icolor, int *lcolor, double *fnorm_vol,
+                                       double *vec_fnorm, double *upts, double *dub, double *tdiag,
+                                       double *tfspr)
+{
+    return do_colored("parallel_rans_upper_sweep", 1, n0, ne, neles, nface, nei_ele, icolor,
+                      lcolor, fnorm_vol, vec_fnorm, upts, dub, dub, tdiag, tfspr, true);
+}
+
+UCFD_API int rans_parallel_lower_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                       int *icolor, int *lcolor, double *fnorm_vol,
+                                       double *vec_fnorm, double *upts, double *rhsb,
+                                       double *dub, double *tdiag, double *tfspr)
+{
+    return do_colored("rans_parallel_lower_sweep", 0, n0, ne, neles, nface, nei_ele, icolor,
+                      lcolor, fnorm_vol, vec_fnorm, upts, rhsb, dub, tdiag, tfspr, true);
+}
+
+UCFD_API int rans_parallel_upper_sweep(int n0, int ne, int neles, int nface, int *nei_ele,
+                                       int *icolor, int *lcolor, double *fnorm_vol,
+                                       double *vec_fnorm, double *upts, double *rhsb,
+                                       double *dub, double *tdiag, double *tfspr)
+{
+    return do_colored("rans_parallel_upper_sweep", 1, n0, ne, neles, nface, nei_ele, icolor,
+                      lcolor, fnorm_vol, vec_fnorm, upts, dub, rhsb, tdiag, tfspr, true);
+}
+
+UCFD_API int serial_rans_lower_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *dub, double *tdiag,
+                                     double *tfspr)
+{
+    return do_serial("serial_rans_lower_sweep", 0, neles, nface, nei_ele, fnorm_vol, vec_fnorm,
+                     upts, dub, dub, tdiag, tfspr, true);
+}
+
+UCFD_API int serial_rans_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *dub, double *tdiag,
+                                     double *tfspr)
+{
+    return do_serial("serial_rans_upper_sweep", 1, neles, nface, nei_ele, fnorm_vol, vec_fnorm,
+                     upts, dub, dub, tdiag, tfspr, true);
+}
+
+UCFD_API int rans_serial_lower_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *rhsb, double *dub,
+                                     double *tdiag, double *tfspr)
+{
+    return do_serial("rans_serial_lower_sweep", 0, neles, nface, nei_ele, fnorm_vol, vec_fnorm,
+                     upts, rhsb, dub, tdiag, tfspr, true);
+}
+
+UCFD_API int rans_serial_upper_sweep(int neles, int nface, int *nei_ele, double *fnorm_vol,
+                                     double *vec_fnorm, double *upts, double *rhsb, double *dub,
+                                     double *tdiag, double *tfspr)
+{
+    return do_serial("rans_serial_upper_sweep", 1, neles, nface, nei_ele, fnorm_vol, vec_fnorm,
+                     upts, dub, rhsb, tdiag, tfspr, true);
 }
 
 // ------------------------------------------------------- declared layouts

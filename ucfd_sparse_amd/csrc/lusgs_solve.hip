@@ -832,6 +832,61 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     return sg.leave(rc);
 }
 
+// Measurement extension: HIP-event times of the solver's kernels, one full
+// pass each (every colour), averaged over reps, on the library stream:
+// ms[0] lower sweep with residual (FP64), ms[1] upper sweep with update
+// (FP64), ms[2] / ms[3] the same in FP32 (the inner kernels of the mixed
+// form), ms[4] the FP64 residual (all colours) and its norm.  The state is a
+// first outer step (dq = 0 with right-hand side b); the solver's work arrays
+// are overwritten.
+UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *b, int reps,
+                                            double *ms)
+{
+    const char *fn = "ucfd_lusgs_solver_time_kernels";
+    if (!s || !b || !ms || reps <= 0) return ucfd_report(UCFD_ERR_ARG, fn, "bad arguments");
+    if (!ucfd_is_device_ptr(b)) return ucfd_report(UCFD_ERR_ARG, fn, "b must be device memory");
+    const hipStream_t st = ucfd_stream();
+    const size_t n = (size_t)NV*s->N;
+    hipEvent_t ev[6];
+    for (int k = 0; k < 6; ++k)
+        if (int rc = ucfd_check_hip(hipEventCreate(&ev[k]), fn, "hipEventCreate")) {
+            for (int j = 0; j < k; ++j) (void)hipEventDestroy(ev[j]);
+            return rc;
+        }
+    // e = dq = 0 (r64 serves as both): FP64 and FP32 state of a first step
+    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->r64, 0.0);
+    hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
+    int rc = residual64(s, st, b, s->r64, nullptr, true, true);
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int r = 0; r < reps && !rc; ++r) {
+        (void)hipEventRecord(ev[0], st);
+        rc = lower_pass<double>(s, st, true, s->q64, b, s->r64, s->dstar64, s->g64, s->d64);
+        (void)hipEventRecord(ev[1], st);
+        if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->r64, s->g64,
+                                         s->d64);
+        (void)hipEventRecord(ev[2], st);
+        if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
+                                        s->d32);
+        (void)hipEventRecord(ev[3], st);
+        if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32,
+                                        s->d32);
+        (void)hipEventRecord(ev[4], st);
+        if (!rc) rc = residual64(s, st, b, s->r64, nullptr, false);
+        (void)hipEventRecord(ev[5], st);
+        if (!rc) rc = ucfd_check_hip(hipEventSynchronize(ev[5]), fn, "hipEventSynchronize");
+        for (int k = 0; k < 5 && !rc; ++k) {
+            float t = 0.f;
+            rc = ucfd_check_hip(hipEventElapsedTime(&t, ev[k], ev[k + 1]), fn,
+                                "hipEventElapsedTime");
+            acc[k] += t;
+        }
+    }
+    for (int k = 0; k < 6; ++k) (void)hipEventDestroy(ev[k]);
+    if (rc) return rc;
+    for (int k = 0; k < 5; ++k) ms[k] = acc[k]/reps;
+    return UCFD_OK;
+}
+
 // ===================================================== no-allocation residual
 //
 // ucfd_lusgs_residual_noalloc: the residual of ucfd_lusgs_residual without a

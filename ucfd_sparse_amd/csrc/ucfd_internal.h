@@ -86,6 +86,17 @@ UCFD_HIDDEN int ucfd_launch_sweep(hipStream_t s, int dir, int mode, int n0, int 
                                   double *out, const double *diag, const double *fspr,
                                   int *flag, int lc_expect);
 
+// The turbulence rows of the legacy RANS sweeps (rows ndims+2 ..
+// ndims+2+nt-1, nt in {1, 2}; nface in {3, 4, 6}); modes and tests as
+// ucfd_launch_sweep.  UCFD_ERR_ARG for an unsupported nface / nt / ndims.
+UCFD_HIDDEN int ucfd_launch_rans_sweep(hipStream_t s, int dir, int mode, int n0, int ne,
+                                       int neles, int nface, int ndims, int nt, const int *nei,
+                                       const int *order, const int *lcolor,
+                                       const double *fnorm_vol, const double *vec_fnorm,
+                                       const double *upts, const double *in, double *out,
+                                       const double *tdiag, const double *tfspr, int *flag,
+                                       int lc_expect);
+
 UCFD_HIDDEN int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x);
 // *out = sum over k < n of a mix of (k, a[k]) mod 2^64 (device array a; out
 // device memory): a content hash, order independent and deterministic.

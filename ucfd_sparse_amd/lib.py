@@ -39,6 +39,15 @@ _SIGS = {
     "ns_serial_lower_sweep": [c_int, c_int] + [c_void_p] * 8,
     "ns_serial_upper_sweep": [c_int, c_int] + [c_void_p] * 8,
     "serial_ns_lusgs_update": [c_int, c_void_p, c_void_p],
+    # RANS turbulence rows (doc/1_introduction.md:19-22; no reference binding)
+    "parallel_rans_lower_sweep": [c_int] * 4 + [c_void_p] * 9,
+    "parallel_rans_upper_sweep": [c_int] * 4 + [c_void_p] * 9,
+    "rans_parallel_lower_sweep": [c_int] * 4 + [c_void_p] * 10,
+    "rans_parallel_upper_sweep": [c_int] * 4 + [c_void_p] * 10,
+    "serial_rans_lower_sweep": [c_int, c_int] + [c_void_p] * 7,
+    "serial_rans_upper_sweep": [c_int, c_int] + [c_void_p] * 7,
+    "rans_serial_lower_sweep": [c_int, c_int] + [c_void_p] * 8,
+    "rans_serial_upper_sweep": [c_int, c_int] + [c_void_p] * 8,
     "lusgs_serial_ns_update": [c_int, c_void_p, c_void_p],
     # extensions (include/ucfd_ext.h)
     "ucfd_set_stream": [c_void_p],
@@ -64,6 +73,7 @@ _SIGS = {
     "ucfd_lusgs_solver_destroy": [c_void_p],
     "ucfd_lusgs_residual": [c_void_p] * 5,
     "ucfd_lusgs_residual_noalloc": [c_int, c_int, c_int] + [c_void_p] * 11,
+    "ucfd_lusgs_solver_time_kernels": [c_void_p, c_void_p, c_int, c_void_p],
     "ucfd_lusgs_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_void_p],
     "ucfd_adjoint_create": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 5,
     "ucfd_adjoint_destroy": [c_void_p],
