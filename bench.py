@@ -581,6 +581,17 @@ def blusgs_workload(args, log):
     SL.call("UCFDLUSGS_NSPrepare", 1.0)
     SL.call("UCFDLUSGS_RANSPrepare", 1.0)
     L.call("ucfd_synchronize")
+    ep = [ev() for _ in range(4)]
+    ep[0].record(stream)
+    SL.call("UCFDLUSGS_Pack", 0, 1.5, 0.3)
+    ep[1].record(stream)
+    SL.call("UCFDLUSGS_NSPrepare", 1.0)
+    ep[2].record(stream)
+    SL.call("UCFDLUSGS_RANSPrepare", 1.0)
+    ep[3].record(stream)
+    L.call("ucfd_synchronize")
+    scalar_prep = {"pack_ms": ep[0].elapsed_time(ep[1]), "ns_prepare_ms": ep[1].elapsed_time(ep[2]),
+                   "rans_prepare_ms": ep[2].elapsed_time(ep[3])}
 
     def lit(evs=None):
         if evs:
@@ -610,12 +621,19 @@ def blusgs_workload(args, log):
     # du (written); per position colidx, slot, side; per face slot the normal
     # (3), area and spectral radius
     b_scalar = N * (4 + 8 + 8 * 5 * 4) + npos * (4 + 4 + 1) + fs_nfaces * 8 * 5
-    scalar = {"ns_lower_upper_ms": ns_l, "rans_lower_upper_ms": rans_l,
+    # the pack: per element the 6 rows of upts, rhs and dsrc and dt read,
+    # the rank rows of u, du and diag written, cell_ids read
+    b_pack = N * (4 + 8 + 6 * 8 * 3 + 6 * 8 * 3)
+    scalar_prep["pack_algorithmic_bytes"] = b_pack
+    scalar_prep["pack_gbs"] = b_pack / (scalar_prep["pack_ms"] * 1e-3) / 1e9
+    scalar = {"ns_lower_upper_ms": ns_l, "rans_lower_upper_ms": rans_l, **scalar_prep,
               "ns_cell_iters_per_s": N / (ns_l * 1e-3),
               "ns_sweep_algorithmic_bytes": b_scalar,
               "ns_sweep_gbs": b_scalar / (0.5 * ns_l) / 1e6}
     log(f"flowsys scalar LU-SGS: NS lower+upper {ns_l:.3f} ms "
-        f"({scalar['ns_sweep_gbs']:.0f} GB/s alg), RANS lower+upper {rans_l:.3f} ms")
+        f"({scalar['ns_sweep_gbs']:.0f} GB/s alg), RANS lower+upper {rans_l:.3f} ms, pack "
+        f"{scalar['pack_ms']:.3f} ms ({scalar['pack_gbs']:.0f} GB/s alg), prepare NS "
+        f"{scalar['ns_prepare_ms']:.3f} RANS {scalar['rans_prepare_ms']:.3f} ms")
     del fsd
     torch.cuda.empty_cache()
 
@@ -1028,7 +1046,9 @@ def blusgs_summary(r):
             "scalar_lusgs": {"ns_lower_upper_ms": sc["ns_lower_upper_ms"],
                              "ns_sweep_gbs": sc["ns_sweep_gbs"],
                              "ns_sweep_frac": sc["ns_sweep_gbs"] / HBM_PEAK_GBS,
-                             "rans_lower_upper_ms": sc["rans_lower_upper_ms"]},
+                             "rans_lower_upper_ms": sc["rans_lower_upper_ms"],
+                             "pack_ms": sc["pack_ms"], "pack_gbs": sc["pack_gbs"],
+                             "ns_prepare_ms": sc["ns_prepare_ms"]},
             "krylov_precon_apply": {k: {"ms": v["ms"], "gbs": v["gbs"],
                                         "frac": v["gbs"] / HBM_PEAK_GBS}
                                     for k, v in r["krylov_precon_apply"].items()},

@@ -42,8 +42,14 @@ UCFD_EXTERN ucfd_status_t UCFDFlowSysDestroy(FlowSys *sys);
 
 /* ucfdlusgs.h:17-25 and flowsys.c:64-81: the scalar LU-SGS method of the
  * FlowSys.  fspr / tfspr: spectral radius per face slot (the pointers are
- * kept and read at every Prepare and sweep); rank arrays u / du / diag are
- * library-owned [nvars, nlocal].  Pack (lusgs.c:8-52): per element, u = uptsb,
+ * kept); rank arrays u / du / diag are library-owned [nvars, nlocal].  Each
+ * Prepare also stores per-position face records (side*normal,
+ * face_area*rcp_vol and the spectral radius of its rows, 40 B per position
+ * plus 8 B for the turbulence radius): the sweeps after a Prepare use the
+ * geometry and fspr / tfspr that Prepare read, bit for bit the same results
+ * as reading them again; a caller that changes them calls Prepare again (as
+ * the diagonal needs anyway).  Sweeps with no Prepare before them read the
+ * caller's arrays.  Pack (lusgs.c:8-52): per element, u = uptsb,
  * du = rhs, diag = 1/(dt*factor) + a0 + dsrc with factor = turb_factor on the
  * turbulence rows.  Prepare (lusgs.c:87-140): diag += sum over the cell's
  * positions of 0.5*kappa*fspr*face_area*rcp_vol (NS rows, or turbulence rows

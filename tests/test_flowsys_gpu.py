@@ -255,3 +255,32 @@ def test_lusgs_state_errors():
     with pytest.raises(L.UCFDError):
         B.call("UCFDLUSGS_NSPrepare", 1.0)
     B.close()
+
+
+@pytest.mark.parametrize("colored", [False, True])
+def test_lusgs_sweeps_without_prepare_and_other_kappa(colored):
+    """The sweeps after a Prepare read the face records it wrote; sweeps
+    with no Prepare before them read the caller's arrays.  Both equal the
+    oracle bit for bit, also with a sweep kappa other than the Prepare's
+    (kappa multiplies the spectral radius in the sweep itself, lusgs.c:196)."""
+    from ucfd_sparse_amd import flowsys as FS
+    mesh, flow, fs, fspr, tfspr, el = lusgs_case(seed=321, m=4, colored=colored)
+    fsd, _, eld = on_device(fs, np.zeros(1), el)
+    for prepared in (False, True):
+        S = FS.FlowSysLUSGS(fsd, 6, 5, torch.from_numpy(fspr).cuda(),
+                            torch.from_numpy(tfspr).cuda())
+        S.set_element(0, eld)
+        Or = O.FlowSysLUSGSOracle(fs, fspr, tfspr, el, 6, 5)
+        S.call("UCFDLUSGS_Pack", 0, 1.3, 0.1)
+        Or.pack(1.3, 0.1)
+        if prepared:
+            S.call("UCFDLUSGS_NSPrepare", 0.9)
+            S.call("UCFDLUSGS_RANSPrepare", 0.9)
+            Or.prepare(0.9)
+            Or.prepare(0.9, turb=True)
+        for name, upper, turb in (("NSLower", False, False), ("NSUpper", True, False),
+                                  ("RANSLower", False, True), ("RANSUpper", True, True)):
+            S.call(f"UCFDLUSGS_{name}Sweep", 1.25)
+            Or.sweep(upper, 1.25, turb=turb)
+            assert np.array_equal(S.export(1), Or.du), (prepared, name)
+        S.close()

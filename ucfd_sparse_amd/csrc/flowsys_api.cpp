@@ -67,6 +67,11 @@ struct _FlowSys {
     // spectral radii (src/lusgs/flowsys.h:20-24)
     double *lu_u = nullptr, *lu_du = nullptr, *lu_diag = nullptr;
     UCFDReal *fspr = nullptr, *tfspr = nullptr;
+    // per-position face records written by the last NS / RANS Prepare
+    // (geometry, spectral radius), read by the sweeps that follow
+    double *lu_geo = nullptr, *lu_lam = nullptr, *lu_tlam = nullptr;
+    int *lu_inv = nullptr;          // Pack's inverse of cell_ids [nlocal]
+    bool lu_geo_ok = false, lu_lam_ok = false, lu_tlam_ok = false;
     int64_t npos = 0;
     bool fp32 = false;
     bool structured = false;
@@ -124,6 +129,11 @@ void free_device(FlowSys s)
     dfree(&s->lu_u);
     dfree(&s->lu_du);
     dfree(&s->lu_diag);
+    dfree(&s->lu_geo);
+    dfree(&s->lu_inv);
+    dfree(&s->lu_lam);
+    dfree(&s->lu_tlam);
+    s->lu_geo_ok = s->lu_lam_ok = s->lu_tlam_ok = false;
     free_packed(s->ns);
     free_packed(s->rans);
     ucfd_free_schedule(s->lower);
@@ -606,7 +616,13 @@ UCFD_API ucfd_status_t UCFDBLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal *o
 // descending, lower / upper neighbours only), run level by level on the
 // schedules of the face-slot structure (one launch per colour on a
 // colour-ordered numbering): results equal the reference bit for bit.
-// fspr / tfspr (per face slot) are the caller's arrays, read at every call.
+// fspr / tfspr (per face slot) are the caller's arrays.  Each Prepare also
+// writes per-position face records (side*normal, face_area*rcp_vol and the
+// spectral radius of its rows) that the sweeps read in position order: the
+// sweeps after a Prepare use the geometry and spectral radius that Prepare
+// saw (the reference reads them again in the sweeps; a caller that changes
+// them calls Prepare again, as it must for the diagonal anyway).  Sweeps
+// with no Prepare before them read the caller's arrays directly.
 
 UCFD_API ucfd_status_t UCFDFlowSysSetLUSGS(FlowSys *sys, UCFDReal *fspr, UCFDReal *tfspr)
 {
@@ -644,13 +660,14 @@ UCFD_API ucfd_status_t UCFDLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal turb_f
     const int ir = sg.add(e->rhs, sizeof(double)*ne*nv, UCFD_IN);
     const int it = sg.add(e->dt, sizeof(double)*ne, UCFD_IN);
     const int isr = sg.add(e->dsrc, sizeof(double)*ne*nv, UCFD_IN);
+    if (dalloc(&sys->lu_inv, (size_t)sys->nlocal, fn)) return UCFD_FAILED;
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     rc = fs_launch_lpack(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, sys->nfvars, turb_factor,
                          a0, (const int *)sg.dev(ic), (const double *)sg.dev(iu),
                          (const double *)sg.dev(ir), (const double *)sg.dev(it),
                          (const double *)sg.dev(isr), sys->lu_u, sys->lu_du, sys->lu_diag,
-                         ucfd_device_flag());
+                         ucfd_device_flag(), sys->lu_inv);
     if (!rc) rc = ucfd_sync_and_check(fn);
     return status(sg.leave(rc));
 }
@@ -704,18 +721,53 @@ ucfd_status_t lusgs_prepare(FlowSys s, bool turb, double kappa, const char *fn)
     const int ifa = sg.add(s->face_area, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
     const int irv = sg.add(s->rcp_vol, sizeof(double)*(size_t)s->nlocal, UCFD_IN);
     const int ilm = sg.add(lam, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
+    const int isd = sg.add(s->sides, (size_t)s->npos, UCFD_IN);
+    const int ifn = sg.add(s->face_normal, sizeof(double)*(size_t)s->nfaces*s->ndims, UCFD_IN);
+    double *&lamp = turb ? s->lu_tlam : s->lu_lam;
+    if (dalloc(&s->lu_geo, 4*(size_t)s->npos, fn) || dalloc(&lamp, (size_t)s->npos, fn))
+        return UCFD_FAILED;
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     rc = fs_launch_lprep(ucfd_stream(), s->nlocal, turb ? s->nfvars : 0, turb ? s->nvars : s->nfvars,
                          kappa, s->rowptr_d, (const int *)sg.dev(isl), (const double *)sg.dev(ifa),
                          (const double *)sg.dev(irv), (const double *)sg.dev(ilm), s->lu_diag);
-    return status(sg.leave(rc));
+    // the geometry records once per Prepare pair: the RANS Prepare after an
+    // NS one (the reference's order) writes only its spectral radius
+    const bool geo = !(turb && s->lu_geo_ok && s->lu_lam_ok);
+    if (!rc)
+        rc = fs_launch_lpos_pack(ucfd_stream(), s->ndims, s->nlocal, s->nfaces, s->rowptr_d,
+                                 (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
+                                 (const double *)sg.dev(ifa), (const double *)sg.dev(ifn),
+                                 (const double *)sg.dev(irv), (const double *)sg.dev(ilm),
+                                 geo ? s->lu_geo : nullptr, lamp);
+    rc = sg.leave(rc);
+    s->lu_geo_ok = rc == UCFD_OK;
+    (turb ? s->lu_tlam_ok : s->lu_lam_ok) = rc == UCFD_OK;
+    return status(rc);
 }
 
 ucfd_status_t lusgs_sweep(FlowSys s, bool turb, bool upper, double kappa, const char *fn)
 {
     const UCFDReal *lam = nullptr;
     if (lusgs_rows(s, turb, fn, &lam)) return UCFD_FAILED;
+    const UcfdSchedule &S = upper ? s->upper : s->lower;
+    if (s->lu_geo_ok && (turb ? s->lu_tlam_ok : s->lu_lam_ok)) {
+        // the face records of the last Prepare (no staging: all device data)
+        int rc = UCFD_OK;
+        for (const UcfdLevel &l : S.lv)
+            if ((rc = fs_launch_lsweep_packed(ucfd_stream(), upper, s->ndims,
+                                              turb ? s->nturbvars : 0, l.base, l.cnt,
+                                              l.ordered ? S.order : nullptr, s->nfvars, kappa,
+                                              s->nlocal, s->rowptr_d, s->colidx_d, s->lu_geo,
+                                              turb ? s->lu_tlam : s->lu_lam, s->lu_diag, s->lu_u,
+                                              s->lu_du)))
+                break;
+        if (!rc) {
+            if (!ucfd_stream_is_user()) rc = ucfd_sync_and_check(fn);
+            else rc = ucfd_check_hip(hipGetLastError(), fn, "kernel launch");
+        }
+        return status(rc);
+    }
     UcfdStage sg(fn);
     const int isd = sg.add(s->sides, (size_t)s->npos, UCFD_IN);
     const int isl = sg.add(s->slots, sizeof(int)*(size_t)s->npos, UCFD_IN);
@@ -725,7 +777,6 @@ ucfd_status_t lusgs_sweep(FlowSys s, bool turb, bool upper, double kappa, const 
     const int ilm = sg.add(lam, sizeof(double)*(size_t)s->nfaces, UCFD_IN);
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
-    const UcfdSchedule &S = upper ? s->upper : s->lower;
     for (const UcfdLevel &l : S.lv)
         if ((rc = fs_launch_lsweep(ucfd_stream(), upper, s->ndims, turb ? s->nturbvars : 0, l.base,
                                    l.cnt, l.ordered ? S.order : nullptr, s->nvars, s->nfvars,

@@ -48,17 +48,33 @@ UCFD_HIDDEN int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int c
 UCFD_HIDDEN int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c);
 
 // Scalar LU-SGS (src/lusgs/lusgs.c); rank arrays u / du / diag are
-// [nvars, nlocal].  lusgs_pack / rank_lusgs_update / pre_lusgs:
+// [nvars, nlocal].  lusgs_pack / rank_lusgs_update / pre_lusgs.  lpack with
+// inv (an nlocal int workspace) writes in rank order through the inverse of
+// cell_ids; inv NULL scatters in element order.
 UCFD_HIDDEN int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
                                 double turb_factor, double a0, const int *cell_ids,
                                 const double *upts, const double *rhs, const double *dt,
                                 const double *dsrc, double *u, double *du, double *diag,
-                                int *flag);
+                                int *flag, int *inv);
 UCFD_HIDDEN int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars,
                                   const int *cell_ids, const double *du, double *upts, int *flag);
 UCFD_HIDDEN int fs_launch_lprep(hipStream_t s, int nlocal, int nv0, int nv1, double kappa,
                                 const int *rowptr, const int *slots, const double *face_area,
                                 const double *rcp_vol, const double *lambdaf, double *diag);
+// Per-position face records of the scalar method (written at Prepare):
+// geo[p] = {side*normal (ndims), face_area*rcp_vol} as 4 doubles (geo may be
+// NULL: only lam), lam[p] = lambdaf[slot of p].
+UCFD_HIDDEN int fs_launch_lpos_pack(hipStream_t s, int ndims, int nlocal, int64_t nfaces,
+                                    const int *rowptr, const int *slots,
+                                    const signed char *sides, const double *face_area,
+                                    const double *face_normal, const double *rcp_vol,
+                                    const double *lambdaf, void *geo, double *lam);
+// A sweep level reading those records (same results as fs_launch_lsweep).
+UCFD_HIDDEN int fs_launch_lsweep_packed(hipStream_t s, bool upper, int ndims, int nt, int base,
+                                        int cnt, const int *order, int nfvars, double kappa,
+                                        int nlocal, const int *rowptr, const int *colidx,
+                                        const void *geo, const double *lam, const double *diag,
+                                        const double *u, double *du);
 // One level of the lower (upper = false) or upper sweep: NS rows (nt = 0,
 // nfvars = ndims + 2) or the nt turbulence rows (1 <= nt <= 4).
 UCFD_HIDDEN int fs_launch_lsweep(hipStream_t s, bool upper, int ndims, int nt, int base, int cnt,

@@ -593,6 +593,56 @@ __global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, 
 }
 
 // rank_lusgs_update (lusgs.c:54-70)
+// lusgs_pack in rank order: inv[r] = the element whose cell_ids entry is r
+// (k_lpack_inv, one 4-byte scatter per element), then one thread per rank
+// gathers the element's rows and writes the rank rows coalesced.  Same
+// values as the element-order scatter of k_lpack (each rank row is a copy or
+// the same expression), without its 18 scattered 8-byte stores per element.
+__global__ __launch_bounds__(FB) void k_lpack_inv(int neles, int nlocal,
+                                                  const int *__restrict__ cell_ids,
+                                                  int *__restrict__ inv, int *flag)
+{
+    const int e = fxcd()*FB + threadIdx.x;
+    if (e >= neles) return;
+    const int r = cell_ids[e];
+    if ((unsigned)r >= (unsigned)nlocal) {
+        atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    inv[r] = e;
+}
+
+__global__ __launch_bounds__(FB) void k_lpack_rank(int neles, int nlocal, int nvars, int nfvars,
+                                                   double turb_factor, double a0,
+                                                   const int *__restrict__ inv,
+                                                   const double *__restrict__ upts,
+                                                   const double *__restrict__ rhs,
+                                                   const double *__restrict__ dt,
+                                                   const double *__restrict__ dsrc,
+                                                   double *__restrict__ u,
+                                                   double *__restrict__ du,
+                                                   double *__restrict__ diag)
+{
+    const int r = fxcd()*FB + threadIdx.x;
+    if (r >= nlocal) return;
+    const int e = inv[r];
+    if (e < 0) return;                  // a rank of another element type
+    const double dte = dt[e];
+    for (int k = 0; k < nvars; ++k) {
+        const int64_t ri = r + (int64_t)k*nlocal, ei = e + (int64_t)k*neles;
+        const double factor = k < nfvars ? 1.0 : turb_factor;
+        u[ri] = upts[ei];
+        du[ri] = rhs[ei];
+        diag[ri] = 1.0/(dte*factor) + a0 + dsrc[ei];
+    }
+}
+
+__global__ __launch_bounds__(FB) void k_fill_int(int64_t n, int *a, int v)
+{
+    const int64_t st = (int64_t)gridDim.x*FB;
+    for (int64_t i = (int64_t)blockIdx.x*FB + threadIdx.x; i < n; i += st) a[i] = v;
+}
+
 __global__ __launch_bounds__(FB) void k_lupdate(int neles, int nlocal, int nvars,
                                                 const int *__restrict__ cell_ids,
                                                 const double *__restrict__ du,
@@ -753,6 +803,109 @@ __global__ __launch_bounds__(FB) void k_lsweep(int base, int cnt, const int *__r
         LFace<ND, NT> G;
         lface_load<ND, NT>(G, nb, slots[p], (double)sides[p], rv, kappa, nfvars, nlocal, nfaces,
                            lambdaf, face_area, face_normal, u, du);
+        lface_add<ND, NT>(G, df);
+    }
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) {
+        const int64_t l = r + (int64_t)(k + nv0)*nlocal;
+        if (UPPER) du[l] -= (0.5*df[k]/diag[l]);
+        else du[l] = (du[l] - 0.5*df[k])/diag[l];
+    }
+}
+
+// Per-position face records of the scalar method, written at Prepare: geo[p]
+// = {side*n (ND values), face_area[slot]*rcp_vol[cell]} padded to 4 doubles,
+// lam[p] = lambdaf[slot].  The sweeps then read a cell's records in position
+// order (coalesced) instead of gathering five values through the slot index;
+// every stored value is the product the sweep itself forms (lusgs.c:185-195),
+// so the results do not change by a bit.
+template <int ND>
+__global__ __launch_bounds__(FB) void k_lpos_pack(int nlocal, int64_t nfaces,
+                                                  const int *__restrict__ rowptr,
+                                                  const int *__restrict__ slots,
+                                                  const signed char *__restrict__ sides,
+                                                  const double *__restrict__ face_area,
+                                                  const double *__restrict__ face_normal,
+                                                  const double *__restrict__ rcp_vol,
+                                                  const double *__restrict__ lambdaf,
+                                                  double4 *__restrict__ geo,
+                                                  double *__restrict__ lam)
+{
+    const int r = fxcd()*FB + threadIdx.x;
+    if (r >= nlocal) return;
+    const double rv = rcp_vol[r];
+    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+        const int64_t sl = slots[p];
+        if (geo) {
+            const double side = (double)sides[p];
+            double g[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < ND; ++d) g[d] = side*face_normal[sl + (int64_t)d*nfaces];
+            g[ND] = face_area[sl]*rv;
+            geo[p] = double4{g[0], g[1], g[2], g[3]};
+        }
+        lam[p] = lambdaf[sl];
+    }
+}
+
+// k_lsweep reading the per-position records of k_lpos_pack
+template <int ND, int NT, bool UPPER>
+__global__ __launch_bounds__(FB) void k_lsweep_p(int base, int cnt, const int *__restrict__ order,
+                                                 int nfvars, double kappa, int nlocal,
+                                                 const int *__restrict__ rowptr,
+                                                 const int *__restrict__ colidx,
+                                                 const double4 *__restrict__ geo,
+                                                 const double *__restrict__ lam,
+                                                 const double *__restrict__ diag,
+                                                 const double *__restrict__ u, double *du)
+{
+    constexpr int NF = ND + 2, DNV = LFace<ND, NT>::DNV;
+    const int t = fxcd()*FB + threadIdx.x;
+    if (t >= cnt) return;
+    const int r = order ? order[base + t] : base + t;
+    const int nv0 = NT > 0 ? nfvars : 0;
+    const int p0 = rowptr[r], p1 = rowptr[r + 1];
+    double df[DNV];
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) df[k] = 0.0;
+    int64_t nbk[LKP];
+    bool tk[LKP];
+#pragma unroll
+    for (int k = 0; k < LKP; ++k) {
+        tk[k] = false;
+        if (p0 + k < p1) {
+            nbk[k] = colidx[p0 + k];
+            tk[k] = UPPER ? nbk[k] > r : nbk[k] < r;
+        }
+    }
+    LFace<ND, NT> F[LKP];
+    auto load = [&](LFace<ND, NT> &G, int64_t nb, int p) {
+        const double4 g = geo[p];
+        const double gv[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+        for (int d = 0; d < ND; ++d) G.nf[d] = gv[d];
+        G.fv = gv[ND];
+        G.lam = kappa*lam[p];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) G.uf[k] = u[nb + (int64_t)k*nlocal];
+#pragma unroll
+        for (int k = 0; k < DNV; ++k) G.dnb[k] = du[nb + (int64_t)(k + nv0)*nlocal];
+        if (NT > 0) {
+#pragma unroll
+            for (int k = 0; k < DNV; ++k) G.ut[k] = u[nb + (int64_t)(nfvars + k)*nlocal];
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < LKP; ++k)
+        if (tk[k]) load(F[k], nbk[k], p0 + k);
+#pragma unroll
+    for (int k = 0; k < LKP; ++k)
+        if (tk[k]) lface_add<ND, NT>(F[k], df);
+    for (int p = p0 + LKP; p < p1; ++p) {
+        const int64_t nb = colidx[p];
+        if (UPPER ? !(nb > r) : !(nb < r)) continue;
+        LFace<ND, NT> G;
+        load(G, nb, p);
         lface_add<ND, NT>(G, df);
     }
 #pragma unroll
@@ -930,11 +1083,21 @@ int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c)
 int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars, double turb_factor,
                     double a0, const int *cell_ids, const double *upts, const double *rhs,
                     const double *dt, const double *dsrc, double *u, double *du, double *diag,
-                    int *flag)
+                    int *flag, int *inv)
 {
     if (neles <= 0) return UCFD_OK;
-    hipLaunchKernelGGL(k_lpack, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
-                       turb_factor, a0, cell_ids, upts, rhs, dt, dsrc, u, du, diag, flag);
+    if (!inv) {
+        hipLaunchKernelGGL(k_lpack, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
+                           turb_factor, a0, cell_ids, upts, rhs, dt, dsrc, u, du, diag, flag);
+        return ok();
+    }
+    const int64_t nb = ((int64_t)nlocal + FB - 1)/FB;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(FB), 0, s,
+                       (int64_t)nlocal, inv, -1);
+    hipLaunchKernelGGL(k_lpack_inv, grid(neles), dim3(FB), 0, s, neles, nlocal, cell_ids, inv,
+                       flag);
+    hipLaunchKernelGGL(k_lpack_rank, grid(nlocal), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
+                       turb_factor, a0, inv, upts, rhs, dt, dsrc, u, du, diag);
     return ok();
 }
 
@@ -1016,5 +1179,67 @@ int fs_launch_lsweep(hipStream_t s, bool upper, int ndims, int nt, int base, int
         return lsweep_nt<2>(s, upper, nt, base, cnt, order, nvars, nfvars, kappa, lambdaf, nlocal,
                             nfaces, rowptr, colidx, sides, slots, face_area, face_normal, rcp_vol,
                             diag, u, du);
+    return UCFD_ERR_ARG;
+}
+
+int fs_launch_lpos_pack(hipStream_t s, int ndims, int nlocal, int64_t nfaces, const int *rowptr,
+                        const int *slots, const signed char *sides, const double *face_area,
+                        const double *face_normal, const double *rcp_vol, const double *lambdaf,
+                        void *geo, double *lam)
+{
+    if (nlocal <= 0) return UCFD_OK;
+    if (ndims == 3)
+        hipLaunchKernelGGL(k_lpos_pack<3>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, rowptr,
+                           slots, sides, face_area, face_normal, rcp_vol, lambdaf, (double4 *)geo,
+                           lam);
+    else if (ndims == 2)
+        hipLaunchKernelGGL(k_lpos_pack<2>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, rowptr,
+                           slots, sides, face_area, face_normal, rcp_vol, lambdaf, (double4 *)geo,
+                           lam);
+    else
+        return UCFD_ERR_ARG;
+    return ok();
+}
+
+namespace {
+
+template <int ND>
+int lsweep_p_nt(hipStream_t s, bool upper, int nt, int base, int cnt, const int *order,
+                int nfvars, double kappa, int nlocal, const int *rowptr, const int *colidx,
+                const double4 *geo, const double *lam, const double *diag, const double *u,
+                double *du)
+{
+#define LP(NT) do { \
+    if (upper) hipLaunchKernelGGL((k_lsweep_p<ND, NT, true>), grid(cnt), dim3(FB), 0, s, base, \
+                                  cnt, order, nfvars, kappa, nlocal, rowptr, colidx, geo, lam, \
+                                  diag, u, du); \
+    else hipLaunchKernelGGL((k_lsweep_p<ND, NT, false>), grid(cnt), dim3(FB), 0, s, base, cnt, \
+                            order, nfvars, kappa, nlocal, rowptr, colidx, geo, lam, diag, u, du); \
+    return ok(); } while (0)
+    switch (nt) {
+    case 0: LP(0);
+    case 1: LP(1);
+    case 2: LP(2);
+    case 3: LP(3);
+    case 4: LP(4);
+    default: return UCFD_ERR_ARG;
+    }
+#undef LP
+}
+
+}  // namespace
+
+int fs_launch_lsweep_packed(hipStream_t s, bool upper, int ndims, int nt, int base, int cnt,
+                            const int *order, int nfvars, double kappa, int nlocal,
+                            const int *rowptr, const int *colidx, const void *geo,
+                            const double *lam, const double *diag, const double *u, double *du)
+{
+    if (cnt <= 0) return UCFD_OK;
+    if (ndims == 3)
+        return lsweep_p_nt<3>(s, upper, nt, base, cnt, order, nfvars, kappa, nlocal, rowptr,
+                              colidx, (const double4 *)geo, lam, diag, u, du);
+    if (ndims == 2)
+        return lsweep_p_nt<2>(s, upper, nt, base, cnt, order, nfvars, kappa, nlocal, rowptr,
+                              colidx, (const double4 *)geo, lam, diag, u, du);
     return UCFD_ERR_ARG;
 }
