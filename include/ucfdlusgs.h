@@ -45,11 +45,12 @@ UCFD_EXTERN ucfd_status_t UCFDFlowSysDestroy(FlowSys *sys);
  * kept); rank arrays u / du / diag are library-owned [nvars, nlocal].  Each
  * Prepare also stores per-position face records (side*normal,
  * face_area*rcp_vol and the spectral radius of its rows, 40 B per position
- * plus 8 B for the turbulence radius): the sweeps after a Prepare use the
- * geometry and fspr / tfspr that Prepare read, bit for bit the same results
- * as reading them again; a caller that changes them calls Prepare again (as
- * the diagonal needs anyway).  Sweeps with no Prepare before them read the
- * caller's arrays.  Pack (lusgs.c:8-52): per element, u = uptsb,
+ * plus 8 B for the turbulence radius): NS (RANS) sweeps use the fspr
+ * (tfspr) that the last NS (RANS) Prepare read and the geometry that the
+ * last NS Prepare read (or a RANS Prepare with none before it), bit for bit
+ * the same results as reading them again; a caller that changes them calls
+ * Prepare again (as the diagonal needs anyway).  Sweeps with no Prepare
+ * before them read the caller's arrays.  Pack (lusgs.c:8-52): per element, u = uptsb,
  * du = rhs, diag = 1/(dt*factor) + a0 + dsrc with factor = turb_factor on the
  * turbulence rows.  Prepare (lusgs.c:87-140): diag += sum over the cell's
  * positions of 0.5*kappa*fspr*face_area*rcp_vol (NS rows, or turbulence rows

@@ -266,17 +266,21 @@ def test_lusgs_sweeps_without_prepare_and_other_kappa(colored):
     from ucfd_sparse_amd import flowsys as FS
     mesh, flow, fs, fspr, tfspr, el = lusgs_case(seed=321, m=4, colored=colored)
     fsd, _, eld = on_device(fs, np.zeros(1), el)
-    for prepared in (False, True):
+    # none: no Prepare; both: NS then RANS Prepare (the RANS one reuses the
+    # geometry records); rans: a RANS Prepare alone writes the geometry, the
+    # NS sweeps read the caller's arrays
+    for prepared in ("none", "both", "rans"):
         S = FS.FlowSysLUSGS(fsd, 6, 5, torch.from_numpy(fspr).cuda(),
                             torch.from_numpy(tfspr).cuda())
         S.set_element(0, eld)
         Or = O.FlowSysLUSGSOracle(fs, fspr, tfspr, el, 6, 5)
         S.call("UCFDLUSGS_Pack", 0, 1.3, 0.1)
         Or.pack(1.3, 0.1)
-        if prepared:
+        if prepared == "both":
             S.call("UCFDLUSGS_NSPrepare", 0.9)
-            S.call("UCFDLUSGS_RANSPrepare", 0.9)
             Or.prepare(0.9)
+        if prepared != "none":
+            S.call("UCFDLUSGS_RANSPrepare", 0.9)
             Or.prepare(0.9, turb=True)
         for name, upper, turb in (("NSLower", False, False), ("NSUpper", True, False),
                                   ("RANSLower", False, True), ("RANSUpper", True, True)):

@@ -619,10 +619,11 @@ UCFD_API ucfd_status_t UCFDBLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal *o
 // fspr / tfspr (per face slot) are the caller's arrays.  Each Prepare also
 // writes per-position face records (side*normal, face_area*rcp_vol and the
 // spectral radius of its rows) that the sweeps read in position order: the
-// sweeps after a Prepare use the geometry and spectral radius that Prepare
-// saw (the reference reads them again in the sweeps; a caller that changes
-// them calls Prepare again, as it must for the diagonal anyway).  Sweeps
-// with no Prepare before them read the caller's arrays directly.
+// sweeps use the spectral radius their rows' last Prepare saw and the
+// geometry of the last NS Prepare (the reference reads them again in the
+// sweeps; a caller that changes them calls Prepare again, as it must for the
+// diagonal anyway).  Sweeps with no Prepare before them read the caller's
+// arrays directly.
 
 UCFD_API ucfd_status_t UCFDFlowSysSetLUSGS(FlowSys *sys, UCFDReal *fspr, UCFDReal *tfspr)
 {
@@ -728,22 +729,30 @@ ucfd_status_t lusgs_prepare(FlowSys s, bool turb, double kappa, const char *fn)
         return UCFD_FAILED;
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
-    rc = fs_launch_lprep(ucfd_stream(), s->nlocal, turb ? s->nfvars : 0, turb ? s->nvars : s->nfvars,
-                         kappa, s->rowptr_d, (const int *)sg.dev(isl), (const double *)sg.dev(ifa),
-                         (const double *)sg.dev(irv), (const double *)sg.dev(ilm), s->lu_diag);
-    // the geometry records once per Prepare pair: the RANS Prepare after an
-    // NS one (the reference's order) writes only its spectral radius
+    // the geometry records are written by every NS Prepare; a RANS Prepare
+    // after one (the reference's order) writes only its spectral radius
     const bool geo = !(turb && s->lu_geo_ok && s->lu_lam_ok);
-    if (!rc)
-        rc = fs_launch_lpos_pack(ucfd_stream(), s->ndims, s->nlocal, s->nfaces, s->rowptr_d,
-                                 (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
-                                 (const double *)sg.dev(ifa), (const double *)sg.dev(ifn),
-                                 (const double *)sg.dev(irv), (const double *)sg.dev(ilm),
-                                 geo ? s->lu_geo : nullptr, lamp);
+    rc = fs_launch_lprep_pos(ucfd_stream(), s->ndims, s->nlocal, s->nfaces,
+                             turb ? s->nfvars : 0, turb ? s->nvars : s->nfvars, kappa, s->rowptr_d,
+                             (const int *)sg.dev(isl), (const signed char *)sg.dev(isd),
+                             (const double *)sg.dev(ifa), (const double *)sg.dev(ifn),
+                             (const double *)sg.dev(irv), (const double *)sg.dev(ilm), s->lu_diag,
+                             geo ? s->lu_geo : nullptr, lamp);
     rc = sg.leave(rc);
     s->lu_geo_ok = rc == UCFD_OK;
     (turb ? s->lu_tlam_ok : s->lu_lam_ok) = rc == UCFD_OK;
     return status(rc);
+}
+
+// The first level of a schedule holds the cells with no neighbour visited
+// before them (ucfd_build_levels gives a cell the level after its earlier
+// neighbours'), so the scalar sweep takes no face there: a kernel that reads
+// only du and diag.
+int lusgs_first_level(FlowSys s, bool turb, bool upper, const UcfdLevel &l, const UcfdSchedule &S)
+{
+    return fs_launch_lsweep_free(ucfd_stream(), upper, l.base, l.cnt, l.ordered ? S.order : nullptr,
+                                 turb ? s->nfvars : 0, turb ? s->nvars : s->nfvars, s->nlocal,
+                                 s->lu_diag, s->lu_du);
 }
 
 ucfd_status_t lusgs_sweep(FlowSys s, bool turb, bool upper, double kappa, const char *fn)
@@ -755,7 +764,8 @@ ucfd_status_t lusgs_sweep(FlowSys s, bool turb, bool upper, double kappa, const 
         // the face records of the last Prepare (no staging: all device data)
         int rc = UCFD_OK;
         for (const UcfdLevel &l : S.lv)
-            if ((rc = fs_launch_lsweep_packed(ucfd_stream(), upper, s->ndims,
+            if ((rc = &l == &S.lv.front() ? lusgs_first_level(s, turb, upper, l, S) :
+                      fs_launch_lsweep_packed(ucfd_stream(), upper, s->ndims,
                                               turb ? s->nturbvars : 0, l.base, l.cnt,
                                               l.ordered ? S.order : nullptr, s->nfvars, kappa,
                                               s->nlocal, s->rowptr_d, s->colidx_d, s->lu_geo,
@@ -778,7 +788,8 @@ ucfd_status_t lusgs_sweep(FlowSys s, bool turb, bool upper, double kappa, const 
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     for (const UcfdLevel &l : S.lv)
-        if ((rc = fs_launch_lsweep(ucfd_stream(), upper, s->ndims, turb ? s->nturbvars : 0, l.base,
+        if ((rc = &l == &S.lv.front() ? lusgs_first_level(s, turb, upper, l, S) :
+                  fs_launch_lsweep(ucfd_stream(), upper, s->ndims, turb ? s->nturbvars : 0, l.base,
                                    l.cnt, l.ordered ? S.order : nullptr, s->nvars, s->nfvars,
                                    kappa, (const double *)sg.dev(ilm), s->nlocal, s->nfaces,
                                    s->rowptr_d, s->colidx_d, (const signed char *)sg.dev(isd),

@@ -58,18 +58,21 @@ UCFD_HIDDEN int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars,
                                 int *flag, int *inv);
 UCFD_HIDDEN int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars,
                                   const int *cell_ids, const double *du, double *upts, int *flag);
-UCFD_HIDDEN int fs_launch_lprep(hipStream_t s, int nlocal, int nv0, int nv1, double kappa,
-                                const int *rowptr, const int *slots, const double *face_area,
-                                const double *rcp_vol, const double *lambdaf, double *diag);
-// Per-position face records of the scalar method (written at Prepare):
-// geo[p] = {side*normal (ndims), face_area*rcp_vol} as 4 doubles (geo may be
-// NULL: only lam), lam[p] = lambdaf[slot of p].
-UCFD_HIDDEN int fs_launch_lpos_pack(hipStream_t s, int ndims, int nlocal, int64_t nfaces,
-                                    const int *rowptr, const int *slots,
+// Scalar Prepare (lusgs.c:87-113): diag rows nv0..nv1-1 += the cell's face
+// sum, and the per-position face records the sweeps read: geo[p] =
+// {side*normal (ndims), face_area*rcp_vol} as 4 doubles (geo may be NULL:
+// only lam), lam[p] = lambdaf[slot of p].
+UCFD_HIDDEN int fs_launch_lprep_pos(hipStream_t s, int ndims, int nlocal, int64_t nfaces, int nv0,
+                                    int nv1, double kappa, const int *rowptr, const int *slots,
                                     const signed char *sides, const double *face_area,
                                     const double *face_normal, const double *rcp_vol,
-                                    const double *lambdaf, void *geo, double *lam);
+                                    const double *lambdaf, double *diag, void *geo, double *lam);
 // A sweep level reading those records (same results as fs_launch_lsweep).
+// The first level of a scalar sweep schedule (no face taken): rows
+// nv0..nv1-1 of du updated with a zero face sum (same bits as the sweeps).
+UCFD_HIDDEN int fs_launch_lsweep_free(hipStream_t s, bool upper, int base, int cnt,
+                                      const int *order, int nv0, int nv1, int nlocal,
+                                      const double *diag, double *du);
 UCFD_HIDDEN int fs_launch_lsweep_packed(hipStream_t s, bool upper, int ndims, int nt, int base,
                                         int cnt, const int *order, int nfvars, double kappa,
                                         int nlocal, const int *rowptr, const int *colidx,

@@ -658,26 +658,6 @@ __global__ __launch_bounds__(FB) void k_lupdate(int neles, int nlocal, int nvars
     for (int k = 0; k < nvars; ++k) upts[e + (int64_t)k*neles] += du[r + (int64_t)k*nlocal];
 }
 
-// pre_lusgs (lusgs.c:87-113): the face sum first, then added to rows nv0..nv1-1
-__global__ __launch_bounds__(FB) void k_lprep(int nlocal, int nv0, int nv1, double kappa,
-                                              const int *__restrict__ rowptr,
-                                              const int *__restrict__ slots,
-                                              const double *__restrict__ face_area,
-                                              const double *__restrict__ rcp_vol,
-                                              const double *__restrict__ lambdaf, double *diag)
-{
-    const int r = fxcd()*FB + threadIdx.x;
-    if (r >= nlocal) return;
-    const double rv = rcp_vol[r];
-    double sd = 0.0;
-    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
-        const int64_t sl = slots[p];
-        const double lamf = lambdaf[sl]*kappa;
-        sd += 0.5*lamf*face_area[sl]*rv;
-    }
-    for (int k = nv0; k < nv1; ++k) diag[r + (int64_t)k*nlocal] += sd;
-}
-
 // One level of lower_sweep / upper_sweep (lusgs.c:157-257): rows
 // base..base+cnt-1 (or order[...]), in place on du.  NS rows (NT = 0) with
 // ns_flux_container, turbulence rows (NT turbulence variables) with
@@ -813,42 +793,49 @@ __global__ __launch_bounds__(FB) void k_lsweep(int base, int cnt, const int *__r
     }
 }
 
-// Per-position face records of the scalar method, written at Prepare: geo[p]
-// = {side*n (ND values), face_area[slot]*rcp_vol[cell]} padded to 4 doubles,
-// lam[p] = lambdaf[slot].  The sweeps then read a cell's records in position
-// order (coalesced) instead of gathering five values through the slot index;
-// every stored value is the product the sweep itself forms (lusgs.c:185-195),
-// so the results do not change by a bit.
+// pre_lusgs (lusgs.c:87-113) with the per-position face records of the
+// scalar method: the face sum sd = sum_p 0.5*kappa*lambdaf*face_area*rcp_vol
+// is added to rows nv0..nv1-1 of diag, and the same pass writes geo[p] =
+// {side*n (ND values), face_area[slot]*rcp_vol[cell]} padded to 4 doubles
+// (when geo is given) and lam[p] = lambdaf[slot].  The sweeps then read a
+// cell's records in position order (coalesced) instead of gathering five
+// values through the slot index; every stored value is the product the
+// sweep itself forms (lusgs.c:185-195), so the results do not change by a
+// bit.
 template <int ND>
-__global__ __launch_bounds__(FB) void k_lpos_pack(int nlocal, int64_t nfaces,
-                                                  const int *__restrict__ rowptr,
+__global__ __launch_bounds__(FB) void k_lprep_pos(int nlocal, int64_t nfaces, int nv0, int nv1,
+                                                  double kappa, const int *__restrict__ rowptr,
                                                   const int *__restrict__ slots,
                                                   const signed char *__restrict__ sides,
                                                   const double *__restrict__ face_area,
                                                   const double *__restrict__ face_normal,
                                                   const double *__restrict__ rcp_vol,
                                                   const double *__restrict__ lambdaf,
-                                                  double4 *__restrict__ geo,
+                                                  double *diag, double4 *__restrict__ geo,
                                                   double *__restrict__ lam)
 {
     const int r = fxcd()*FB + threadIdx.x;
     if (r >= nlocal) return;
     const double rv = rcp_vol[r];
+    double sd = 0.0;
     for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
         const int64_t sl = slots[p];
+        const double lf = lambdaf[sl], fa = face_area[sl];
+        sd += 0.5*(lf*kappa)*fa*rv;
         if (geo) {
             const double side = (double)sides[p];
             double g[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < ND; ++d) g[d] = side*face_normal[sl + (int64_t)d*nfaces];
-            g[ND] = face_area[sl]*rv;
+            g[ND] = fa*rv;
             geo[p] = double4{g[0], g[1], g[2], g[3]};
         }
-        lam[p] = lambdaf[sl];
+        lam[p] = lf;
     }
+    for (int k = nv0; k < nv1; ++k) diag[r + (int64_t)k*nlocal] += sd;
 }
 
-// k_lsweep reading the per-position records of k_lpos_pack
+// k_lsweep reading the per-position records of k_lprep_pos
 template <int ND, int NT, bool UPPER>
 __global__ __launch_bounds__(FB) void k_lsweep_p(int base, int cnt, const int *__restrict__ order,
                                                  int nfvars, double kappa, int nlocal,
@@ -913,6 +900,28 @@ __global__ __launch_bounds__(FB) void k_lsweep_p(int base, int cnt, const int *_
         const int64_t l = r + (int64_t)(k + nv0)*nlocal;
         if (UPPER) du[l] -= (0.5*df[k]/diag[l]);
         else du[l] = (du[l] - 0.5*df[k])/diag[l];
+    }
+}
+
+// The first level of a scalar sweep schedule: cells with no neighbour
+// visited before them (no lower neighbour in the lower sweep, no upper one
+// in the upper sweep), so no face is taken.  The update is k_lsweep_p's with
+// its face sum still +0.0, written out literally so the bits agree (0.5*0.0
+// divided by diag keeps diag's sign); no index or face data is read.
+template <bool UPPER>
+__global__ __launch_bounds__(FB) void k_lsweep_free(int base, int cnt,
+                                                    const int *__restrict__ order, int nv0,
+                                                    int nv1, int nlocal,
+                                                    const double *__restrict__ diag, double *du)
+{
+    const int t = fxcd()*FB + threadIdx.x;
+    if (t >= cnt) return;
+    const int r = order ? order[base + t] : base + t;
+    const double df = 0.0;
+    for (int k = nv0; k < nv1; ++k) {
+        const int64_t l = r + (int64_t)k*nlocal;
+        if (UPPER) du[l] -= (0.5*df/diag[l]);
+        else du[l] = (du[l] - 0.5*df)/diag[l];
     }
 }
 
@@ -1110,16 +1119,6 @@ int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars, const int
     return ok();
 }
 
-int fs_launch_lprep(hipStream_t s, int nlocal, int nv0, int nv1, double kappa, const int *rowptr,
-                    const int *slots, const double *face_area, const double *rcp_vol,
-                    const double *lambdaf, double *diag)
-{
-    if (nlocal <= 0) return UCFD_OK;
-    hipLaunchKernelGGL(k_lprep, grid(nlocal), dim3(FB), 0, s, nlocal, nv0, nv1, kappa, rowptr,
-                       slots, face_area, rcp_vol, lambdaf, diag);
-    return ok();
-}
-
 namespace {
 
 template <int ND, int NT>
@@ -1182,20 +1181,20 @@ int fs_launch_lsweep(hipStream_t s, bool upper, int ndims, int nt, int base, int
     return UCFD_ERR_ARG;
 }
 
-int fs_launch_lpos_pack(hipStream_t s, int ndims, int nlocal, int64_t nfaces, const int *rowptr,
-                        const int *slots, const signed char *sides, const double *face_area,
-                        const double *face_normal, const double *rcp_vol, const double *lambdaf,
-                        void *geo, double *lam)
+int fs_launch_lprep_pos(hipStream_t s, int ndims, int nlocal, int64_t nfaces, int nv0, int nv1,
+                        double kappa, const int *rowptr, const int *slots, const signed char *sides,
+                        const double *face_area, const double *face_normal, const double *rcp_vol,
+                        const double *lambdaf, double *diag, void *geo, double *lam)
 {
     if (nlocal <= 0) return UCFD_OK;
     if (ndims == 3)
-        hipLaunchKernelGGL(k_lpos_pack<3>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, rowptr,
-                           slots, sides, face_area, face_normal, rcp_vol, lambdaf, (double4 *)geo,
-                           lam);
+        hipLaunchKernelGGL(k_lprep_pos<3>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, nv0, nv1,
+                           kappa, rowptr, slots, sides, face_area, face_normal, rcp_vol, lambdaf,
+                           diag, (double4 *)geo, lam);
     else if (ndims == 2)
-        hipLaunchKernelGGL(k_lpos_pack<2>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, rowptr,
-                           slots, sides, face_area, face_normal, rcp_vol, lambdaf, (double4 *)geo,
-                           lam);
+        hipLaunchKernelGGL(k_lprep_pos<2>, grid(nlocal), dim3(FB), 0, s, nlocal, nfaces, nv0, nv1,
+                           kappa, rowptr, slots, sides, face_area, face_normal, rcp_vol, lambdaf,
+                           diag, (double4 *)geo, lam);
     else
         return UCFD_ERR_ARG;
     return ok();
@@ -1242,4 +1241,17 @@ int fs_launch_lsweep_packed(hipStream_t s, bool upper, int ndims, int nt, int ba
         return lsweep_p_nt<2>(s, upper, nt, base, cnt, order, nfvars, kappa, nlocal, rowptr,
                               colidx, (const double4 *)geo, lam, diag, u, du);
     return UCFD_ERR_ARG;
+}
+
+int fs_launch_lsweep_free(hipStream_t s, bool upper, int base, int cnt, const int *order, int nv0,
+                          int nv1, int nlocal, const double *diag, double *du)
+{
+    if (cnt <= 0) return UCFD_OK;
+    if (upper)
+        hipLaunchKernelGGL(k_lsweep_free<true>, grid(cnt), dim3(FB), 0, s, base, cnt, order, nv0,
+                           nv1, nlocal, diag, du);
+    else
+        hipLaunchKernelGGL(k_lsweep_free<false>, grid(cnt), dim3(FB), 0, s, base, cnt, order, nv0,
+                           nv1, nlocal, diag, du);
+    return ok();
 }
