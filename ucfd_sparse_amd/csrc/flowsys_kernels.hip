@@ -592,7 +592,6 @@ __global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, 
     }
 }
 
-// rank_lusgs_update (lusgs.c:54-70)
 // lusgs_pack in rank order: inv[r] = the element whose cell_ids entry is r
 // (k_lpack_inv, one 4-byte scatter per element), then one thread per rank
 // gathers the element's rows and writes the rank rows coalesced.  Same
@@ -643,6 +642,7 @@ __global__ __launch_bounds__(FB) void k_fill_int(int64_t n, int *a, int v)
     for (int64_t i = (int64_t)blockIdx.x*FB + threadIdx.x; i < n; i += st) a[i] = v;
 }
 
+// rank_lusgs_update (lusgs.c:54-70)
 __global__ __launch_bounds__(FB) void k_lupdate(int neles, int nlocal, int nvars,
                                                 const int *__restrict__ cell_ids,
                                                 const double *__restrict__ du,
