@@ -39,13 +39,19 @@ __device__ __forceinline__ int wave_excl_scan(int v)
     return x - v;
 }
 
-// one-wave workgroups remapped so that each XCD owns a contiguous range of
-// waves (hardware deals workgroup ids round-robin over the 8 XCDs)
+// one-wave workgroups remapped so that each XCD owns runs of 32 consecutive
+// waves, the 8 XCDs neighbouring runs (hardware deals workgroup ids
+// round-robin over the XCDs): adjacent rows share one L2 while the XCDs
+// work on nearby rows (3-4 % faster block sweeps than one contiguous eighth
+// per XCD).  The last incomplete group of runs keeps the identity.
 __device__ __forceinline__ int wave_xcd_index()
 {
+    constexpr int RUN = 32;
     const int b = blockIdx.x, nblk = gridDim.x;
-    const int x = b % 8, k = b / 8, per = nblk / 8, rem = nblk % 8;
-    return x*per + (x < rem ? x : rem) + k;
+    const int full = nblk/(8*RUN)*(8*RUN);
+    if (b >= full) return b;
+    const int x = b % 8, k = b / 8;
+    return (k/RUN)*(8*RUN) + x*RUN + k%RUN;
 }
 
 }  // namespace ucfd
