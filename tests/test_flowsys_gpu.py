@@ -288,3 +288,47 @@ def test_lusgs_sweeps_without_prepare_and_other_kappa(colored):
             Or.sweep(upper, 1.25, turb=turb)
             assert np.array_equal(S.export(1), Or.du), (prepared, name)
         S.close()
+
+
+def quad_mesh(m, n, seed):
+    """2-D quads: the first four faces (x and y) of an m x n x 1 hex grid."""
+    h = M.hex_mesh(m, n, 1, seed=seed)
+    return M.Mesh(np.ascontiguousarray(h.nei_ele[:4]), np.ascontiguousarray(h.fnorm_vol[:4]),
+                  np.ascontiguousarray(h.vec_fnorm[:4, :2]), h.orig_id, h.key)
+
+
+@pytest.mark.parametrize("kind, nt, colored", [("quad", 1, False), ("quad", 2, True),
+                                               ("tet", 2, True), ("tet", 4, False)])
+def test_lusgs_two_dimensions_and_turbulence_rows(kind, nt, colored):
+    """The scalar method in 2-D (nfvars = 4) and with 1 to 4 turbulence rows,
+    on a random and on a colour-ordered numbering (the first schedule level
+    then an index list or a range): every stage of two iterations bit for bit
+    against the oracle (lusgs.c:8-321)."""
+    from ucfd_sparse_amd import flowsys as FS
+    from ucfd_sparse_amd.layout import colored_layout
+    seed = 61 + nt
+    mesh = quad_mesh(9, 7, seed) if kind == "quad" else M.tet_mesh(3, seed=seed)
+    nfv = mesh.ndims + 2
+    flow = M.flow_fields(mesh, seed=seed, nvars=nfv + nt)
+    if colored:
+        mesh.key = None
+        lay = colored_layout(mesh, flow, use_key=False)
+        mesh, flow = lay.mesh, lay.flow
+    fs = M.flowsys_system(mesh, flow, nv=nfv, seed=seed)
+    rng = np.random.default_rng(seed + 5)
+    fspr = rng.uniform(0.5, 2.0, fs.nfaces)
+    tfspr = rng.uniform(0.2, 1.0, fs.nfaces)
+    el = M.flowsys_elements(mesh, flow, nvars=nfv + nt, seed=seed)
+    say(f"scalar LU-SGS {kind} ({mesh.ndims}-D), nt={nt}, colour-ordered={colored}: "
+        f"{fs.nlocal} cells, {fs.nfaces} slots")
+    ref = O.flowsys_lusgs_run(fs, fspr, tfspr, el, nfv + nt, nfv, a0=0.2, turb_factor=1.7,
+                              kappa=1.1, iterations=2)
+    fsd, _, eld = on_device(fs, np.zeros(1), el)
+    S = FS.FlowSysLUSGS(fsd, nfv + nt, nfv, torch.from_numpy(fspr).cuda(),
+                        torch.from_numpy(tfspr).cuda(), ndims=mesh.ndims)
+    S.set_element(0, eld)
+    got = FS.run_lusgs(S, a0=0.2, turb_factor=1.7, kappa=1.1, iterations=2)
+    S.close()
+    assert sorted(got) == sorted(ref)
+    for k, v in ref.items():
+        assert np.array_equal(np.asarray(got[k]).reshape(v.shape), v), k

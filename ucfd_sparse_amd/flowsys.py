@@ -112,8 +112,11 @@ class FlowSys:
 
 class FlowSysLUSGS(FlowSys):
     """The scalar LU-SGS method of the FlowSys (UCFDFlowSysSetLUSGS):
-    fspr / tfspr are per face slot (kept by the library, read at every
-    Prepare and sweep)."""
+    fspr / tfspr are per face slot (kept by the library).  Each Prepare also
+    stores per-position face records that the sweeps after it read: the
+    sweeps use the geometry and spectral radii of the last Prepare; sweeps
+    with no Prepare before them read the arrays.  ndims as in
+    UCFDFlowSysCreate (face_normal holds ndims * nfaces values)."""
 
     def __init__(self, fs, nvars, nfvars, fspr, tfspr=None, ndims=3, nelem=1):
         self._lib = L.load()
