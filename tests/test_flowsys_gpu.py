@@ -251,6 +251,15 @@ def test_lusgs_state_errors():
     with pytest.raises(L.UCFDError):
         S.call("UCFDLUSGS_Pack", 3, 1.5, 0.3)      # no such element
     S.close()
+    # two elements on one cell: reported, not a silent race
+    eld = dict(el)
+    eld["cell_ids"] = el["cell_ids"].copy()
+    eld["cell_ids"][7] = eld["cell_ids"][3]
+    S = FS.FlowSysLUSGS(fs, 6, 5, fspr, tfspr)
+    S.set_element(0, eld)
+    with pytest.raises(L.UCFDError):
+        S.call("UCFDLUSGS_Pack", 0, 1.5, 0.3)
+    S.close()
     B = FS.FlowSys(fs, 6, 5)                       # block LU-SGS handle
     with pytest.raises(L.UCFDError):
         B.call("UCFDLUSGS_NSPrepare", 1.0)

@@ -593,10 +593,11 @@ __global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, 
 }
 
 // lusgs_pack in rank order: inv[r] = the element whose cell_ids entry is r
-// (k_lpack_inv, one 4-byte scatter per element), then one thread per rank
-// gathers the element's rows and writes the rank rows coalesced.  Same
-// values as the element-order scatter of k_lpack (each rank row is a copy or
-// the same expression), without its 18 scattered 8-byte stores per element.
+// (k_lpack_inv, one 4-byte exchange per element; a second element on the
+// same cell raises UCFD_KFLAG_DUP), then one thread per rank gathers the
+// element's rows and writes the rank rows coalesced.  Same values as the
+// element-order scatter of k_lpack (each rank row is a copy or the same
+// expression), without its 18 scattered 8-byte stores per element.
 __global__ __launch_bounds__(FB) void k_lpack_inv(int neles, int nlocal,
                                                   const int *__restrict__ cell_ids,
                                                   int *__restrict__ inv, int *flag)
@@ -608,7 +609,7 @@ __global__ __launch_bounds__(FB) void k_lpack_inv(int neles, int nlocal,
         atomicOr(flag, UCFD_KFLAG_INDEX);
         return;
     }
-    inv[r] = e;
+    if (atomicExch(inv + r, e) >= 0) atomicOr(flag, UCFD_KFLAG_DUP);
 }
 
 __global__ __launch_bounds__(FB) void k_lpack_rank(int neles, int nlocal, int nvars, int nfvars,

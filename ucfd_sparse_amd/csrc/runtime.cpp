@@ -146,6 +146,9 @@ int ucfd_sync_and_check(const char *fn)
         if (f & UCFD_KFLAG_INDEX)
             return ucfd_report(UCFD_ERR_INDEX, fn, "an index array holds an entry outside "
                                                    "[0, neles); the affected cells were skipped");
+        if (f & UCFD_KFLAG_DUP)
+            return ucfd_report(UCFD_ERR_INDEX, fn, "cell_ids maps two elements to the same cell; "
+                                                   "the rank rows of that cell are undefined");
         if (f & UCFD_KFLAG_LAYOUT)
             return ucfd_report(UCFD_ERR_STATE, fn, "icolor/lcolor no longer hold the layout "
                                                    "declared with these pointers (changed, or "

@@ -21,7 +21,7 @@ enum {
 };
 
 // Bits raised by kernels in the device error word.
-enum { UCFD_KFLAG_INDEX = 1, UCFD_KFLAG_PIVOT = 2, UCFD_KFLAG_LAYOUT = 4 };
+enum { UCFD_KFLAG_INDEX = 1, UCFD_KFLAG_PIVOT = 2, UCFD_KFLAG_LAYOUT = 4, UCFD_KFLAG_DUP = 8 };
 
 // Flow-variable configuration of the legacy API (compile-time in the
 // reference: NVARS / NFVARS / NDIMS, Makefile.inc:21-24).
