@@ -853,6 +853,14 @@ __global__ __launch_bounds__(FB) void k_lsweep_p(int base, int cnt, const int *_
     const int r = order ? order[base + t] : base + t;
     const int nv0 = NT > 0 ? nfvars : 0;
     const int p0 = rowptr[r], p1 = rowptr[r + 1];
+    // the cell's own rows first, so their latency overlaps the gathers
+    double odu[DNV], odg[DNV];
+#pragma unroll
+    for (int k = 0; k < DNV; ++k) {
+        const int64_t l = r + (int64_t)(k + nv0)*nlocal;
+        odu[k] = du[l];
+        odg[k] = diag[l];
+    }
     double df[DNV];
 #pragma unroll
     for (int k = 0; k < DNV; ++k) df[k] = 0.0;
@@ -899,8 +907,8 @@ __global__ __launch_bounds__(FB) void k_lsweep_p(int base, int cnt, const int *_
 #pragma unroll
     for (int k = 0; k < DNV; ++k) {
         const int64_t l = r + (int64_t)(k + nv0)*nlocal;
-        if (UPPER) du[l] -= (0.5*df[k]/diag[l]);
-        else du[l] = (du[l] - 0.5*df[k])/diag[l];
+        if (UPPER) du[l] = odu[k] - (0.5*df[k]/odg[k]);
+        else du[l] = (odu[k] - 0.5*df[k])/odg[k];
     }
 }
 
