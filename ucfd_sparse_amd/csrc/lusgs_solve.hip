@@ -269,6 +269,115 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
     }
 }
 
+// ---- the first colour's upper sweep and update (k_upper_upd on [0, cnt)),
+// followed by the defect-correction residual of the next pass for the same
+// cells, s_i = rhs_i - A(e)_i with the updated e.  The first colour is the
+// last one the upper sweep visits, so every neighbour's e is final here,
+// and all its neighbours lie in later colours: the faces the upper sweep
+// takes are exactly the faces the residual takes (nb != i), and the
+// neighbour states qb_nb it gathers are the residual's.  Same expressions
+// and order as k_lower_res's residual, so s and the block partials
+// (part[pofs + block], same blocks) are bit for bit those the next pass's
+// lower launch would compute; that launch then only forms dstar = s / D
+// (k_lower_pre).
+template <typename T, int FC>
+__global__ __launch_bounds__(kSB) void k_upper_upd_res(
+    int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
+    const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ dstar,
+    T *delta, T *e, const T *__restrict__ rhs, T *__restrict__ sres, double *part, int pofs)
+{
+    const int b = xcd_remap();
+    const int t = b*kSB + threadIdx.x;
+    double ss = 0.0;
+    if (t < cnt) {
+        const int i = t, ne = cnt;
+        int nbv[FC];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
+        const T dg = ldg_nt(diag + i);
+        T ds[NV], ei[NV], rh[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            ds[q] = ldg_nt(dstar + (size_t)q*N + i);
+            ei[q] = ldg_nt(e + (size_t)q*N + i);
+            rh[q] = ldg_nt(rhs + (size_t)q*N + i);
+        }
+        bool up_f[FC];
+        Face4<T> g[FC];
+        T U[FC][NV], d[FC][NV], en[FC][NV];
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            const int nb = nbv[f];
+            up_f[f] = nb >= ne;
+            if (!up_f[f]) continue;
+            g[f] = ldg_nt(geo + (size_t)f*N + i);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                U[f][q] = qb[(size_t)q*N + nb];
+                d[f][q] = delta[(size_t)q*N + nb];
+                en[f][q] = e[(size_t)q*N + nb];
+            }
+        }
+        T acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] = T(0);
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            if (!up_f[f]) continue;
+            T o[NV];
+            face_term<T>(U[f], d[f], g[f], o);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) acc[q] += o[q];
+        }
+        T enew[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const size_t o = (size_t)q*N + i;
+            const T dl = ds[q] - T(0.5)*acc[q]/dg;
+            enew[q] = ei[q] + dl;
+            __builtin_nontemporal_store(dl, delta + o);
+            __builtin_nontemporal_store(enew[q], e + o);
+        }
+        T res[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) res[q] = T(0);
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+            if (!up_f[f]) continue;
+            T o[NV];
+            face_term<T>(U[f], en[f], g[f], o);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) res[q] += o[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const T sq = rh[q] - (dg*enew[q] + T(0.5)*res[q]);
+            ss += (double)sq*(double)sq;
+            sres[(size_t)q*N + i] = sq;
+        }
+    }
+    block_sum<kSB>(ss, part, pofs + b);
+}
+
+// ---- the first colour's lower sweep when k_upper_upd_res formed its
+// residual s (and block partials) in the previous pass: no face is lower
+// than the first colour, so dstar = (s - 1/2*0) / D, k_lower_res's
+// expression with an empty face sum.
+template <typename T>
+__global__ __launch_bounds__(kSB) void k_lower_pre(int cnt, int N, const T *__restrict__ diag,
+                                                   const T *__restrict__ sres, T *dstar)
+{
+    const int t = xcd_remap()*kSB + threadIdx.x;
+    if (t >= cnt) return;
+    const T dg = ldg_nt(diag + t);
+    const T acc = T(0);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const T sq = ldg_nt(sres + (size_t)q*N + t);
+        __builtin_nontemporal_store((sq - T(0.5)*acc)/dg, dstar + (size_t)q*N + t);
+    }
+}
+
 // ---- FP64 residual r = b - A(dq) on base Q over all cells.  Optionally
 // writes fl32(r) and the FP32 base state fl32(Q + dq) for the inner loop.
 // Launched once per colour range [n0, n0+cnt) like the sweeps: all XCDs
@@ -520,6 +629,9 @@ struct ucfd_lusgs_solver {
            *delta64 = nullptr;
     float *d32 = nullptr, *qb32 = nullptr, *r32 = nullptr, *e32 = nullptr, *dstar32 = nullptr,
           *delta32 = nullptr;
+    // the first colour's residual for the next pass, formed by its upper launch
+    double *sres64 = nullptr;
+    float *sres32 = nullptr;
     double *part = nullptr, *norm = nullptr, *hnorm = nullptr;
     double *hnorm_dev = nullptr;   // device address of the pinned hnorm
     int nparts = 0;
@@ -548,16 +660,24 @@ int copy_in(void *dst, const void *src, size_t bytes, hipStream_t st, const char
                           "hipMemcpyAsync(in)");
 }
 
-// lower (with residual) over all colours, norm of s to *norm
+// lower (with residual) over all colours, norm of s to *norm.  pre0: the
+// previous pass's upper launch formed the first colour's residual and block
+// partials (k_upper_upd_res into sres), so that colour only divides by D.
 template <typename T>
 int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb, const T *rhs,
-               const T *e, T *dstar, const Face4<T> *geo, const T *diag)
+               const T *e, T *dstar, const Face4<T> *geo, const T *diag, bool pre0 = false,
+               const T *sres = nullptr)
 {
     int pofs = 0;
     for (int c = 0; c < s->nc; ++c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
+        if (c == 0 && pre0) {
+            hipLaunchKernelGGL(k_lower_pre<T>, g, dim3(kSB), 0, st, cnt, s->N, diag, sres, dstar);
+            pofs += (int)g.x;
+            continue;
+        }
 #define L4(RES) hipLaunchKernelGGL((k_lower_res<T, 4, RES>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
                                    s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
 #define L6(RES) hipLaunchKernelGGL((k_lower_res<T, 6, RES>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
@@ -573,14 +693,26 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
+// upper sweep and update over all colours (last colour first).  rhs given:
+// the first colour's launch also forms the next pass's residual of its
+// cells into sres, with their block partials at the start of part.
 template <typename T>
 int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar, T *delta, T *e,
-               const Face4<T> *geo, const T *diag)
+               const Face4<T> *geo, const T *diag, const T *rhs = nullptr, T *sres = nullptr)
 {
     for (int c = s->nc - 1; c >= 0; --c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
+        if (c == 0 && rhs) {
+            if (s->F == 4)
+                hipLaunchKernelGGL((k_upper_upd_res<T, 4>), g, dim3(kSB), 0, st, cnt, s->N, s->nei,
+                                   geo, diag, qb, dstar, delta, e, rhs, sres, s->part, 0);
+            else
+                hipLaunchKernelGGL((k_upper_upd_res<T, 6>), g, dim3(kSB), 0, st, cnt, s->N, s->nei,
+                                   geo, diag, qb, dstar, delta, e, rhs, sres, s->part, 0);
+            continue;
+        }
         if (s->F == 4)
             hipLaunchKernelGGL((k_upper_upd<T, 4>), g, dim3(kSB), 0, st, n0, cnt, s->N, s->nei, geo,
                                diag, qb, dstar, delta, e);
@@ -683,6 +815,7 @@ UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nf
         (rc = dalloc(s, &s->delta64, NV*N, fn)) || (rc = dalloc(s, &s->qb32, NV*N, fn)) ||
         (rc = dalloc(s, &s->r32, NV*N, fn)) || (rc = dalloc(s, &s->e32, NV*N, fn)) ||
         (rc = dalloc(s, &s->dstar32, NV*N, fn)) || (rc = dalloc(s, &s->delta32, NV*N, fn)) ||
+        (rc = dalloc(s, &s->sres64, NV*N, fn)) || (rc = dalloc(s, &s->sres32, NV*N, fn)) ||
         (rc = dalloc(s, &s->part, (size_t)s->nparts, fn)) || (rc = dalloc(s, &s->norm, 8, fn)) ||
         (rc = dalloc(s, &tfnv, F*N, fn)) || (rc = dalloc(s, &tvfn, 3*F*N, fn)) ||
         (rc = dalloc(s, &tsp, F*N, fn)) || (rc = dalloc(s, &tdt, N, fn))) {
@@ -783,9 +916,10 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     if (mode == UCFD_SOLVE_FP64) {
         // e = dq; iteration j: s = b - A(e) (in the lower launches), e += M^-1 s
         for (int it = 0; it < maxiter && !rc; ++it) {
-            rc = lower_pass<double>(s, st, it > 0, s->q64, B, DQ, s->dstar64, s->g64, s->d64);
+            rc = lower_pass<double>(s, st, it > 0, s->q64, B, DQ, s->dstar64, s->g64, s->d64,
+                                    it > 0, s->sres64);
             if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, DQ, s->g64,
-                                             s->d64);
+                                             s->d64, B, s->sres64);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
             if (it == 0) bnorm = rn;
@@ -815,9 +949,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
             double s0 = 0.0;
             for (int j = 0; info.passes < maxiter && !rc; ++j) {
                 rc = lower_pass<float>(s, st, j > 0, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                       s->d32);
+                                       s->d32, j > 0, s->sres32);
                 if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32,
-                                                s->g32, s->d32);
+                                                s->g32, s->d32, s->r32, s->sres32);
                 double sn = 0.0;
                 if (!rc) rc = read_norm(s, st, &sn, fn);
                 info.passes++;
@@ -842,9 +976,11 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
 
 // Measurement extension: HIP-event times of the solver's kernels, one full
 // pass each (every colour), averaged over reps, on the library stream:
-// ms[0] lower sweep with residual (FP64), ms[1] upper sweep with update
-// (FP64), ms[2] / ms[3] the same in FP32 (the inner kernels of the mixed
-// form), ms[4] the FP64 residual (all colours) and its norm.  The state is a
+// ms[0] lower sweep with residual (FP64; the first colour's residual comes
+// from the previous pass, as in the solve), ms[1] upper sweep with update
+// and the first colour's next residual (FP64), ms[2] / ms[3] the same in
+// FP32 (the inner kernels of the mixed form), ms[4] the FP64 residual (all
+// colours) and its norm.  The state is a
 // first outer step (dq = 0 with right-hand side b); the solver's work arrays
 // are overwritten.
 UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *b, int reps,
@@ -865,19 +1001,28 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
     hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->r64, 0.0);
     hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
     int rc = residual64(s, st, b, s->r64, nullptr, true, true);
+    // one untimed pass of each form, so the first colour's residual exists
+    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->r64, s->dstar64, s->g64, s->d64);
+    if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->r64, s->g64, s->d64,
+                                     b, s->sres64);
+    if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
+                                    s->d32);
+    if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32, s->d32,
+                                    s->r32, s->sres32);
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps && !rc; ++r) {
         (void)hipEventRecord(ev[0], st);
-        rc = lower_pass<double>(s, st, true, s->q64, b, s->r64, s->dstar64, s->g64, s->d64);
+        rc = lower_pass<double>(s, st, true, s->q64, b, s->r64, s->dstar64, s->g64, s->d64, true,
+                                s->sres64);
         (void)hipEventRecord(ev[1], st);
         if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->r64, s->g64,
-                                         s->d64);
+                                         s->d64, b, s->sres64);
         (void)hipEventRecord(ev[2], st);
         if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                        s->d32);
+                                        s->d32, true, s->sres32);
         (void)hipEventRecord(ev[3], st);
         if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32,
-                                        s->d32);
+                                        s->d32, s->r32, s->sres32);
         (void)hipEventRecord(ev[4], st);
         if (!rc) rc = residual64(s, st, b, s->r64, nullptr, false);
         (void)hipEventRecord(ev[5], st);
