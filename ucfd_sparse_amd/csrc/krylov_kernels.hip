@@ -690,6 +690,62 @@ __global__ __launch_bounds__(64) void k_bsgs_w(int base, int cnt, int cap,
                                       dv + (int64_t)(act ? i : base + t0)*B2, x32, b);
 }
 
+// A level whose rows take no block in the sweep's direction (the first
+// colour of a lower sweep, the last of an upper one): k_bsgs_w's row update
+// with an empty product sum, its expressions kept literally (a = b - 0 for
+// the FP32 lower sweep, a = 0 for the LU-SGS upper one), so the bits are
+// the same.  The row's own entries and its LU block are requested together
+// at the start, with no index loads in front of them: one memory round trip
+// per wave instead of three.  (The lower BILU sweep of such a level leaves
+// b unchanged and is not launched.)
+template <int B, bool UPPER, bool F32, int K>
+__global__ __launch_bounds__(64) void k_bsgs_diag(int base, int cnt, const double *__restrict__ dv,
+                                                  float *x32, double *b)
+{
+    constexpr int B2 = B*B, G = 64/B;
+    constexpr bool ADD = UPPER && K == SW_LUSGS;
+    static_assert(K == SW_LUSGS || (K == SW_BILU && UPPER && !F32), "sweep kind");
+    const int lane = threadIdx.x;
+    const int t0 = ucfd::wave_xcd_index()*G;
+    if (t0 >= cnt) return;
+    const int g = lane/B, r = lane%B;
+    const int i = base + t0 + g;
+    const bool act = g < G && t0 + g < cnt;
+    const int64_t o = (int64_t)i*B + r;
+    double lu[B2];
+    double bo = 0.0;
+    float xo = 0.f;
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < B2; ++q) lu[q] = dv[(int64_t)i*B2 + q];
+        if (!ADD) bo = b[o];
+        if (F32 && UPPER) xo = x32[o];
+    }
+    double a = 0.0;
+    if (!F32 && !ADD && act) a = bo;
+    if (F32) {
+        if (UPPER) a = (double)0.0f;
+        else if (act) a = bo - (double)0.0f;
+    }
+    double arr[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) arr[k] = __shfl(a, g*B + k, 64);
+    if (!act) return;
+    ucfd::lusub<B>(lu, arr);
+    if (F32) {
+        if (UPPER) {
+            const double v = (double)xo - arr[r];
+            x32[o] = (float)v;
+            b[o] = v;
+        } else {
+            x32[o] = (float)arr[r];
+        }
+    } else {
+        if (ADD) b[o] = b[o] - arr[r];
+        else b[o] = arr[r];
+    }
+}
+
 // --------------------------------------------------------------- BLAS
 __device__ __forceinline__ double block_reduce(double v)
 {
@@ -1055,6 +1111,17 @@ int sweep64_launch(hipStream_t s, int upper, int base, int cnt, const int *ord, 
     if constexpr (B > 1) {
         if (!ord) {
             const dim3 g((cnt + 64/B - 1)/(64/B));
+            if (maxw == 0 && K != SW_ILU) {
+                // no row of the level takes a block in this direction
+                if (K == SW_BILU && !upper) return UCFD_OK;
+                if (upper)
+                    hipLaunchKernelGGL((k_bsgs_diag<B, true, false, K>), g, dim3(64), 0, s, base,
+                                       cnt, dv, nullptr, b);
+                else if constexpr (K == SW_LUSGS)
+                    hipLaunchKernelGGL((k_bsgs_diag<B, false, false, K>), g, dim3(64), 0, s, base,
+                                       cnt, dv, nullptr, b);
+                return ok();
+            }
             const int cap = stage_cap<B, double>(maxw);
             const size_t lds = stage_bytes<B, double>(cap);
             if (upper)
@@ -1102,6 +1169,16 @@ template <int B> struct Sw32L {
     {
         if (B > 1 && !ord) {
             const dim3 g((cnt + 64/B - 1)/(64/B));
+            if (maxw == 0) {
+                // no row of the level takes a block in this direction
+                if (upper)
+                    hipLaunchKernelGGL((k_bsgs_diag<B, true, true, SW_LUSGS>), g, dim3(64), 0, s,
+                                       base, cnt, dv, x32, b);
+                else
+                    hipLaunchKernelGGL((k_bsgs_diag<B, false, true, SW_LUSGS>), g, dim3(64), 0, s,
+                                       base, cnt, dv, x32, b);
+                return ok();
+            }
             const int cap = stage_cap<B, float>(maxw);
             const size_t lds = stage_bytes<B, float>(cap);
             if (upper)
