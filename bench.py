@@ -614,11 +614,17 @@ def blusgs_workload(args, log):
     evl = [[ev() for _ in range(3)] for _ in range(args.steps)]
     for r in range(args.steps):
         lit(evl[r])
+    # Update (uptsb += du per element), timed once after the sweeps
+    eu = [ev() for _ in range(2)]
+    eu[0].record(stream)
+    SL.call("UCFDLUSGS_Update", 0)
+    eu[1].record(stream)
     L.call("ucfd_synchronize")
     torch.cuda.synchronize()
     L.call("ucfd_set_stream", None)
     ns_l = sum(x[0].elapsed_time(x[1]) for x in evl) / args.steps
     rans_l = sum(x[1].elapsed_time(x[2]) for x in evl) / args.steps
+    scalar_prep["update_ms"] = eu[0].elapsed_time(eu[1])
     SL.close()
     # algorithmic bytes of one NS scalar sweep, every array element counted
     # once: per cell rowptr, rcp_vol and the 5 rows of diag, u, du (read) and
@@ -637,7 +643,8 @@ def blusgs_workload(args, log):
     log(f"flowsys scalar LU-SGS: NS lower+upper {ns_l:.3f} ms "
         f"({scalar['ns_sweep_gbs']:.0f} GB/s alg), RANS lower+upper {rans_l:.3f} ms, pack "
         f"{scalar['pack_ms']:.3f} ms ({scalar['pack_gbs']:.0f} GB/s alg), prepare NS "
-        f"{scalar['ns_prepare_ms']:.3f} RANS {scalar['rans_prepare_ms']:.3f} ms")
+        f"{scalar['ns_prepare_ms']:.3f} RANS {scalar['rans_prepare_ms']:.3f} ms, update "
+        f"{scalar['update_ms']:.3f} ms")
     del fsd
     torch.cuda.empty_cache()
 
@@ -1052,6 +1059,7 @@ def blusgs_summary(r):
                              "ns_sweep_frac": sc["ns_sweep_gbs"] / HBM_PEAK_GBS,
                              "rans_lower_upper_ms": sc["rans_lower_upper_ms"],
                              "pack_ms": sc["pack_ms"], "pack_gbs": sc["pack_gbs"],
+                             "update_ms": sc["update_ms"],
                              "ns_prepare_ms": sc["ns_prepare_ms"]},
             "krylov_precon_apply": {k: {"ms": v["ms"], "gbs": v["gbs"],
                                         "frac": v["gbs"] / HBM_PEAK_GBS}

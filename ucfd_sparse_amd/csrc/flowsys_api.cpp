@@ -71,6 +71,7 @@ struct _FlowSys {
     // (geometry, spectral radius), read by the sweeps that follow
     double *lu_geo = nullptr, *lu_lam = nullptr, *lu_tlam = nullptr;
     int *lu_inv = nullptr;          // Pack's inverse of cell_ids [nlocal]
+    double *lu_rec = nullptr;       // Pack / Update records [nlocal][fs_rec_words(nvars)]
     bool lu_geo_ok = false, lu_lam_ok = false, lu_tlam_ok = false;
     int64_t npos = 0;
     bool fp32 = false;
@@ -131,6 +132,7 @@ void free_device(FlowSys s)
     dfree(&s->lu_diag);
     dfree(&s->lu_geo);
     dfree(&s->lu_inv);
+    dfree(&s->lu_rec);
     dfree(&s->lu_lam);
     dfree(&s->lu_tlam);
     s->lu_geo_ok = s->lu_lam_ok = s->lu_tlam_ok = false;
@@ -661,14 +663,16 @@ UCFD_API ucfd_status_t UCFDLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal turb_f
     const int ir = sg.add(e->rhs, sizeof(double)*ne*nv, UCFD_IN);
     const int it = sg.add(e->dt, sizeof(double)*ne, UCFD_IN);
     const int isr = sg.add(e->dsrc, sizeof(double)*ne*nv, UCFD_IN);
-    if (dalloc(&sys->lu_inv, (size_t)sys->nlocal, fn)) return UCFD_FAILED;
+    if (dalloc(&sys->lu_inv, (size_t)sys->nlocal, fn) ||
+        dalloc(&sys->lu_rec, (size_t)sys->nlocal*fs_rec_words(sys->nvars), fn))
+        return UCFD_FAILED;
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     rc = fs_launch_lpack(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, sys->nfvars, turb_factor,
                          a0, (const int *)sg.dev(ic), (const double *)sg.dev(iu),
                          (const double *)sg.dev(ir), (const double *)sg.dev(it),
                          (const double *)sg.dev(isr), sys->lu_u, sys->lu_du, sys->lu_diag,
-                         ucfd_device_flag(), sys->lu_inv);
+                         ucfd_device_flag(), sys->lu_inv, sys->lu_rec);
     if (!rc) rc = ucfd_sync_and_check(fn);
     return status(sg.leave(rc));
 }
@@ -683,11 +687,13 @@ UCFD_API ucfd_status_t UCFDLUSGS_Update(FlowSys sys, UCFDInt eidx)
     UcfdStage sg(fn);
     const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
     const int iu = sg.add(e->uptsb, sizeof(double)*ne*nv, UCFD_INOUT);
+    if (dalloc(&sys->lu_rec, (size_t)sys->nlocal*fs_rec_words(sys->nvars), fn))
+        return UCFD_FAILED;
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     rc = fs_launch_lupdate(ucfd_stream(), e->neles, sys->nlocal, sys->nvars,
                            (const int *)sg.dev(ic), sys->lu_du, (double *)sg.dev(iu),
-                           ucfd_device_flag());
+                           ucfd_device_flag(), sys->lu_rec);
     if (!rc) rc = ucfd_sync_and_check(fn);
     return status(sg.leave(rc));
 }

@@ -55,9 +55,14 @@ UCFD_HIDDEN int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars,
                                 double turb_factor, double a0, const int *cell_ids,
                                 const double *upts, const double *rhs, const double *dt,
                                 const double *dsrc, double *u, double *du, double *diag,
-                                int *flag, int *inv);
+                                int *flag, int *inv, double *rec);
+// rec (optional, nlocal * fs_rec_words(nvars) doubles): record forms of Pack
+// and Update for nvars 4..9 (one contiguous record per cell instead of a
+// gather or scatter per row); other nvars, or rec NULL, use the row forms.
 UCFD_HIDDEN int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars,
-                                  const int *cell_ids, const double *du, double *upts, int *flag);
+                                  const int *cell_ids, const double *du, double *upts, int *flag,
+                                  double *rec);
+inline size_t fs_rec_words(int nvars) { return (size_t)((3*nvars + 1) & ~1); }
 // Scalar Prepare (lusgs.c:87-113): diag rows nv0..nv1-1 += the cell's face
 // sum, and the per-position face records the sweeps read: geo[p] =
 // {side*normal (ndims), face_area*rcp_vol} as 4 doubles (geo may be NULL:

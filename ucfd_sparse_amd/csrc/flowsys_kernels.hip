@@ -643,6 +643,125 @@ __global__ __launch_bounds__(FB) void k_fill_int(int64_t n, int *a, int v)
     for (int64_t i = (int64_t)blockIdx.x*FB + threadIdx.x; i < n; i += st) a[i] = v;
 }
 
+// Record forms of lusgs_pack / rank_lusgs_update for a random element ->
+// cell map.  Pack: (1) element order, every input read coalesced, the
+// element's 3*NV output values written as one contiguous record at
+// rec[r*W] (with the inverse-map exchange of k_lpack_inv); (2) rank order,
+// each record read whole and written to the [nvars, nlocal] rows
+// coalesced.  Update: (1) rank order, du's NV rows to records; (2) element
+// order, one record per element gathered (one or two cache lines instead of
+// NV) and added to uptsb.  Same values and expressions as k_lpack_rank /
+// k_lupdate.  W = 3*NV (pack) or NV (update), rounded up to even, so the
+// records are read and written as 16-byte pairs.
+template <int NV> constexpr int rec_w(int per) { return (per*NV + 1) & ~1; }
+
+typedef double fs_d2 __attribute__((ext_vector_type(2)));
+
+template <int NV>
+__global__ __launch_bounds__(FB) void k_lpack_rec(int neles, int nlocal, int nfvars,
+                                                  double turb_factor, double a0,
+                                                  const int *__restrict__ cell_ids,
+                                                  int *__restrict__ inv,
+                                                  const double *__restrict__ upts,
+                                                  const double *__restrict__ rhs,
+                                                  const double *__restrict__ dt,
+                                                  const double *__restrict__ dsrc,
+                                                  double *__restrict__ rec, int *flag)
+{
+    constexpr int W = rec_w<NV>(3);
+    const int e = fxcd()*FB + threadIdx.x;
+    if (e >= neles) return;
+    const int r = cell_ids[e];
+    if ((unsigned)r >= (unsigned)nlocal) {
+        atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    if (atomicExch(inv + r, e) >= 0) atomicOr(flag, UCFD_KFLAG_DUP);
+    const double dte = dt[e];
+    double v[W];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int64_t ei = e + (int64_t)k*neles;
+        const double factor = k < nfvars ? 1.0 : turb_factor;
+        v[k] = upts[ei];
+        v[NV + k] = rhs[ei];
+        v[2*NV + k] = 1.0/(dte*factor) + a0 + dsrc[ei];
+    }
+    if (W > 3*NV) v[W - 1] = 0.0;
+    fs_d2 *o = (fs_d2 *)(rec + (int64_t)r*W);
+#pragma unroll
+    for (int q = 0; q < W/2; ++q) o[q] = fs_d2{v[2*q], v[2*q + 1]};
+}
+
+template <int NV>
+__global__ __launch_bounds__(FB) void k_lpack_from_rec(int nlocal, const int *__restrict__ inv,
+                                                       const double *__restrict__ rec,
+                                                       double *__restrict__ u,
+                                                       double *__restrict__ du,
+                                                       double *__restrict__ diag)
+{
+    constexpr int W = rec_w<NV>(3);
+    const int r = fxcd()*FB + threadIdx.x;
+    if (r >= nlocal || inv[r] < 0) return;      // a cell of another element type
+    const fs_d2 *in = (const fs_d2 *)(rec + (int64_t)r*W);
+    double v[W];
+#pragma unroll
+    for (int q = 0; q < W/2; ++q) {
+        const fs_d2 t = in[q];
+        v[2*q] = t.x;
+        v[2*q + 1] = t.y;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int64_t ri = r + (int64_t)k*nlocal;
+        u[ri] = v[k];
+        du[ri] = v[NV + k];
+        diag[ri] = v[2*NV + k];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(FB) void k_lupdate_rec(int nlocal, const double *__restrict__ du,
+                                                    double *__restrict__ rec)
+{
+    constexpr int W = rec_w<NV>(1);
+    const int r = fxcd()*FB + threadIdx.x;
+    if (r >= nlocal) return;
+    double v[W];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = du[r + (int64_t)k*nlocal];
+    if (W > NV) v[W - 1] = 0.0;
+    fs_d2 *o = (fs_d2 *)(rec + (int64_t)r*W);
+#pragma unroll
+    for (int q = 0; q < W/2; ++q) o[q] = fs_d2{v[2*q], v[2*q + 1]};
+}
+
+template <int NV>
+__global__ __launch_bounds__(FB) void k_lupdate_from_rec(int neles, int nlocal,
+                                                         const int *__restrict__ cell_ids,
+                                                         const double *__restrict__ rec,
+                                                         double *__restrict__ upts, int *flag)
+{
+    constexpr int W = rec_w<NV>(1);
+    const int e = fxcd()*FB + threadIdx.x;
+    if (e >= neles) return;
+    const int r = cell_ids[e];
+    if ((unsigned)r >= (unsigned)nlocal) {
+        atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    const fs_d2 *in = (const fs_d2 *)(rec + (int64_t)r*W);
+    double v[W];
+#pragma unroll
+    for (int q = 0; q < W/2; ++q) {
+        const fs_d2 t = in[q];
+        v[2*q] = t.x;
+        v[2*q + 1] = t.y;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) upts[e + (int64_t)k*neles] += v[k];
+}
+
 // rank_lusgs_update (lusgs.c:54-70)
 __global__ __launch_bounds__(FB) void k_lupdate(int neles, int nlocal, int nvars,
                                                 const int *__restrict__ cell_ids,
@@ -1101,7 +1220,7 @@ int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c)
 int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars, double turb_factor,
                     double a0, const int *cell_ids, const double *upts, const double *rhs,
                     const double *dt, const double *dsrc, double *u, double *du, double *diag,
-                    int *flag, int *inv)
+                    int *flag, int *inv, double *rec)
 {
     if (neles <= 0) return UCFD_OK;
     if (!inv) {
@@ -1112,6 +1231,20 @@ int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
     const int64_t nb = ((int64_t)nlocal + FB - 1)/FB;
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(FB), 0, s,
                        (int64_t)nlocal, inv, -1);
+#define FS_PACK_REC(NVC) \
+    case NVC: \
+        hipLaunchKernelGGL(k_lpack_rec<NVC>, grid(neles), dim3(FB), 0, s, neles, nlocal, nfvars, \
+                           turb_factor, a0, cell_ids, inv, upts, rhs, dt, dsrc, rec, flag); \
+        hipLaunchKernelGGL(k_lpack_from_rec<NVC>, grid(nlocal), dim3(FB), 0, s, nlocal, inv, rec, \
+                           u, du, diag); \
+        return ok();
+    if (rec) {
+        switch (nvars) {
+        FS_PACK_REC(4) FS_PACK_REC(5) FS_PACK_REC(6) FS_PACK_REC(7) FS_PACK_REC(8) FS_PACK_REC(9)
+        default: break;
+        }
+    }
+#undef FS_PACK_REC
     hipLaunchKernelGGL(k_lpack_inv, grid(neles), dim3(FB), 0, s, neles, nlocal, cell_ids, inv,
                        flag);
     hipLaunchKernelGGL(k_lpack_rank, grid(nlocal), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
@@ -1120,9 +1253,22 @@ int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
 }
 
 int fs_launch_lupdate(hipStream_t s, int neles, int nlocal, int nvars, const int *cell_ids,
-                      const double *du, double *upts, int *flag)
+                      const double *du, double *upts, int *flag, double *rec)
 {
     if (neles <= 0) return UCFD_OK;
+#define FS_UPD_REC(NVC) \
+    case NVC: \
+        hipLaunchKernelGGL(k_lupdate_rec<NVC>, grid(nlocal), dim3(FB), 0, s, nlocal, du, rec); \
+        hipLaunchKernelGGL(k_lupdate_from_rec<NVC>, grid(neles), dim3(FB), 0, s, neles, nlocal, \
+                           cell_ids, rec, upts, flag); \
+        return ok();
+    if (rec) {
+        switch (nvars) {
+        FS_UPD_REC(4) FS_UPD_REC(5) FS_UPD_REC(6) FS_UPD_REC(7) FS_UPD_REC(8) FS_UPD_REC(9)
+        default: break;
+        }
+    }
+#undef FS_UPD_REC
     hipLaunchKernelGGL(k_lupdate, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, cell_ids, du,
                        upts, flag);
     return ok();
