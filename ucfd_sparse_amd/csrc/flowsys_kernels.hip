@@ -937,20 +937,55 @@ __global__ __launch_bounds__(FB) void k_lprep_pos(int nlocal, int64_t nfaces, in
     const int r = fxcd()*FB + threadIdx.x;
     if (r >= nlocal) return;
     const double rv = rcp_vol[r];
+    const int p0 = rowptr[r], p1 = rowptr[r + 1];
+    // the slot indices of the first LKP positions together, then all their
+    // gathers, then the face terms in position order
+    int64_t sl[LKP];
+    double lf[LKP], fa[LKP], fn[LKP][ND], side[LKP];
+#pragma unroll
+    for (int k = 0; k < LKP; ++k) {
+        sl[k] = 0;
+        lf[k] = fa[k] = side[k] = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) fn[k][d] = 0.0;
+        if (p0 + k < p1) sl[k] = slots[p0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < LKP; ++k)
+        if (p0 + k < p1) {
+            lf[k] = lambdaf[sl[k]];
+            fa[k] = face_area[sl[k]];
+            if (geo) {
+                side[k] = (double)sides[p0 + k];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) fn[k][d] = face_normal[sl[k] + (int64_t)d*nfaces];
+            }
+        }
     double sd = 0.0;
-    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
-        const int64_t sl = slots[p];
-        const double lf = lambdaf[sl], fa = face_area[sl];
-        sd += 0.5*(lf*kappa)*fa*rv;
+    auto face = [&](int p, double lfk, double fak, const double *fnk, double sidek) {
+        sd += 0.5*(lfk*kappa)*fak*rv;
         if (geo) {
-            const double side = (double)sides[p];
             double g[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int d = 0; d < ND; ++d) g[d] = side*face_normal[sl + (int64_t)d*nfaces];
-            g[ND] = fa*rv;
+            for (int d = 0; d < ND; ++d) g[d] = sidek*fnk[d];
+            g[ND] = fak*rv;
             geo[p] = double4{g[0], g[1], g[2], g[3]};
         }
-        lam[p] = lf;
+        lam[p] = lfk;
+    };
+#pragma unroll
+    for (int k = 0; k < LKP; ++k)
+        if (p0 + k < p1) face(p0 + k, lf[k], fa[k], fn[k], side[k]);
+    for (int p = p0 + LKP; p < p1; ++p) {
+        const int64_t s2 = slots[p];
+        double f2[ND] = {};
+        double sd2 = 0.0;
+        if (geo) {
+            sd2 = (double)sides[p];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) f2[d] = face_normal[s2 + (int64_t)d*nfaces];
+        }
+        face(p, lambdaf[s2], face_area[s2], f2, sd2);
     }
     for (int k = nv0; k < nv1; ++k) diag[r + (int64_t)k*nlocal] += sd;
 }
