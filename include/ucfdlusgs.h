@@ -52,7 +52,9 @@ UCFD_EXTERN ucfd_status_t UCFDFlowSysDestroy(FlowSys *sys);
  * Prepare again (as the diagonal needs anyway).  Sweeps with no Prepare
  * before them read the caller's arrays.  Pack (lusgs.c:8-52): per element, u = uptsb,
  * du = rhs, diag = 1/(dt*factor) + a0 + dsrc with factor = turb_factor on the
- * turbulence rows.  Prepare (lusgs.c:87-140): diag += sum over the cell's
+ * turbulence rows (cell_ids mapping two elements to one cell returns
+ * UCFD_FAILED; Pack and Update keep a work buffer of
+ * nlocal * (3*nvars rounded up to even) doubles).  Prepare (lusgs.c:87-140): diag += sum over the cell's
  * positions of 0.5*kappa*fspr*face_area*rcp_vol (NS rows, or turbulence rows
  * with tfspr).  Sweeps (lusgs.c:157-321): in-place Gauss-Seidel over the lower
  * (upper) neighbours, NS rows with the Euler flux difference, turbulence rows
