@@ -48,9 +48,8 @@ UCFD_HIDDEN int fs_launch_sweep(hipStream_t s, int nv, bool f32, int base, int c
 UCFD_HIDDEN int fs_launch_zero(hipStream_t s, int64_t n, double *a, double *b, float *c);
 
 // Scalar LU-SGS (src/lusgs/lusgs.c); rank arrays u / du / diag are
-// [nvars, nlocal].  lusgs_pack / rank_lusgs_update / pre_lusgs.  lpack with
-// inv (an nlocal int workspace) writes in rank order through the inverse of
-// cell_ids; inv NULL scatters in element order.
+// [nvars, nlocal].  lusgs_pack / rank_lusgs_update / pre_lusgs.  lpack needs
+// inv (an nlocal int workspace: the inverse of cell_ids, built here).
 UCFD_HIDDEN int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
                                 double turb_factor, double a0, const int *cell_ids,
                                 const double *upts, const double *rhs, const double *dt,

@@ -564,40 +564,12 @@ __global__ __launch_bounds__(64) void k_sweep_w(int base, int cnt, int cap, int 
 // ------------------------------------------------------ scalar LU-SGS
 // Rank arrays u / du / diag are [nvars, nlocal] (src/lusgs/flowsys.h:20-24).
 
-// lusgs_pack (lusgs.c:8-37)
-__global__ __launch_bounds__(FB) void k_lpack(int neles, int nlocal, int nvars, int nfvars,
-                                              double turb_factor, double a0,
-                                              const int *__restrict__ cell_ids,
-                                              const double *__restrict__ upts,
-                                              const double *__restrict__ rhs,
-                                              const double *__restrict__ dt,
-                                              const double *__restrict__ dsrc,
-                                              double *__restrict__ u, double *__restrict__ du,
-                                              double *__restrict__ diag, int *flag)
-{
-    const int e = fxcd()*FB + threadIdx.x;
-    if (e >= neles) return;
-    const int r = cell_ids[e];
-    if ((unsigned)r >= (unsigned)nlocal) {
-        atomicOr(flag, UCFD_KFLAG_INDEX);
-        return;
-    }
-    const double dte = dt[e];
-    for (int k = 0; k < nvars; ++k) {
-        const int64_t ri = r + (int64_t)k*nlocal, ei = e + (int64_t)k*neles;
-        const double factor = k < nfvars ? 1.0 : turb_factor;
-        u[ri] = upts[ei];
-        du[ri] = rhs[ei];
-        diag[ri] = 1.0/(dte*factor) + a0 + dsrc[ei];
-    }
-}
-
-// lusgs_pack in rank order: inv[r] = the element whose cell_ids entry is r
-// (k_lpack_inv, one 4-byte exchange per element; a second element on the
-// same cell raises UCFD_KFLAG_DUP), then one thread per rank gathers the
-// element's rows and writes the rank rows coalesced.  Same values as the
-// element-order scatter of k_lpack (each rank row is a copy or the same
-// expression), without its 18 scattered 8-byte stores per element.
+// lusgs_pack (lusgs.c:8-37) in rank order, for nvars without a record form
+// below: inv[r] = the element whose cell_ids entry is r (k_lpack_inv, one
+// 4-byte exchange per element; a second element on the same cell raises
+// UCFD_KFLAG_DUP), then one thread per rank gathers the element's rows and
+// writes the rank rows coalesced: u = uptsb, du = rhs, diag =
+// 1/(dt*factor) + a0 + dsrc (factor = turb_factor on turbulence rows).
 __global__ __launch_bounds__(FB) void k_lpack_inv(int neles, int nlocal,
                                                   const int *__restrict__ cell_ids,
                                                   int *__restrict__ inv, int *flag)
@@ -1258,11 +1230,7 @@ int fs_launch_lpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
                     int *flag, int *inv, double *rec)
 {
     if (neles <= 0) return UCFD_OK;
-    if (!inv) {
-        hipLaunchKernelGGL(k_lpack, grid(neles), dim3(FB), 0, s, neles, nlocal, nvars, nfvars,
-                           turb_factor, a0, cell_ids, upts, rhs, dt, dsrc, u, du, diag, flag);
-        return ok();
-    }
+    if (!inv) return UCFD_ERR_ARG;
     const int64_t nb = ((int64_t)nlocal + FB - 1)/FB;
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(FB), 0, s,
                        (int64_t)nlocal, inv, -1);
