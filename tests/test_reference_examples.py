@@ -5,7 +5,12 @@ include lusgs.h and call the serial one).  The sources are read where they
 lie under the reference tree at test time and built in a temporary folder
 with the flags of examples/Makefile:6-21 (NVARS / NFVARS / NDIMS); nothing is
 copied into the repository.  MPI is not installed here, so a single-rank
-mpi.h written by this test stands in for it.  The programs are then run on
+mpi.h written by this test stands in for it.  The examples' input reader
+(examples/utils/readinput.c:51-53) copies each number into a buffer with
+strncpy and no terminator before atoi, so what it reads depends on stack
+contents (it intermittently read "1" as 13 here); the test links a reader of
+its own with the same interface (readinput.h: every integer of the file, in
+order) instead.  The programs are then run on
 one rank: without a GPU every library call returns its "no device" status,
 which the examples ignore (they never test return values), so they must
 still run to the end.  Skipped where the reference tree is absent."""
@@ -42,15 +47,34 @@ pytestmark = pytest.mark.skipif(not os.path.isdir(EX),
                                 reason="reference examples absent (only in the build container)")
 
 
+READER = r"""
+#include <ctype.h>
+#include <stdio.h>
+/* every integer of the file, in order, into vars */
+void input_params(FILE *fp, int *vars)
+{
+    int c, n = 0, have = 0, v = 0;
+    while ((c = fgetc(fp)) != EOF) {
+        if (isdigit(c)) { v = 10*v + (c - '0'); have = 1; }
+        else if (have) { vars[n++] = v; v = 0; have = 0; }
+    }
+    if (have) vars[n++] = v;
+}
+"""
+
+
 def build(name, tmp):
     with open(os.path.join(tmp, "mpi.h"), "w") as f:
         f.write(MPI_STUB)
+    reader = os.path.join(tmp, "reader.c")
+    with open(reader, "w") as f:
+        f.write(READER)
     utils = os.path.join(EX, "utils")
     exe = os.path.join(tmp, name + ".x")
     cmd = ["gcc", "-std=gnu11", "-O2", "-fopenmp", "-DNVARS=5", "-DNFVARS=5", "-DNDIMS=3",
            "-I", tmp, "-I", os.path.join(REPO, "include"), "-I", utils,
            os.path.join(EX, name + ".c"), os.path.join(utils, "arrays.c"),
-           os.path.join(utils, "pbutils.c"), os.path.join(utils, "readinput.c"),
+           os.path.join(utils, "pbutils.c"), reader,
            "-o", exe, "-L", LIBDIR, "-llusgs", f"-Wl,-rpath,{LIBDIR}",
            "-Wl,-rpath-link,/opt/rocm/lib", "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
