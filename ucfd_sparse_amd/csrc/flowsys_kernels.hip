@@ -477,12 +477,18 @@ __device__ __forceinline__ void fs_rows(int row, int g, bool act, int r, int st,
             val += blk[c]*(F32 ? (T)du32[var0 + c + nb*nvars] : (T)du[var0 + c + nb*nvars]);
         x -= (double)(val*fvs[q]);
     }
-    double xv[NV], lu[D2];
+    double xv[NV], lu[D2], myrow[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) xv[c] = __shfl(x, g*NV + c, 64);
-    if (!act) return;
+    // each lane reads its own row of the LU block; the group's lanes then
+    // exchange the rows, so every lane holds the whole block
 #pragma unroll
-    for (int q = 0; q < D2; ++q) lu[q] = lu_src[q];
+    for (int c = 0; c < NV; ++c) myrow[c] = act ? lu_src[row*NV + c] : 0.0;
+#pragma unroll
+    for (int rr = 0; rr < NV; ++rr)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) lu[rr*NV + c] = __shfl(myrow[c], g*NV + rr, 64);
+    if (!act) return;
     ucfd::lusub<NV>(lu, xv);
     const int64_t o = var0 + row + (int64_t)r*nvars;
     du[o] = xv[row];
