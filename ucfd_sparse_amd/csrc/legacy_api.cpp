@@ -321,8 +321,8 @@ int do_update(const char *fn, int neles, double *upts, const double *x)
     const int a_x = sg.add(x, n*8, UCFD_IN);
     int rc = sg.enter();
     if (rc) return rc;
-    rc = ucfd_launch_axpy_rows(ucfd_stream(), (int64_t)n, (double *)sg.dev(a_u),
-                               (const double *)sg.dev(a_x));
+    rc = ucfd_launch_update(ucfd_stream(), neles, ucfd_config().nvars, (double *)sg.dev(a_u),
+                            (const double *)sg.dev(a_x));
     if (rc) ucfd_report(rc, fn, "kernel launch failed");
     return sg.leave(rc);
 }

@@ -483,12 +483,33 @@ int launch_rans_m(hipStream_t s, int ndims, int nt, int n0, int cnt, int N, int 
     return UCFD_ERR_ARG;
 }
 
-__global__ __launch_bounds__(kBlock) void k_axpy(int64_t n, double *__restrict__ y,
-                                                 const double *__restrict__ x)
+// Update y += x over nrows rows of n cells ([nrows][n]).  One thread per
+// cell, cells in descending order, every row of a cell issued together: the
+// last lines written are the first cells' rows.  On a colour-contiguous
+// numbering the next pass's lower sweep of the later colours gathers the
+// first colour's solution rows, which are then the newest lines in the
+// device's last-level cache (measured at 5.3M tets, 2 colours: that launch
+// 219 -> 211 us, the pass 0.700 -> 0.685 ms against a flat ascending y += x).
+__global__ __launch_bounds__(kBlock) void k_update_desc(int n, int nrows, double *__restrict__ y,
+                                                        const double *__restrict__ x)
 {
-    const int64_t stride = (int64_t)gridDim.x*kBlock;
-    for (int64_t j = (int64_t)blockIdx.x*kBlock + threadIdx.x; j < n; j += stride)
-        y[j] += x[j];
+    const int64_t t = (int64_t)blockIdx.x*kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t j = n - 1 - t;
+    for (int r0 = 0; r0 < nrows; r0 += 8) {
+        double xv[8], yv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (r0 + q >= nrows) break;
+            xv[q] = x[(int64_t)(r0 + q)*n + j];
+            yv[q] = y[(int64_t)(r0 + q)*n + j];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (r0 + q >= nrows) break;
+            y[(int64_t)(r0 + q)*n + j] = yv[q] + xv[q];
+        }
+    }
 }
 
 template <int DIR, int MODE, int ND>
@@ -643,12 +664,11 @@ int ucfd_launch_hash(hipStream_t s, const int *a, int64_t n, unsigned long long 
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
-int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x)
+int ucfd_launch_update(hipStream_t s, int neles, int nrows, double *y, const double *x)
 {
-    if (n <= 0) return UCFD_OK;
-    int64_t blocks = (n + kBlock - 1)/kBlock;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_axpy, dim3((unsigned)blocks), dim3(kBlock), 0, s, n, y, x);
+    if (neles <= 0 || nrows <= 0) return UCFD_OK;
+    const unsigned blocks = (unsigned)(((int64_t)neles + kBlock - 1)/kBlock);
+    hipLaunchKernelGGL(k_update_desc, dim3(blocks), dim3(kBlock), 0, s, neles, nrows, y, x);
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 

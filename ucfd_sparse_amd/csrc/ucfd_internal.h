@@ -97,7 +97,8 @@ UCFD_HIDDEN int ucfd_launch_rans_sweep(hipStream_t s, int dir, int mode, int n0,
                                        const double *tdiag, const double *tfspr, int *flag,
                                        int lc_expect);
 
-UCFD_HIDDEN int ucfd_launch_axpy_rows(hipStream_t s, int64_t n, double *y, const double *x);
+// y += x over nrows rows of neles cells ([nrows][neles])
+UCFD_HIDDEN int ucfd_launch_update(hipStream_t s, int neles, int nrows, double *y, const double *x);
 // *out = sum over k < n of a mix of (k, a[k]) mod 2^64 (device array a; out
 // device memory): a content hash, order independent and deterministic.
 UCFD_HIDDEN int ucfd_launch_hash(hipStream_t s, const int *a, int64_t n, unsigned long long *out);
