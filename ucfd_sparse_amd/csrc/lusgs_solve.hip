@@ -64,6 +64,72 @@ __device__ __forceinline__ Face4<double> ldg_nt(const Face4<double> *p)
     return Face4<double>{a.x, a.y, b.x, b.y};
 }
 
+// Solver-owned 5-vectors (the base state, the increment and the sweep
+// intermediates) use a split layout, one buffer of 5N values: components 0-3
+// of cell i at p[4i .. 4i+3], component 4 at p[4N + i].  A gathered
+// neighbour state is then one 16-byte load (two in FP64) and one scalar load
+// instead of five scalar loads from five rows; the bytes are the same.
+// Arrays the caller passes (right-hand side, dq of the public residual) and
+// cell-own arrays keep the [var][cell] layout.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <bool NT> __device__ __forceinline__ void ld_split(const float *p, int N, int i, float *v)
+{
+    const f4v *a = (const f4v *)p + i;
+    const f4v x = NT ? __builtin_nontemporal_load(a) : *a;
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    v[4] = NT ? __builtin_nontemporal_load(p + 4*(size_t)N + i) : p[4*(size_t)N + i];
+}
+
+template <bool NT> __device__ __forceinline__ void ld_split(const double *p, int N, int i, double *v)
+{
+    const d2v *a = (const d2v *)p + 2*(size_t)i;
+    const d2v x = NT ? __builtin_nontemporal_load(a) : a[0];
+    const d2v y = NT ? __builtin_nontemporal_load(a + 1) : a[1];
+    v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+    v[4] = NT ? __builtin_nontemporal_load(p + 4*(size_t)N + i) : p[4*(size_t)N + i];
+}
+
+template <bool NT> __device__ __forceinline__ void st_split(float *p, int N, int i, const float *v)
+{
+    const f4v x = {v[0], v[1], v[2], v[3]};
+    if (NT) {
+        __builtin_nontemporal_store(x, (f4v *)p + i);
+        __builtin_nontemporal_store(v[4], p + 4*(size_t)N + i);
+    } else {
+        ((f4v *)p)[i] = x;
+        p[4*(size_t)N + i] = v[4];
+    }
+}
+
+template <bool NT> __device__ __forceinline__ void st_split(double *p, int N, int i, const double *v)
+{
+    const d2v x = {v[0], v[1]}, y = {v[2], v[3]};
+    d2v *a = (d2v *)p + 2*(size_t)i;
+    if (NT) {
+        __builtin_nontemporal_store(x, a);
+        __builtin_nontemporal_store(y, a + 1);
+        __builtin_nontemporal_store(v[4], p + 4*(size_t)N + i);
+    } else {
+        a[0] = x;
+        a[1] = y;
+        p[4*(size_t)N + i] = v[4];
+    }
+}
+
+// a 5-vector of cell i from a split (SPLIT) or [var][cell] array
+template <bool SPLIT, typename T>
+__device__ __forceinline__ void ld_vec(const T *p, int N, int i, T *v)
+{
+    if (SPLIT) {
+        ld_split<false>(p, N, i, v);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = p[(size_t)q*N + i];
+    }
+}
+
 // Workgroup ids are dealt round-robin over the 8 XCDs (each with its own
 // L2); each XCD takes runs of 32 consecutive workgroups (4096 cells), the
 // XCDs neighbouring runs, so adjacent cells share one L2 and the XCDs work
@@ -157,10 +223,8 @@ __global__ __launch_bounds__(kSB) void k_lower_res(
         const T dg = ldg_nt(diag + i);
         T s[NV], ei[NV];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            s[q] = ldg_nt(rhs + (size_t)q*N + i);
-            if (WITH_RES) ei[q] = ldg_nt(e + (size_t)q*N + i);
-        }
+        for (int q = 0; q < NV; ++q) s[q] = ldg_nt(rhs + (size_t)q*N + i);
+        if (WITH_RES) ld_split<true>(e, N, i, ei);
         bool res_f[FC], low_f[FC];
         Face4<T> g[FC];
         T U[FC][NV], dr[FC][NV], dl[FC][NV];
@@ -171,12 +235,9 @@ __global__ __launch_bounds__(kSB) void k_lower_res(
             low_f[f] = nb < n0;
             if (!(res_f[f] || low_f[f])) continue;
             g[f] = ldg_nt(geo + (size_t)f*N + i);
-#pragma unroll
-            for (int q = 0; q < NV; ++q) {
-                U[f][q] = qb[(size_t)q*N + nb];
-                if (res_f[f]) dr[f][q] = e[(size_t)q*N + nb];
-                if (low_f[f]) dl[f][q] = dstar[(size_t)q*N + nb];
-            }
+            ld_split<false>(qb, N, nb, U[f]);
+            if (res_f[f]) ld_split<false>(e, N, nb, dr[f]);
+            if (low_f[f]) ld_split<false>(dstar, N, nb, dl[f]);
         }
         if (WITH_RES) {
             T res[NV];
@@ -206,9 +267,10 @@ __global__ __launch_bounds__(kSB) void k_lower_res(
 #pragma unroll
             for (int q = 0; q < NV; ++q) acc[q] += o[q];
         }
+        T o[NV];
 #pragma unroll
-        for (int q = 0; q < NV; ++q)
-            __builtin_nontemporal_store((s[q] - T(0.5)*acc[q])/dg, dstar + (size_t)q*N + i);
+        for (int q = 0; q < NV; ++q) o[q] = (s[q] - T(0.5)*acc[q])/dg;
+        st_split<true>(dstar, N, i, o);
     }
     block_sum<kSB>(ss, part, pofs + b);
 }
@@ -229,11 +291,8 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
     for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
     const T dg = ldg_nt(diag + i);
     T ds[NV], ei[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        ds[q] = ldg_nt(dstar + (size_t)q*N + i);
-        ei[q] = ldg_nt(e + (size_t)q*N + i);
-    }
+    ld_split<true>(dstar, N, i, ds);
+    ld_split<true>(e, N, i, ei);
     bool up_f[FC];
     Face4<T> g[FC];
     T U[FC][NV], d[FC][NV];
@@ -243,11 +302,8 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
         up_f[f] = nb >= ne;
         if (!up_f[f]) continue;
         g[f] = ldg_nt(geo + (size_t)f*N + i);
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            U[f][q] = qb[(size_t)q*N + nb];
-            d[f][q] = delta[(size_t)q*N + nb];
-        }
+        ld_split<false>(qb, N, nb, U[f]);
+        ld_split<false>(delta, N, nb, d[f]);
     }
     T acc[NV];
 #pragma unroll
@@ -260,13 +316,14 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
 #pragma unroll
         for (int q = 0; q < NV; ++q) acc[q] += o[q];
     }
+    T dl[NV], en[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        const size_t o = (size_t)q*N + i;
-        const T dl = ds[q] - T(0.5)*acc[q]/dg;
-        __builtin_nontemporal_store(dl, delta + o);
-        __builtin_nontemporal_store(ei[q] + dl, e + o);
+        dl[q] = ds[q] - T(0.5)*acc[q]/dg;
+        en[q] = ei[q] + dl[q];
     }
+    st_split<true>(delta, N, i, dl);
+    st_split<true>(e, N, i, en);
 }
 
 // ---- the first colour's upper sweep and update (k_upper_upd on [0, cnt)),
@@ -296,12 +353,10 @@ __global__ __launch_bounds__(kSB) void k_upper_upd_res(
         for (int f = 0; f < FC; ++f) nbv[f] = ldg_nt(nei + (size_t)f*N + i);
         const T dg = ldg_nt(diag + i);
         T ds[NV], ei[NV], rh[NV];
+        ld_split<true>(dstar, N, i, ds);
+        ld_split<true>(e, N, i, ei);
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            ds[q] = ldg_nt(dstar + (size_t)q*N + i);
-            ei[q] = ldg_nt(e + (size_t)q*N + i);
-            rh[q] = ldg_nt(rhs + (size_t)q*N + i);
-        }
+        for (int q = 0; q < NV; ++q) rh[q] = ldg_nt(rhs + (size_t)q*N + i);
         bool up_f[FC];
         Face4<T> g[FC];
         T U[FC][NV], d[FC][NV], en[FC][NV];
@@ -311,12 +366,9 @@ __global__ __launch_bounds__(kSB) void k_upper_upd_res(
             up_f[f] = nb >= ne;
             if (!up_f[f]) continue;
             g[f] = ldg_nt(geo + (size_t)f*N + i);
-#pragma unroll
-            for (int q = 0; q < NV; ++q) {
-                U[f][q] = qb[(size_t)q*N + nb];
-                d[f][q] = delta[(size_t)q*N + nb];
-                en[f][q] = e[(size_t)q*N + nb];
-            }
+            ld_split<false>(qb, N, nb, U[f]);
+            ld_split<false>(delta, N, nb, d[f]);
+            ld_split<false>(e, N, nb, en[f]);
         }
         T acc[NV];
 #pragma unroll
@@ -329,15 +381,14 @@ __global__ __launch_bounds__(kSB) void k_upper_upd_res(
 #pragma unroll
             for (int q = 0; q < NV; ++q) acc[q] += o[q];
         }
-        T enew[NV];
+        T enew[NV], dl[NV];
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-            const size_t o = (size_t)q*N + i;
-            const T dl = ds[q] - T(0.5)*acc[q]/dg;
-            enew[q] = ei[q] + dl;
-            __builtin_nontemporal_store(dl, delta + o);
-            __builtin_nontemporal_store(enew[q], e + o);
+            dl[q] = ds[q] - T(0.5)*acc[q]/dg;
+            enew[q] = ei[q] + dl[q];
         }
+        st_split<true>(delta, N, i, dl);
+        st_split<true>(e, N, i, enew);
         T res[NV];
 #pragma unroll
         for (int q = 0; q < NV; ++q) res[q] = T(0);
@@ -371,15 +422,16 @@ __global__ __launch_bounds__(kSB) void k_lower_pre(int cnt, int N, const T *__re
     if (t >= cnt) return;
     const T dg = ldg_nt(diag + t);
     const T acc = T(0);
+    T o[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        const T sq = ldg_nt(sres + (size_t)q*N + t);
-        __builtin_nontemporal_store((sq - T(0.5)*acc)/dg, dstar + (size_t)q*N + t);
-    }
+    for (int q = 0; q < NV; ++q) o[q] = (ldg_nt(sres + (size_t)q*N + t) - T(0.5)*acc)/dg;
+    st_split<true>(dstar, N, t, o);
 }
 
-// ---- FP64 residual r = b - A(dq) on base Q over all cells.  Optionally
-// writes fl32(r) and the FP32 base state fl32(Q + dq) for the inner loop.
+// ---- FP64 residual r = b - A(dq) on base Q (split layout) over all cells;
+// dq in the split layout (DQS: the solver's own increment) or [var][cell]
+// (a caller's array).  Optionally writes fl32(r) ([var][cell]) and the FP32
+// base state fl32(Q + dq) (split) for the inner loop.
 // Launched once per colour range [n0, n0+cnt) like the sweeps: all XCDs
 // then work on slices of one colour and its neighbours at a time.
 // ZERO: dq is known to be zero (the first outer step).  Every face term of a
@@ -387,7 +439,7 @@ __global__ __launch_bounds__(kSB) void k_lower_pre(int cnt, int N, const T *__re
 // states, or products with zero summed onto +0), so r = b - (D*0 + 0.5*0)
 // bit for bit without the neighbour gathers; the partial sums keep the
 // same blocks and order, so the norm is identical too.
-template <int FC, bool OUT32, bool ZERO = false>
+template <int FC, bool OUT32, bool ZERO, bool DQS>
 __global__ __launch_bounds__(kSB) void k_residual64(
     int n0, int cnt, int N, const int *__restrict__ nei, const Face4<double> *__restrict__ geo,
     const double *__restrict__ diag, const double *__restrict__ q, const double *__restrict__ b,
@@ -405,12 +457,9 @@ __global__ __launch_bounds__(kSB) void k_residual64(
         const double dg = ldg_nt(diag + i);
         double bi[NV], dqi[NV], qi[NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const size_t o = (size_t)k*N + i;
-            bi[k] = ldg_nt(b + o);
-            dqi[k] = dq[o];
-            if (OUT32) qi[k] = q[o];
-        }
+        for (int k = 0; k < NV; ++k) bi[k] = ldg_nt(b + (size_t)k*N + i);
+        ld_vec<DQS>(dq, N, i, dqi);
+        if (OUT32) ld_split<false>(q, N, i, qi);
         bool in_f[FC];
         Face4<double> g[FC];
         double U[FC][NV], d[FC][NV];
@@ -420,11 +469,8 @@ __global__ __launch_bounds__(kSB) void k_residual64(
             in_f[f] = !ZERO && nb != i;
             if (!in_f[f]) continue;
             g[f] = ldg_nt(geo + (size_t)f*N + i);
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
-                U[f][k] = q[(size_t)k*N + nb];
-                d[f][k] = dq[(size_t)k*N + nb];
-            }
+            ld_split<false>(q, N, nb, U[f]);
+            ld_vec<DQS>(dq, N, nb, d[f]);
         }
         double res[NV];
 #pragma unroll
@@ -437,6 +483,7 @@ __global__ __launch_bounds__(kSB) void k_residual64(
 #pragma unroll
             for (int k = 0; k < NV; ++k) res[k] += o[k];
         }
+        float qo[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             const size_t o = (size_t)k*N + i;
@@ -445,9 +492,10 @@ __global__ __launch_bounds__(kSB) void k_residual64(
             if (r) r[o] = rv;
             if (OUT32) {
                 r32[o] = (float)rv;
-                qb32[o] = (float)(qi[k] + dqi[k]);
+                qo[k] = (float)(qi[k] + dqi[k]);
             }
         }
+        if (OUT32) st_split<false>(qb32, N, i, qo);
     }
     block_sum<kSB>(ss, part, pofs + blk);
 }
@@ -584,6 +632,27 @@ template <typename T> __global__ void k_fill(int64_t n, T *y, T v)
     for (int64_t j = (int64_t)blockIdx.x*kB + threadIdx.x; j < n; j += st) y[j] = v;
 }
 
+// [var][cell] <-> split layout of the solver's 5-vectors
+__global__ void k_rows_to_split(int N, const double *__restrict__ rows, double *__restrict__ sp)
+{
+    const int i = blockIdx.x*kB + threadIdx.x;
+    if (i >= N) return;
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = rows[(size_t)q*N + i];
+    st_split<false>(sp, N, i, v);
+}
+
+__global__ void k_split_to_rows(int N, const double *__restrict__ sp, double *__restrict__ rows)
+{
+    const int i = blockIdx.x*kB + threadIdx.x;
+    if (i >= N) return;
+    double v[NV];
+    ld_split<false>(sp, N, i, v);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) rows[(size_t)q*N + i] = v[q];
+}
+
 // packs S = fnv*n, L = fnv*lam and D = 1/dt + 1/2 sum fspr*fnv (FP64 and FP32)
 __global__ void k_pack(int N, int F, const double *__restrict__ fnv, const double *__restrict__ vfn,
                        const double *__restrict__ fspr, const double *__restrict__ dt,
@@ -625,7 +694,9 @@ struct ucfd_lusgs_solver {
     int *nei = nullptr;
     Face4<double> *g64 = nullptr;
     Face4<float> *g32 = nullptr;
-    double *d64 = nullptr, *q64 = nullptr, *r64 = nullptr, *dstar64 = nullptr,
+    // q64 (base state), dq64 (the solve's increment), dstar*/delta*, e32 and
+    // qb32: split layout; r32 and sres*: [var][cell]
+    double *d64 = nullptr, *q64 = nullptr, *dq64 = nullptr, *dstar64 = nullptr,
            *delta64 = nullptr;
     float *d32 = nullptr, *qb32 = nullptr, *r32 = nullptr, *e32 = nullptr, *dstar32 = nullptr,
           *delta32 = nullptr;
@@ -723,26 +794,29 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
     return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP;
 }
 
-// zero_dq: dq is known to be zero (only with out32, the first outer step)
+// zero_dq: dq is known to be zero (only with out32, the first outer step);
+// dq_split: dq is in the split layout (the solver's dq64), else [var][cell]
 int residual64(ucfd_lusgs_solver *s, hipStream_t st, const double *b, const double *dq, double *r,
-               bool out32, bool zero_dq = false)
+               bool out32, bool zero_dq, bool dq_split)
 {
     int pofs = 0;
     for (int c = 0; c < s->nc; ++c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
-#define R(FC, O, Z) hipLaunchKernelGGL((k_residual64<FC, O, Z>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
-                                       s->nei, s->g64, s->d64, s->q64, b, dq, r, s->r32, s->qb32, \
-                                       s->part, pofs)
+#define R(FC, O, Z, S) hipLaunchKernelGGL((k_residual64<FC, O, Z, S>), g, dim3(kSB), 0, st, n0, \
+                                          cnt, s->N, s->nei, s->g64, s->d64, s->q64, b, dq, r, \
+                                          s->r32, s->qb32, s->part, pofs)
         if (s->F == 4) {
-            if (out32 && zero_dq) R(4, true, true);
-            else if (out32) R(4, true, false);
-            else R(4, false, false);
+            if (out32 && zero_dq) R(4, true, true, true);
+            else if (out32) R(4, true, false, true);
+            else if (dq_split) R(4, false, false, true);
+            else R(4, false, false, false);
         } else {
-            if (out32 && zero_dq) R(6, true, true);
-            else if (out32) R(6, true, false);
-            else R(6, false, false);
+            if (out32 && zero_dq) R(6, true, true, true);
+            else if (out32) R(6, true, false, true);
+            else if (dq_split) R(6, false, false, true);
+            else R(6, false, false, false);
         }
 #undef R
         pofs += (int)g.x;
@@ -807,18 +881,19 @@ UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nf
     for (int c = 0; c < ncolors; ++c) maxparts += (s->off[c + 1] - s->off[c] + kSB - 1)/kSB;
     s->nparts = std::max(maxparts, (neles + kSB - 1)/kSB) + 8;
     const hipStream_t st = ucfd_stream();
-    double *tfnv = nullptr, *tvfn = nullptr, *tsp = nullptr, *tdt = nullptr;
+    double *tfnv = nullptr, *tvfn = nullptr, *tsp = nullptr, *tdt = nullptr, *tq = nullptr;
     if ((rc = dalloc(s, &s->nei, F*N, fn)) || (rc = dalloc(s, &s->g64, F*N, fn)) ||
         (rc = dalloc(s, &s->g32, F*N, fn)) || (rc = dalloc(s, &s->d64, N, fn)) ||
         (rc = dalloc(s, &s->d32, N, fn)) || (rc = dalloc(s, &s->q64, NV*N, fn)) ||
-        (rc = dalloc(s, &s->r64, NV*N, fn)) || (rc = dalloc(s, &s->dstar64, NV*N, fn)) ||
+        (rc = dalloc(s, &s->dq64, NV*N, fn)) || (rc = dalloc(s, &s->dstar64, NV*N, fn)) ||
         (rc = dalloc(s, &s->delta64, NV*N, fn)) || (rc = dalloc(s, &s->qb32, NV*N, fn)) ||
         (rc = dalloc(s, &s->r32, NV*N, fn)) || (rc = dalloc(s, &s->e32, NV*N, fn)) ||
         (rc = dalloc(s, &s->dstar32, NV*N, fn)) || (rc = dalloc(s, &s->delta32, NV*N, fn)) ||
         (rc = dalloc(s, &s->sres64, NV*N, fn)) || (rc = dalloc(s, &s->sres32, NV*N, fn)) ||
         (rc = dalloc(s, &s->part, (size_t)s->nparts, fn)) || (rc = dalloc(s, &s->norm, 8, fn)) ||
         (rc = dalloc(s, &tfnv, F*N, fn)) || (rc = dalloc(s, &tvfn, 3*F*N, fn)) ||
-        (rc = dalloc(s, &tsp, F*N, fn)) || (rc = dalloc(s, &tdt, N, fn))) {
+        (rc = dalloc(s, &tsp, F*N, fn)) || (rc = dalloc(s, &tdt, N, fn)) ||
+        (rc = dalloc(s, &tq, NV*N, fn))) {
         free_all(s);
         delete s;
         return rc;
@@ -836,14 +911,16 @@ UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nf
     if (!rc) rc = copy_in(tvfn, vec_fnorm, 3*F*N*8, st, fn);
     if (!rc) rc = copy_in(tsp, fspr, F*N*8, st, fn);
     if (!rc) rc = copy_in(tdt, dt, N*8, st, fn);
-    if (!rc) rc = copy_in(s->q64, upts, NV*N*8, st, fn);
+    if (!rc) rc = copy_in(tq, upts, NV*N*8, st, fn);
     if (!rc) {
+        hipLaunchKernelGGL(k_rows_to_split, dim3((neles + kB - 1)/kB), dim3(kB), 0, st, neles,
+                           tq, s->q64);
         hipLaunchKernelGGL(k_pack, dim3((neles + kB - 1)/kB), dim3(kB), 0, st, neles, nface, tfnv,
                            tvfn, tsp, tdt, s->g64, s->g32, s->d64, s->d32);
         rc = ucfd_check_hip(hipStreamSynchronize(st), fn, "pack");
     }
-    // the FP64 staging copies of the raw geometry are no longer needed
-    for (void *p : {(void *)tfnv, (void *)tvfn, (void *)tsp, (void *)tdt}) {
+    // the FP64 staging copies of the raw geometry and state are no longer needed
+    for (void *p : {(void *)tfnv, (void *)tvfn, (void *)tsp, (void *)tdt, (void *)tq}) {
         (void)hipFree(p);
         s->allocs.erase(std::remove(s->allocs.begin(), s->allocs.end(), p), s->allocs.end());
     }
@@ -879,7 +956,7 @@ UCFD_API int ucfd_lusgs_residual(ucfd_lusgs_solver *s, const double *b, const do
     if (rc) return rc;
     const hipStream_t st = ucfd_stream();
     rc = residual64(s, st, (const double *)sg.dev(ab), (const double *)sg.dev(aq),
-                    ar >= 0 ? (double *)sg.dev(ar) : nullptr, false);
+                    ar >= 0 ? (double *)sg.dev(ar) : nullptr, false, false, false);
     double nrm = 0.0;
     if (!rc) rc = read_norm(s, st, &nrm, fn);
     if (rnorm) *rnorm = nrm;
@@ -906,7 +983,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     double *DQ = (double *)sg.dev(aq);
     ucfd_solve_stats info;
     memset(&info, 0, sizeof info);
-    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, DQ, 0.0);
+    // the increment is accumulated in the split dq64 and written to DQ at the end
+    double *E = s->dq64;
+    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, E, 0.0);
     double bnorm = 0.0, rn = 0.0;
 
     auto record = [&](double v) {
@@ -916,9 +995,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     if (mode == UCFD_SOLVE_FP64) {
         // e = dq; iteration j: s = b - A(e) (in the lower launches), e += M^-1 s
         for (int it = 0; it < maxiter && !rc; ++it) {
-            rc = lower_pass<double>(s, st, it > 0, s->q64, B, DQ, s->dstar64, s->g64, s->d64,
+            rc = lower_pass<double>(s, st, it > 0, s->q64, B, E, s->dstar64, s->g64, s->d64,
                                     it > 0, s->sres64);
-            if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, DQ, s->g64,
+            if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, E, s->g64,
                                              s->d64, B, s->sres64);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
@@ -929,13 +1008,13 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
         }
         // the last check saw the iterate before its update; report the
         // residual of the dq actually returned
-        if (!rc) rc = residual64(s, st, B, DQ, nullptr, false);
+        if (!rc) rc = residual64(s, st, B, E, nullptr, false, false, true);
         if (!rc) rc = read_norm(s, st, &rn, fn);
         info.outer = info.passes + 1;
     } else {
         // outer FP64 residual; inner FP32 defect correction on the correction
         while (!rc && info.passes < maxiter) {
-            rc = residual64(s, st, B, DQ, nullptr, true, info.outer == 0);
+            rc = residual64(s, st, B, E, nullptr, true, info.outer == 0, true);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
             if (info.outer == 0) bnorm = rn;
@@ -958,14 +1037,20 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
                 if (j == 0) s0 = sn;
                 if (s0 == 0.0 || sn <= inner_tol*s0) break;
             }
-            hipLaunchKernelGGL(k_add32, grid1(n), dim3(kB), 0, st, (int64_t)n, DQ, s->e32);
+            // dq64 and e32 share the split layout: elementwise over 5N
+            hipLaunchKernelGGL(k_add32, grid1(n), dim3(kB), 0, st, (int64_t)n, E, s->e32);
         }
         if (!rc && info.passes >= maxiter) {
             // report the residual of the returned dq
-            rc = residual64(s, st, B, DQ, nullptr, false);
+            rc = residual64(s, st, B, E, nullptr, false, false, true);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             record(bnorm > 0 ? rn/bnorm : 0.0);
         }
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_split_to_rows, dim3((s->N + kB - 1)/kB), dim3(kB), 0, st, s->N, E,
+                           DQ);
+        rc = ucfd_check_hip(hipGetLastError(), fn, "kernel launch");
     }
     info.rnorm0 = bnorm;
     info.rel_residual = bnorm > 0 ? rn/bnorm : 0.0;
@@ -997,14 +1082,14 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
             for (int j = 0; j < k; ++j) (void)hipEventDestroy(ev[j]);
             return rc;
         }
-    // e = dq = 0 (r64 serves as both): FP64 and FP32 state of a first step
-    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->r64, 0.0);
+    // e = dq = 0 (dq64 serves as both): FP64 and FP32 state of a first step
+    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->dq64, 0.0);
     hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
-    int rc = residual64(s, st, b, s->r64, nullptr, true, true);
+    int rc = residual64(s, st, b, s->dq64, nullptr, true, true, true);
     // one untimed pass of each form, so the first colour's residual exists
-    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->r64, s->dstar64, s->g64, s->d64);
-    if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->r64, s->g64, s->d64,
-                                     b, s->sres64);
+    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64);
+    if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
+                                     s->d64, b, s->sres64);
     if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
                                     s->d32);
     if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32, s->d32,
@@ -1012,10 +1097,10 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps && !rc; ++r) {
         (void)hipEventRecord(ev[0], st);
-        rc = lower_pass<double>(s, st, true, s->q64, b, s->r64, s->dstar64, s->g64, s->d64, true,
+        rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64, true,
                                 s->sres64);
         (void)hipEventRecord(ev[1], st);
-        if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->r64, s->g64,
+        if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
                                          s->d64, b, s->sres64);
         (void)hipEventRecord(ev[2], st);
         if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
@@ -1024,7 +1109,7 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
         if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32,
                                         s->d32, s->r32, s->sres32);
         (void)hipEventRecord(ev[4], st);
-        if (!rc) rc = residual64(s, st, b, s->r64, nullptr, false);
+        if (!rc) rc = residual64(s, st, b, s->dq64, nullptr, false, false, true);
         (void)hipEventRecord(ev[5], st);
         if (!rc) rc = ucfd_check_hip(hipEventSynchronize(ev[5]), fn, "hipEventSynchronize");
         for (int k = 0; k < 5 && !rc; ++k) {
