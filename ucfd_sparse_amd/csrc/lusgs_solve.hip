@@ -326,6 +326,33 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
     st_split<true>(e, N, i, en);
 }
 
+// ---- the last colour's upper sweep and update: no neighbour lies at or
+// above N, so no face is taken and the neighbour indices are not read;
+// k_upper_upd's expressions with an empty face sum.  The outputs are stored
+// with ordinary (temporal) stores: the next launch gathers them.
+template <typename T>
+__global__ __launch_bounds__(kSB) void k_upper_free(int n0, int cnt, int N,
+                                                    const T *__restrict__ diag,
+                                                    const T *__restrict__ dstar, T *delta, T *e)
+{
+    const int t = xcd_remap()*kSB + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = n0 + t;
+    const T dg = ldg_nt(diag + i);
+    T ds[NV], ei[NV];
+    ld_split<true>(dstar, N, i, ds);
+    ld_split<true>(e, N, i, ei);
+    const T acc = T(0);
+    T dl[NV], en[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        dl[q] = ds[q] - T(0.5)*acc/dg;
+        en[q] = ei[q] + dl[q];
+    }
+    st_split<false>(delta, N, i, dl);
+    st_split<false>(e, N, i, en);
+}
+
 // ---- the first colour's upper sweep and update (k_upper_upd on [0, cnt)),
 // followed by the defect-correction residual of the next pass for the same
 // cells, s_i = rhs_i - A(e)_i with the updated e.  The first colour is the
@@ -425,7 +452,7 @@ __global__ __launch_bounds__(kSB) void k_lower_pre(int cnt, int N, const T *__re
     T o[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) o[q] = (ldg_nt(sres + (size_t)q*N + t) - T(0.5)*acc)/dg;
-    st_split<true>(dstar, N, t, o);
+    st_split<false>(dstar, N, t, o);
 }
 
 // ---- FP64 residual r = b - A(dq) on base Q (split layout) over all cells;
@@ -782,6 +809,11 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
             else
                 hipLaunchKernelGGL((k_upper_upd_res<T, 6>), g, dim3(kSB), 0, st, cnt, s->N, s->nei,
                                    geo, diag, qb, dstar, delta, e, rhs, sres, s->part, 0);
+            continue;
+        }
+        if (c == s->nc - 1) {
+            hipLaunchKernelGGL(k_upper_free<T>, g, dim3(kSB), 0, st, n0, cnt, s->N, diag, dstar,
+                               delta, e);
             continue;
         }
         if (s->F == 4)
