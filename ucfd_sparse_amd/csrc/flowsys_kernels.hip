@@ -899,7 +899,14 @@ __global__ __launch_bounds__(FB) void k_lsweep(int base, int cnt, const int *__r
 // cell's records in position order (coalesced) instead of gathering five
 // values through the slot index; every stored value is the product the
 // sweep itself forms (lusgs.c:185-195), so the results do not change by a
-// bit.
+// bit.  A wave's 64 cells own one contiguous range of positions: the
+// records of its first LPCAP positions are staged in LDS and written out
+// with consecutive lanes on consecutive records (one lane per cell would
+// write 32-byte pieces at the position stride, partial cache lines that
+// measured 2.15x the record bytes in HBM writes); positions beyond LPCAP
+// (cells with many faces) are written directly.
+constexpr int LPCAP = 256;   // staged positions per wave (64 tets)
+
 template <int ND>
 __global__ __launch_bounds__(FB) void k_lprep_pos(int nlocal, int64_t nfaces, int nv0, int nv1,
                                                   double kappa, const int *__restrict__ rowptr,
@@ -912,10 +919,16 @@ __global__ __launch_bounds__(FB) void k_lprep_pos(int nlocal, int64_t nfaces, in
                                                   double *diag, double4 *__restrict__ geo,
                                                   double *__restrict__ lam)
 {
-    const int r = fxcd()*FB + threadIdx.x;
-    if (r >= nlocal) return;
-    const double rv = rcp_vol[r];
-    const int p0 = rowptr[r], p1 = rowptr[r + 1];
+    __shared__ double4 sgeo[FB/64][LPCAP];
+    __shared__ double slam[FB/64][LPCAP];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int w0 = fxcd()*FB + wv*64;   // the wave's first cell
+    if (w0 >= nlocal) return;
+    const int r = w0 + lane;
+    const bool act = r < nlocal;
+    const int P0 = rowptr[w0], P1 = rowptr[min(w0 + 64, nlocal)];
+    const int p0 = act ? rowptr[r] : 0, p1 = act ? rowptr[r + 1] : 0;
+    const double rv = act ? rcp_vol[r] : 0.0;
     // the slot indices of the first LKP positions together, then all their
     // gathers, then the face terms in position order
     int64_t sl[LKP];
@@ -942,14 +955,18 @@ __global__ __launch_bounds__(FB) void k_lprep_pos(int nlocal, int64_t nfaces, in
     double sd = 0.0;
     auto face = [&](int p, double lfk, double fak, const double *fnk, double sidek) {
         sd += 0.5*(lfk*kappa)*fak*rv;
+        const int j = p - P0;
         if (geo) {
             double g[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < ND; ++d) g[d] = sidek*fnk[d];
             g[ND] = fak*rv;
-            geo[p] = double4{g[0], g[1], g[2], g[3]};
+            const double4 gv = double4{g[0], g[1], g[2], g[3]};
+            if (j < LPCAP) sgeo[wv][j] = gv;
+            else geo[p] = gv;
         }
-        lam[p] = lfk;
+        if (j < LPCAP) slam[wv][j] = lfk;
+        else lam[p] = lfk;
     };
 #pragma unroll
     for (int k = 0; k < LKP; ++k)
@@ -965,7 +982,14 @@ __global__ __launch_bounds__(FB) void k_lprep_pos(int nlocal, int64_t nfaces, in
         }
         face(p, lambdaf[s2], face_area[s2], f2, sd2);
     }
-    for (int k = nv0; k < nv1; ++k) diag[r + (int64_t)k*nlocal] += sd;
+    if (act)
+        for (int k = nv0; k < nv1; ++k) diag[r + (int64_t)k*nlocal] += sd;
+    ucfd::wave_lds_sync();
+    const int nst = min(P1 - P0, LPCAP);
+    for (int j = lane; j < nst; j += 64) {
+        if (geo) geo[P0 + j] = sgeo[wv][j];
+        lam[P0 + j] = slam[wv][j];
+    }
 }
 
 // k_lsweep reading the per-position records of k_lprep_pos
