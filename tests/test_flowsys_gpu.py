@@ -307,16 +307,24 @@ def quad_mesh(m, n, seed):
 
 
 @pytest.mark.parametrize("kind, nt, colored", [("quad", 1, False), ("quad", 2, True),
-                                               ("tet", 2, True), ("tet", 4, False)])
+                                               ("tet", 2, True), ("tet", 4, False),
+                                               ("hex", 1, False), ("hex", 2, True)])
 def test_lusgs_two_dimensions_and_turbulence_rows(kind, nt, colored):
     """The scalar method in 2-D (nfvars = 4) and with 1 to 4 turbulence rows,
     on a random and on a colour-ordered numbering (the first schedule level
     then an index list or a range): every stage of two iterations bit for bit
-    against the oracle (lusgs.c:8-321)."""
+    against the oracle (lusgs.c:8-321).  Hex cells (six faces) give a wave
+    more positions than the Prepare stages in LDS, so the rest of them are
+    written directly."""
     from ucfd_sparse_amd import flowsys as FS
     from ucfd_sparse_amd.layout import colored_layout
     seed = 61 + nt
-    mesh = quad_mesh(9, 7, seed) if kind == "quad" else M.tet_mesh(3, seed=seed)
+    if kind == "quad":
+        mesh = quad_mesh(9, 7, seed)
+    elif kind == "hex":
+        mesh = M.hex_mesh(7, 6, 5, seed=seed, permute=True)
+    else:
+        mesh = M.tet_mesh(3, seed=seed)
     nfv = mesh.ndims + 2
     flow = M.flow_fields(mesh, seed=seed, nvars=nfv + nt)
     if colored:
