@@ -70,8 +70,8 @@ struct _FlowSys {
     // per-position face records written by the last NS / RANS Prepare
     // (geometry, spectral radius), read by the sweeps that follow
     double *lu_geo = nullptr, *lu_lam = nullptr, *lu_tlam = nullptr;
-    int *lu_inv = nullptr;          // Pack's inverse of cell_ids [nlocal]
-    double *lu_rec = nullptr;       // Pack / Update records [nlocal][fs_rec_words(nvars)]
+    int *lu_inv = nullptr;          // Pack's inverse of cell_ids [nlocal] (both methods)
+    double *lu_rec = nullptr;       // Pack / Update records [nlocal][fs_rec_words(nvars)] (both)
     bool lu_geo_ok = false, lu_lam_ok = false, lu_tlam_ok = false;
     int64_t npos = 0;
     bool fp32 = false;
@@ -464,11 +464,15 @@ UCFD_API ucfd_status_t UCFDBLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal a0)
     const int ic = sg.add(e->cell_ids, sizeof(int)*ne, UCFD_IN);
     const int ir = sg.add(e->rhs, sizeof(double)*ne*sys->nvars, UCFD_IN);
     const int idt = sg.add(e->dt, sizeof(double)*ne, UCFD_IN);
+    if (dalloc(&sys->lu_inv, (size_t)sys->nlocal, fn) ||
+        dalloc(&sys->lu_rec, (size_t)sys->nlocal*fs_rec_words(sys->nvars), fn))
+        return UCFD_FAILED;
     int rc = sg.enter();
     if (rc) return UCFD_FAILED;
     rc = fs_launch_pack(ucfd_stream(), e->neles, sys->nlocal, sys->nvars, sys->nfvars, a0,
                         (const int *)sg.dev(ic), (const double *)sg.dev(ir),
-                        (const double *)sg.dev(idt), sys->rhs, sys->diag, ucfd_device_flag());
+                        (const double *)sg.dev(idt), sys->rhs, sys->diag, ucfd_device_flag(),
+                        sys->lu_inv, sys->lu_rec);
     return status(sg.leave(rc));
 }
 

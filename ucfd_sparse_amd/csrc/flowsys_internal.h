@@ -8,9 +8,12 @@
 
 // Element-level kernels; cell_ids entries outside [0, nlocal) raise
 // UCFD_KFLAG_INDEX and are skipped.
+// inv [nlocal] and rec [nlocal * ((nvars + 2) & ~1)] (both optional): the
+// record form of the pack
 UCFD_HIDDEN int fs_launch_pack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
                                double a0, const int *cell_ids, const double *rhs, const double *dt,
-                               double *rank_rhs, double *diag, int *flag);
+                               double *rank_rhs, double *diag, int *flag, int *inv = nullptr,
+                               double *rec = nullptr);
 // kind 1 = Spalart-Allmaras, 2 = k-omega SST source Jacobian
 UCFD_HIDDEN int fs_launch_tpack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars,
                                 double a0, double factor, int kind, const int *cell_ids,

@@ -40,6 +40,7 @@ int ok() { return hipGetLastError() == hipSuccess ? UCFD_OK : UCFD_ERR_HIP; }
 // words of each row (one thread per element would scatter every store over
 // up to 64 cache lines).
 constexpr int PACK_MAXV = 16;
+typedef double fs_d2 __attribute__((ext_vector_type(2)));
 
 __global__ __launch_bounds__(64) void k_pack(int neles, int nlocal, int nvars, int nfvars, double a0,
                                              const int *__restrict__ cell_ids,
@@ -81,6 +82,93 @@ __global__ __launch_bounds__(64) void k_pack(int neles, int nlocal, int nvars, i
         if (f < nact*d2 && re >= 0) {
             const int row = q/nfvars, col = q - row*nfvars;
             diag[(int64_t)re*d2 + q] = row == col ? de : 0.0;
+        }
+    }
+}
+
+// The same through one record per cell, for cell_ids that scatter elements
+// over the rank cells: (1) element order, rhs read coalesced, the element's
+// diagonal value 1/dt + a0 and rhs row written as one contiguous record
+// [d, rhs_0 .. rhs_{nvars-1}] at its cell (and inv[r] = e); (2) rank order,
+// a wave's 64 consecutive cells read their records and write the rank rows
+// and diagonal blocks as flat ranges, consecutive lanes on consecutive words,
+// skipping cells that no element of this type maps to (inv[r] < 0).  The
+// scattered writes shrink from a rank row and a whole diagonal block per
+// element to one record of (nvars + 2) & ~1 doubles (nvars 4-9; k_pack
+// otherwise).
+template <int NV>
+__global__ __launch_bounds__(FB) void k_bpack_rec(int neles, int nlocal, double a0,
+                                                  const int *__restrict__ cell_ids,
+                                                  int *__restrict__ inv,
+                                                  const double *__restrict__ rhs,
+                                                  const double *__restrict__ dt,
+                                                  double *__restrict__ rec, int *flag)
+{
+    constexpr int W = (NV + 2) & ~1;
+    const int e = fxcd()*FB + threadIdx.x;
+    if (e >= neles) return;
+    const int r = cell_ids[e];
+    if ((unsigned)r >= (unsigned)nlocal) {
+        atomicOr(flag, UCFD_KFLAG_INDEX);
+        return;
+    }
+    inv[r] = e;
+    double v[W];
+    v[0] = 1/dt[e] + a0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[1 + k] = rhs[e + (int64_t)k*neles];
+    if (W > NV + 1) v[W - 1] = 0.0;
+    fs_d2 *o = (fs_d2 *)(rec + (int64_t)r*W);
+#pragma unroll
+    for (int q = 0; q < W/2; ++q) o[q] = fs_d2{v[2*q], v[2*q + 1]};
+}
+
+template <int NV>
+__global__ __launch_bounds__(64) void k_bpack_from_rec(int nlocal, int nfvars,
+                                                       const int *__restrict__ inv,
+                                                       const double *__restrict__ rec,
+                                                       double *rank_rhs, double *diag)
+{
+    constexpr int W = (NV + 2) & ~1;
+    __shared__ double srhs[64*NV];
+    const int lane = threadIdx.x;
+    const int r0 = ucfd::wave_xcd_index()*64;
+    if (r0 >= nlocal) return;
+    const int nact = min(64, nlocal - r0);
+    const int r = r0 + lane;
+    int has = 0;
+    double dg = 0.0;
+    if (lane < nact && inv[r] >= 0) {
+        has = 1;
+        const fs_d2 *in = (const fs_d2 *)(rec + (int64_t)r*W);
+        double v[W];
+#pragma unroll
+        for (int q = 0; q < W/2; ++q) {
+            const fs_d2 t = in[q];
+            v[2*q] = t.x;
+            v[2*q + 1] = t.y;
+        }
+        dg = v[0];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) srhs[lane*NV + k] = v[1 + k];
+    }
+    ucfd::wave_lds_sync();
+    // uniform trip counts: every lane takes part in every shuffle
+    for (int f0 = 0; f0 < nact*NV; f0 += 64) {
+        const int f = f0 + lane;
+        const int el = min(f/NV, nact - 1);
+        const int h = __shfl(has, el, 64);
+        if (f < nact*NV && h) rank_rhs[(int64_t)r0*NV + f] = srhs[f];
+    }
+    const int d2 = nfvars*nfvars;
+    for (int f0 = 0; f0 < nact*d2; f0 += 64) {
+        const int f = f0 + lane;
+        const int el = min(f/d2, nact - 1), q = f - el*d2;
+        const int h = __shfl(has, el, 64);
+        const double de = __shfl(dg, el, 64);
+        if (f < nact*d2 && h) {
+            const int row = q/nfvars, col = q - row*nfvars;
+            diag[(int64_t)r0*d2 + f] = row == col ? de : 0.0;
         }
     }
 }
@@ -633,7 +721,6 @@ __global__ __launch_bounds__(FB) void k_fill_int(int64_t n, int *a, int v)
 // records are read and written as 16-byte pairs.
 template <int NV> constexpr int rec_w(int per) { return (per*NV + 1) & ~1; }
 
-typedef double fs_d2 __attribute__((ext_vector_type(2)));
 
 template <int NV>
 __global__ __launch_bounds__(FB) void k_lpack_rec(int neles, int nlocal, int nfvars,
@@ -1179,10 +1266,27 @@ int by_nv(int nv, A... a)
 
 int fs_launch_pack(hipStream_t s, int neles, int nlocal, int nvars, int nfvars, double a0,
                    const int *cell_ids, const double *rhs, const double *dt, double *rank_rhs,
-                   double *diag, int *flag)
+                   double *diag, int *flag, int *inv, double *rec)
 {
     if (neles <= 0) return UCFD_OK;
     if (nvars > PACK_MAXV) return UCFD_ERR_ARG;
+    if (inv && rec && nvars >= 4 && nvars <= 9) {
+        const int64_t nb = ((int64_t)nlocal + FB - 1)/FB;
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(FB), 0, s,
+                           (int64_t)nlocal, inv, -1);
+#define FS_BPACK(NVC) \
+        case NVC: \
+            hipLaunchKernelGGL(k_bpack_rec<NVC>, grid(neles), dim3(FB), 0, s, neles, nlocal, a0, \
+                               cell_ids, inv, rhs, dt, rec, flag); \
+            hipLaunchKernelGGL(k_bpack_from_rec<NVC>, dim3((unsigned)((nlocal + 63)/64)), dim3(64), \
+                               0, s, nlocal, nfvars, inv, rec, rank_rhs, diag); \
+            return ok();
+        switch (nvars) {
+        FS_BPACK(4) FS_BPACK(5) FS_BPACK(6) FS_BPACK(7) FS_BPACK(8) FS_BPACK(9)
+        default: break;
+        }
+#undef FS_BPACK
+    }
     hipLaunchKernelGGL(k_pack, dim3((unsigned)((neles + 63)/64)), dim3(64), 0, s, neles, nlocal,
                        nvars, nfvars, a0, cell_ids, rhs, dt, rank_rhs, diag, flag);
     return ok();
