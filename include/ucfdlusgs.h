@@ -76,7 +76,8 @@ UCFD_EXTERN ucfd_status_t UCFDLUSGS_Export(FlowSys sys, UCFDInt which, UCFDReal 
 /* flowsys.c:91-117: jmat [2][nfvars][nfvars][nfaces] (A+, A- per face),
  * tjmat [2][nturbvars][nturbvars][nfaces] or NULL */
 UCFD_EXTERN ucfd_status_t UCFDFlowSysSetBLUSGS(FlowSys *sys, UCFDReal *jmat, UCFDReal *tjmat);
-/* blusgs.c:47-57: rank rhs from the element rhs, diag = (1/dt + a0) I */
+/* blusgs.c:47-57: rank rhs from the element rhs, diag = (1/dt + a0) I
+ * (through one record per cell in the scalar method's work buffer) */
 UCFD_EXTERN ucfd_status_t UCFDBLUSGS_Pack(FlowSys sys, UCFDInt eidx, UCFDReal a0);
 /* blusgs.c:96-123: turbulence diagonal with the k-omega SST / SA source
  * Jacobian plus 1/(dt turb_factor) + a0 */
