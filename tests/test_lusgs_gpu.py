@@ -170,6 +170,37 @@ def test_boundary_only_cells(frac):
     assert np.array_equal(dq[:, alone], dstar[:, alone])
 
 
+@pytest.mark.parametrize("nvars,nfvars,ndims", [(4, 4, 2), (7, 5, 3), (12, 5, 3), (16, 5, 3)])
+def test_update_every_row_count(nvars, nfvars, ndims):
+    """upts += x over NVARS rows (lusgs.c:54-84): the kernel takes the cells in
+    descending order and every row of a cell in one thread, in chunks of
+    eight rows; host and device arrays, all four spellings, an odd cell
+    count.  Elementwise addition: bit for bit against numpy."""
+    import torch
+    L.call("ucfd_set_config", nvars, nfvars, ndims)
+    try:
+        n = 1001
+        rng = np.random.default_rng(nvars)
+        for k, name in enumerate(("lusgs_parallel_ns_update", "parallel_ns_lusgs_update",
+                                  "serial_ns_lusgs_update", "lusgs_serial_ns_update")):
+            u0 = rng.normal(size=(nvars, n))
+            x = rng.normal(size=(nvars, n))
+            want = u0 + x
+            if k % 2 == 0:
+                u = u0.copy()
+                L.call(name, n, u, x)
+                got = u
+            else:
+                ud = torch.from_numpy(u0.copy()).cuda()
+                L.call(name, n, ud, torch.from_numpy(x).cuda())
+                L.call("ucfd_synchronize")
+                got = ud.cpu().numpy()
+            say(f"  {name}, nvars={nvars}: bitwise {np.array_equal(got, want)}")
+            assert np.array_equal(got, want), name
+    finally:
+        L.call("ucfd_set_config", 5, 5, 3)
+
+
 def test_two_array_form_equals_in_place_form():
     z, mesh, flow = load_case("tet5p")
     lcolor = z["lcolor"].astype(np.int32)
