@@ -349,3 +349,109 @@ def test_lusgs_two_dimensions_and_turbulence_rows(kind, nt, colored):
     assert sorted(got) == sorted(ref)
     for k, v in ref.items():
         assert np.array_equal(np.asarray(got[k]).reshape(v.shape), v), k
+
+
+# ------------------------------------------------------ two element types
+def split_elements(el, seed):
+    """The elements of one type dealt at random into two types, each with its
+    own arrays ([nvars][neles_t] columns, per-element values) and cell_ids."""
+    ne = len(el["cell_ids"])
+    pick = np.random.default_rng(seed).permutation(ne)
+    parts = [np.sort(pick[:ne // 3]), np.sort(pick[ne // 3:])]
+    out = []
+    for idx in parts:
+        t = {}
+        for k, v in el.items():
+            v = np.asarray(v)
+            t[k] = np.ascontiguousarray(v[..., idx] if v.ndim == 2 else v[idx])
+        out.append(t)
+    return parts, out
+
+
+@pytest.mark.parametrize("colored", [False, True])
+def test_block_method_two_element_types(colored):
+    """Pack per element type writes only that type's cells (the record form
+    skips cells no element of the type maps to): packing two types that
+    share the cells equals packing them as one type, and every later stage
+    (Prepare, sweeps, SubResidual and Update per type) matches the one-type
+    oracle run bit for bit."""
+    from ucfd_sparse_amd import flowsys as FS
+    fs, tj, el = case(4, 73, colored, 6)
+    ref = O.flowsys_run(fs, tj, el, 6, 5, kind="sa", iterations=2)
+    parts, els = split_elements(el, 5)
+    fsd, tjd, _ = on_device(fs, tj, el)
+    S = FS.FlowSys(fsd, 6, 5, nelem=2, tjmat=tjd)
+    for t, e in enumerate(els):
+        S.set_element(t, {k: torch.from_numpy(v).cuda() for k, v in e.items()})
+    for t in range(2):
+        S.call("UCFDBLUSGS_Pack", t, 0.3)
+        S.call("UCFDBLUSGS_SA_Pack", t, 1.5, 0.3)
+    got = {"rank_rhs": S.export(FS.RHS), "diag_pack": S.export(FS.DIAG),
+           "tdiag_pack": S.export(FS.TDIAG)}
+    S.call("UCFDBLUSGS_NSPrepare")
+    got["diag"] = S.export(FS.DIAG)
+    S.call("UCFDBLUSGS_RANSPrepare")
+    got["tdiag"] = S.export(FS.TDIAG)
+    S.call("UCFDBLUSGS_Reset")
+    for it in range(2):
+        S.call("UCFDBLUSGS_NSLowerSweep")
+        got[f"du_lower{it}"] = S.export(FS.DU)
+        S.call("UCFDBLUSGS_NSUpperSweep")
+        got[f"du_upper{it}"] = S.export(FS.DU)
+        S.call("UCFDBLUSGS_RANSLowerSweep")
+        S.call("UCFDBLUSGS_RANSUpperSweep")
+        got[f"du_rans{it}"] = S.export(FS.DU)
+        res = np.zeros_like(np.asarray(ref[f"resid{it}"]).reshape(6, -1))
+        for t in range(2):
+            S.call("UCFDBLUSGS_SubResidual", t)
+            res[:, parts[t]] = S.element_array(t, "resid_out").cpu().numpy().reshape(6, -1)
+        got[f"resid{it}"] = res
+    up = np.zeros_like(np.asarray(ref["upts"]).reshape(6, -1))
+    for t in range(2):
+        S.call("UCFDBLUSGS_Update", t)
+        up[:, parts[t]] = S.element_array(t, "uptsb").cpu().numpy().reshape(6, -1)
+    got["upts"] = up
+    S.close()
+    say(f"two element types ({len(parts[0])} + {len(parts[1])} elements), colour-ordered={colored}")
+    for k, v in ref.items():
+        assert np.array_equal(np.asarray(got[k]).reshape(v.shape), v), k
+
+
+@pytest.mark.parametrize("colored", [False, True])
+def test_lusgs_two_element_types(colored):
+    """The scalar method with the elements in two types: Pack and Update per
+    type through the per-cell records, bit for bit equal to the one-type
+    oracle run."""
+    from ucfd_sparse_amd import flowsys as FS
+    mesh, flow, fs, fspr, tfspr, el = lusgs_case(seed=306, m=3, colored=colored)
+    ref = O.flowsys_lusgs_run(fs, fspr, tfspr, el, 6, 5, a0=0.2, turb_factor=1.7, kappa=1.1,
+                              iterations=2)
+    parts, els = split_elements(el, 9)
+    fsd, _, _ = on_device(fs, np.zeros(1), el)
+    S = FS.FlowSysLUSGS(fsd, 6, 5, torch.from_numpy(fspr).cuda(), torch.from_numpy(tfspr).cuda(),
+                        nelem=2)
+    for t, e in enumerate(els):
+        S.set_element(t, {k: torch.from_numpy(v).cuda() for k, v in e.items()})
+    for t in range(2):
+        S.call("UCFDLUSGS_Pack", t, 1.7, 0.2)
+    got = {"u": S.export(0), "du_pack": S.export(1), "diag_pack": S.export(2)}
+    S.call("UCFDLUSGS_NSPrepare", 1.1)
+    got["diag_ns"] = S.export(2)
+    S.call("UCFDLUSGS_RANSPrepare", 1.1)
+    got["diag"] = S.export(2)
+    for it in range(2):
+        for name, upper, key in (("NSLower", False, "du_nslower"), ("NSUpper", True, "du_nsupper"),
+                                 ("RANSLower", False, "du_ranslower"),
+                                 ("RANSUpper", True, "du_ransupper")):
+            S.call(f"UCFDLUSGS_{name}Sweep", 1.1)
+            got[f"{key}{it}"] = S.export(1)
+    up = np.zeros_like(np.asarray(ref["upts"]).reshape(6, -1))
+    for t in range(2):
+        S.call("UCFDLUSGS_Update", t)
+        up[:, parts[t]] = S.element_array(t, "uptsb").cpu().numpy().reshape(6, -1)
+    got["upts"] = up
+    S.close()
+    say(f"scalar method, two element types, colour-ordered={colored}")
+    assert sorted(got) == sorted(ref)
+    for k, v in ref.items():
+        assert np.array_equal(np.asarray(got[k]).reshape(v.shape), v), k
