@@ -206,11 +206,16 @@ __device__ __forceinline__ void block_sum(double v, double *part, int slot)
 //   s_i = rhs_i - A(e)_i        (WITH_RES; otherwise s_i = rhs_i, e = 0)
 //   dstar_i = (s_i - 1/2 sum_{nb < n0} term(Qb_nb; dstar_nb)) / D_i
 // Writes sum_i |s_i|^2 of this launch's blocks to part[pofs + block].
-template <typename T, int FC, bool WITH_RES>
+// LAST: [n0, n0+cnt) is the last colour, which the upper sweep visits first
+// and where it takes no face (every neighbour lies in an earlier colour):
+// the launch also does that colour's upper step, delta = dstar - 1/2*0/D
+// and e += delta (k_upper_free's expressions, on the values it would load),
+// and stores delta and e instead of dstar, which nothing else reads.
+template <typename T, int FC, bool WITH_RES, bool LAST = false>
 __global__ __launch_bounds__(kSB) void k_lower_res(
     int n0, int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
     const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ rhs,
-    const T *__restrict__ e, T *dstar, double *part, int pofs)
+    T *e, T *dstar, double *part, int pofs, T *delta = nullptr)
 {
     const int b = xcd_remap();
     const int t = b*kSB + threadIdx.x;
@@ -270,7 +275,26 @@ __global__ __launch_bounds__(kSB) void k_lower_res(
         T o[NV];
 #pragma unroll
         for (int q = 0; q < NV; ++q) o[q] = (s[q] - T(0.5)*acc[q])/dg;
-        st_split<true>(dstar, N, i, o);
+        if (LAST) {
+            T ev[NV], dlt[NV], en[NV];
+            if (WITH_RES) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) ev[q] = ei[q];
+            } else {
+                ld_split<true>(e, N, i, ev);
+            }
+            const T acc0 = T(0);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                dlt[q] = o[q] - T(0.5)*acc0/dg;
+                en[q] = ev[q] + dlt[q];
+            }
+            // temporal stores: the next launch gathers them
+            st_split<false>(delta, N, i, dlt);
+            st_split<false>(e, N, i, en);
+        } else {
+            st_split<true>(dstar, N, i, o);
+        }
     }
     block_sum<kSB>(ss, part, pofs + b);
 }
@@ -761,10 +785,12 @@ int copy_in(void *dst, const void *src, size_t bytes, hipStream_t st, const char
 // lower (with residual) over all colours, norm of s to *norm.  pre0: the
 // previous pass's upper launch formed the first colour's residual and block
 // partials (k_upper_upd_res into sres), so that colour only divides by D.
+// delta: the last colour's launch also does that colour's upper step (with
+// more than one colour; upper_pass is then called with last_done)
 template <typename T>
 int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb, const T *rhs,
-               const T *e, T *dstar, const Face4<T> *geo, const T *diag, bool pre0 = false,
-               const T *sres = nullptr)
+               T *e, T *dstar, const Face4<T> *geo, const T *diag, bool pre0 = false,
+               const T *sres = nullptr, T *delta = nullptr)
 {
     int pofs = 0;
     for (int c = 0; c < s->nc; ++c) {
@@ -776,14 +802,18 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
             pofs += (int)g.x;
             continue;
         }
-#define L4(RES) hipLaunchKernelGGL((k_lower_res<T, 4, RES>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
-                                   s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
-#define L6(RES) hipLaunchKernelGGL((k_lower_res<T, 6, RES>), g, dim3(kSB), 0, st, n0, cnt, s->N, \
-                                   s->nei, geo, diag, qb, rhs, e, dstar, s->part, pofs)
-        if (s->F == 4) { if (with_res) L4(true); else L4(false); }
-        else           { if (with_res) L6(true); else L6(false); }
-#undef L4
-#undef L6
+        const bool last = delta && s->nc > 1 && c == s->nc - 1;
+#define LR(FC, RES, LAST) hipLaunchKernelGGL((k_lower_res<T, FC, RES, LAST>), g, dim3(kSB), 0, st, \
+                                             n0, cnt, s->N, s->nei, geo, diag, qb, rhs, e, dstar, \
+                                             s->part, pofs, delta)
+        if (s->F == 4) {
+            if (last) { if (with_res) LR(4, true, true); else LR(4, false, true); }
+            else      { if (with_res) LR(4, true, false); else LR(4, false, false); }
+        } else {
+            if (last) { if (with_res) LR(6, true, true); else LR(6, false, true); }
+            else      { if (with_res) LR(6, true, false); else LR(6, false, false); }
+        }
+#undef LR
         pofs += (int)g.x;
     }
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, s->part, pofs, s->norm,
@@ -794,13 +824,16 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
 // upper sweep and update over all colours (last colour first).  rhs given:
 // the first colour's launch also forms the next pass's residual of its
 // cells into sres, with their block partials at the start of part.
+// last_done: the lower pass already did the last colour's upper step
 template <typename T>
 int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar, T *delta, T *e,
-               const Face4<T> *geo, const T *diag, const T *rhs = nullptr, T *sres = nullptr)
+               const Face4<T> *geo, const T *diag, const T *rhs = nullptr, T *sres = nullptr,
+               bool last_done = false)
 {
     for (int c = s->nc - 1; c >= 0; --c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
+        if (last_done && s->nc > 1 && c == s->nc - 1) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
         if (c == 0 && rhs) {
             if (s->F == 4)
@@ -1028,9 +1061,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
         // e = dq; iteration j: s = b - A(e) (in the lower launches), e += M^-1 s
         for (int it = 0; it < maxiter && !rc; ++it) {
             rc = lower_pass<double>(s, st, it > 0, s->q64, B, E, s->dstar64, s->g64, s->d64,
-                                    it > 0, s->sres64);
+                                    it > 0, s->sres64, s->delta64);
             if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, E, s->g64,
-                                             s->d64, B, s->sres64);
+                                             s->d64, B, s->sres64, true);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
             if (it == 0) bnorm = rn;
@@ -1060,9 +1093,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
             double s0 = 0.0;
             for (int j = 0; info.passes < maxiter && !rc; ++j) {
                 rc = lower_pass<float>(s, st, j > 0, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                       s->d32, j > 0, s->sres32);
+                                       s->d32, j > 0, s->sres32, s->delta32);
                 if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32,
-                                                s->g32, s->d32, s->r32, s->sres32);
+                                                s->g32, s->d32, s->r32, s->sres32, true);
                 double sn = 0.0;
                 if (!rc) rc = read_norm(s, st, &sn, fn);
                 info.passes++;
@@ -1119,27 +1152,28 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
     hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
     int rc = residual64(s, st, b, s->dq64, nullptr, true, true, true);
     // one untimed pass of each form, so the first colour's residual exists
-    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64);
+    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64,
+                                     false, nullptr, s->delta64);
     if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
-                                     s->d64, b, s->sres64);
+                                     s->d64, b, s->sres64, true);
     if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                    s->d32);
+                                    s->d32, false, nullptr, s->delta32);
     if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32, s->d32,
-                                    s->r32, s->sres32);
+                                    s->r32, s->sres32, true);
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps && !rc; ++r) {
         (void)hipEventRecord(ev[0], st);
         rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64, true,
-                                s->sres64);
+                                s->sres64, s->delta64);
         (void)hipEventRecord(ev[1], st);
         if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
-                                         s->d64, b, s->sres64);
+                                         s->d64, b, s->sres64, true);
         (void)hipEventRecord(ev[2], st);
         if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                        s->d32, true, s->sres32);
+                                        s->d32, true, s->sres32, s->delta32);
         (void)hipEventRecord(ev[3], st);
         if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32,
-                                        s->d32, s->r32, s->sres32);
+                                        s->d32, s->r32, s->sres32, true);
         (void)hipEventRecord(ev[4], st);
         if (!rc) rc = residual64(s, st, b, s->dq64, nullptr, false, false, true);
         (void)hipEventRecord(ev[5], st);
