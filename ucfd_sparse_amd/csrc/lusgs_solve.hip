@@ -209,7 +209,7 @@ __device__ __forceinline__ void block_sum(double v, double *part, int slot)
 // LAST: [n0, n0+cnt) is the last colour, which the upper sweep visits first
 // and where it takes no face (every neighbour lies in an earlier colour):
 // the launch also does that colour's upper step, delta = dstar - 1/2*0/D
-// and e += delta (k_upper_free's expressions, on the values it would load),
+// and e += delta (the general upper step's expressions with no face term),
 // and stores delta and e instead of dstar, which nothing else reads.
 template <typename T, int FC, bool WITH_RES, bool LAST = false>
 __global__ __launch_bounds__(kSB) void k_lower_res(
@@ -348,33 +348,6 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
     }
     st_split<true>(delta, N, i, dl);
     st_split<true>(e, N, i, en);
-}
-
-// ---- the last colour's upper sweep and update: no neighbour lies at or
-// above N, so no face is taken and the neighbour indices are not read;
-// k_upper_upd's expressions with an empty face sum.  The outputs are stored
-// with ordinary (temporal) stores: the next launch gathers them.
-template <typename T>
-__global__ __launch_bounds__(kSB) void k_upper_free(int n0, int cnt, int N,
-                                                    const T *__restrict__ diag,
-                                                    const T *__restrict__ dstar, T *delta, T *e)
-{
-    const int t = xcd_remap()*kSB + threadIdx.x;
-    if (t >= cnt) return;
-    const int i = n0 + t;
-    const T dg = ldg_nt(diag + i);
-    T ds[NV], ei[NV];
-    ld_split<true>(dstar, N, i, ds);
-    ld_split<true>(e, N, i, ei);
-    const T acc = T(0);
-    T dl[NV], en[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        dl[q] = ds[q] - T(0.5)*acc/dg;
-        en[q] = ei[q] + dl[q];
-    }
-    st_split<false>(delta, N, i, dl);
-    st_split<false>(e, N, i, en);
 }
 
 // ---- the first colour's upper sweep and update (k_upper_upd on [0, cnt)),
@@ -785,12 +758,12 @@ int copy_in(void *dst, const void *src, size_t bytes, hipStream_t st, const char
 // lower (with residual) over all colours, norm of s to *norm.  pre0: the
 // previous pass's upper launch formed the first colour's residual and block
 // partials (k_upper_upd_res into sres), so that colour only divides by D.
-// delta: the last colour's launch also does that colour's upper step (with
-// more than one colour; upper_pass is then called with last_done)
+// With more than one colour the last colour's launch also does that
+// colour's upper step, writing delta and e (upper_pass then skips it).
 template <typename T>
 int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb, const T *rhs,
-               T *e, T *dstar, const Face4<T> *geo, const T *diag, bool pre0 = false,
-               const T *sres = nullptr, T *delta = nullptr)
+               T *e, T *dstar, T *delta, const Face4<T> *geo, const T *diag, bool pre0 = false,
+               const T *sres = nullptr)
 {
     int pofs = 0;
     for (int c = 0; c < s->nc; ++c) {
@@ -802,7 +775,7 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
             pofs += (int)g.x;
             continue;
         }
-        const bool last = delta && s->nc > 1 && c == s->nc - 1;
+        const bool last = s->nc > 1 && c == s->nc - 1;
 #define LR(FC, RES, LAST) hipLaunchKernelGGL((k_lower_res<T, FC, RES, LAST>), g, dim3(kSB), 0, st, \
                                              n0, cnt, s->N, s->nei, geo, diag, qb, rhs, e, dstar, \
                                              s->part, pofs, delta)
@@ -824,16 +797,16 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
 // upper sweep and update over all colours (last colour first).  rhs given:
 // the first colour's launch also forms the next pass's residual of its
 // cells into sres, with their block partials at the start of part.
-// last_done: the lower pass already did the last colour's upper step
+// The last colour's upper step was done by the lower pass (more than one
+// colour).
 template <typename T>
 int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar, T *delta, T *e,
-               const Face4<T> *geo, const T *diag, const T *rhs = nullptr, T *sres = nullptr,
-               bool last_done = false)
+               const Face4<T> *geo, const T *diag, const T *rhs = nullptr, T *sres = nullptr)
 {
     for (int c = s->nc - 1; c >= 0; --c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
         if (cnt <= 0) continue;
-        if (last_done && s->nc > 1 && c == s->nc - 1) continue;
+        if (s->nc > 1 && c == s->nc - 1) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
         if (c == 0 && rhs) {
             if (s->F == 4)
@@ -842,11 +815,6 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
             else
                 hipLaunchKernelGGL((k_upper_upd_res<T, 6>), g, dim3(kSB), 0, st, cnt, s->N, s->nei,
                                    geo, diag, qb, dstar, delta, e, rhs, sres, s->part, 0);
-            continue;
-        }
-        if (c == s->nc - 1) {
-            hipLaunchKernelGGL(k_upper_free<T>, g, dim3(kSB), 0, st, n0, cnt, s->N, diag, dstar,
-                               delta, e);
             continue;
         }
         if (s->F == 4)
@@ -1060,10 +1028,10 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
     if (mode == UCFD_SOLVE_FP64) {
         // e = dq; iteration j: s = b - A(e) (in the lower launches), e += M^-1 s
         for (int it = 0; it < maxiter && !rc; ++it) {
-            rc = lower_pass<double>(s, st, it > 0, s->q64, B, E, s->dstar64, s->g64, s->d64,
-                                    it > 0, s->sres64, s->delta64);
+            rc = lower_pass<double>(s, st, it > 0, s->q64, B, E, s->dstar64, s->delta64, s->g64,
+                                    s->d64, it > 0, s->sres64);
             if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, E, s->g64,
-                                             s->d64, B, s->sres64, true);
+                                             s->d64, B, s->sres64);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
             if (it == 0) bnorm = rn;
@@ -1092,10 +1060,10 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
             hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
             double s0 = 0.0;
             for (int j = 0; info.passes < maxiter && !rc; ++j) {
-                rc = lower_pass<float>(s, st, j > 0, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                       s->d32, j > 0, s->sres32, s->delta32);
+                rc = lower_pass<float>(s, st, j > 0, s->qb32, s->r32, s->e32, s->dstar32,
+                                       s->delta32, s->g32, s->d32, j > 0, s->sres32);
                 if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32,
-                                                s->g32, s->d32, s->r32, s->sres32, true);
+                                                s->g32, s->d32, s->r32, s->sres32);
                 double sn = 0.0;
                 if (!rc) rc = read_norm(s, st, &sn, fn);
                 info.passes++;
@@ -1152,28 +1120,28 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
     hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(kB), 0, st, (int64_t)n, s->e32, 0.0f);
     int rc = residual64(s, st, b, s->dq64, nullptr, true, true, true);
     // one untimed pass of each form, so the first colour's residual exists
-    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64,
-                                     false, nullptr, s->delta64);
+    if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->delta64,
+                                     s->g64, s->d64);
     if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
-                                     s->d64, b, s->sres64, true);
-    if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                    s->d32, false, nullptr, s->delta32);
+                                     s->d64, b, s->sres64);
+    if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->delta32,
+                                    s->g32, s->d32);
     if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32, s->d32,
-                                    s->r32, s->sres32, true);
+                                    s->r32, s->sres32);
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps && !rc; ++r) {
         (void)hipEventRecord(ev[0], st);
-        rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->g64, s->d64, true,
-                                s->sres64, s->delta64);
+        rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->delta64, s->g64,
+                                s->d64, true, s->sres64);
         (void)hipEventRecord(ev[1], st);
         if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
-                                         s->d64, b, s->sres64, true);
+                                         s->d64, b, s->sres64);
         (void)hipEventRecord(ev[2], st);
-        if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->g32,
-                                        s->d32, true, s->sres32, s->delta32);
+        if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32,
+                                        s->delta32, s->g32, s->d32, true, s->sres32);
         (void)hipEventRecord(ev[3], st);
         if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32,
-                                        s->d32, s->r32, s->sres32, true);
+                                        s->d32, s->r32, s->sres32);
         (void)hipEventRecord(ev[4], st);
         if (!rc) rc = residual64(s, st, b, s->dq64, nullptr, false, false, true);
         (void)hipEventRecord(ev[5], st);
