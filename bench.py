@@ -215,11 +215,12 @@ def precision_forms(lay, T, rtol, repeats, log):
     err = float((bm - a).abs().max() / a.abs().max())
     k64 = out["fp64"]["passes"]
     kern = {}
-    # one pass = the lower launches (residual and lower sweep; with more than
-    # one colour the last colour's launch also does its upper step) and the
-    # upper launches of the other colours (upper sweep, update and, in the
-    # first colour's launch, that colour's residual for the next pass):
-    # bandwidth over the whole pass
+    # one pass = the lower launches of every colour but the first (residual
+    # and lower sweep; with more than one colour the last colour's launch
+    # also does its upper step) and the upper launches of every colour but
+    # the last (upper sweep, update and, in the first colour's launch, that
+    # colour's residual and lower step for the next pass): bandwidth over
+    # the whole pass
     for key, lo, sz in (("pass_fp64", 0, 8), ("pass_fp32", 2, 4)):
         lb, ub = solver_kernel_bytes(F, sz)
         nb = (lb + ub) * N
@@ -227,7 +228,7 @@ def precision_forms(lay, T, rtol, repeats, log):
         gbs = nb / (t * 1e-3) / 1e9
         kern[key] = {"ms_per_pass": t, "lower_launches_ms": ms[lo], "upper_launches_ms": ms[lo + 1],
                      "algorithmic_bytes": nb, "gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
-                     "launches": 2 * lay.ncolors - (lay.ncolors > 1)}
+                     "launches": max(2 * lay.ncolors - 2, 1)}
     rb = (4 * F + 8 * (4 * F + 1 + 15)) * N        # nei, geo, diag, q, b, dq
     kern["residual_fp64"] = {"ms_per_pass": ms[4], "algorithmic_bytes": rb,
                              "gbs": rb / (ms[4] * 1e-3) / 1e9,

@@ -166,11 +166,12 @@ int ucfd_lusgs_solve(ucfd_lusgs_solver *solver, int mode, const double *b, doubl
 
 /* Measurement extension (no counterpart in the reference): HIP-event times
  * of the solver's kernels over one full pass each, averaged over reps, on the
- * library stream: ms[0] lower sweep with residual (FP64; with more than one
- * colour its last-colour launch also does that colour's upper step and
- * update, as in the solve), ms[1] upper sweep with update of the other
- * colours (FP64; its first-colour launch also forms that colour's
- * residual for the next pass, as in the solve), ms[2] / ms[3] the same
+ * library stream, as in the solve: ms[0] lower sweep with residual of the
+ * colours after the first (FP64; with more than one colour the last
+ * colour's launch also does that colour's upper step and update), ms[1]
+ * upper sweep with update of the other colours (FP64; the first colour's
+ * launch also forms that colour's residual and lower step for the next
+ * pass), ms[2] / ms[3] the same
  * kernels in FP32 (the inner kernels of UCFD_SOLVE_MIXED), ms[4] the FP64
  * residual and its norm.  b is a device array [5][neles]; the solver's work
  * arrays are overwritten. */

@@ -359,13 +359,16 @@ __global__ __launch_bounds__(kSB) void k_upper_upd(
 // neighbour states qb_nb it gathers are the residual's.  Same expressions
 // and order as k_lower_res's residual, so s and the block partials
 // (part[pofs + block], same blocks) are bit for bit those the next pass's
-// lower launch would compute; that launch then only forms dstar = s / D
-// (k_lower_pre).
+// lower launch would compute.  No face is lower than the first colour, so
+// that launch's step for these cells is dstar = (s - 1/2*0) / D
+// (k_lower_res's expression with an empty face sum): it is done here, over
+// this thread's own dstar (read above as the current pass's), and the next
+// lower pass starts at the second colour.
 template <typename T, int FC>
 __global__ __launch_bounds__(kSB) void k_upper_upd_res(
     int cnt, int N, const int *__restrict__ nei, const Face4<T> *__restrict__ geo,
-    const T *__restrict__ diag, const T *__restrict__ qb, const T *__restrict__ dstar,
-    T *delta, T *e, const T *__restrict__ rhs, T *__restrict__ sres, double *part, int pofs)
+    const T *__restrict__ diag, const T *__restrict__ qb, T *dstar, T *delta, T *e,
+    const T *__restrict__ rhs, double *part, int pofs)
 {
     const int b = xcd_remap();
     const int t = b*kSB + threadIdx.x;
@@ -411,7 +414,8 @@ __global__ __launch_bounds__(kSB) void k_upper_upd_res(
             dl[q] = ds[q] - T(0.5)*acc[q]/dg;
             enew[q] = ei[q] + dl[q];
         }
-        st_split<true>(delta, N, i, dl);
+        // delta of the first colour is not stored: only the upper launches
+        // of earlier colours gather a colour's delta, and there are none
         st_split<true>(e, N, i, enew);
         T res[NV];
 #pragma unroll
@@ -424,32 +428,17 @@ __global__ __launch_bounds__(kSB) void k_upper_upd_res(
 #pragma unroll
             for (int q = 0; q < NV; ++q) res[q] += o[q];
         }
+        const T acc0 = T(0);
+        T o[NV];
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             const T sq = rh[q] - (dg*enew[q] + T(0.5)*res[q]);
             ss += (double)sq*(double)sq;
-            sres[(size_t)q*N + i] = sq;
+            o[q] = (sq - T(0.5)*acc0)/dg;
         }
+        st_split<false>(dstar, N, i, o);
     }
     block_sum<kSB>(ss, part, pofs + b);
-}
-
-// ---- the first colour's lower sweep when k_upper_upd_res formed its
-// residual s (and block partials) in the previous pass: no face is lower
-// than the first colour, so dstar = (s - 1/2*0) / D, k_lower_res's
-// expression with an empty face sum.
-template <typename T>
-__global__ __launch_bounds__(kSB) void k_lower_pre(int cnt, int N, const T *__restrict__ diag,
-                                                   const T *__restrict__ sres, T *dstar)
-{
-    const int t = xcd_remap()*kSB + threadIdx.x;
-    if (t >= cnt) return;
-    const T dg = ldg_nt(diag + t);
-    const T acc = T(0);
-    T o[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) o[q] = (ldg_nt(sres + (size_t)q*N + t) - T(0.5)*acc)/dg;
-    st_split<false>(dstar, N, t, o);
 }
 
 // ---- FP64 residual r = b - A(dq) on base Q (split layout) over all cells;
@@ -719,14 +708,11 @@ struct ucfd_lusgs_solver {
     Face4<double> *g64 = nullptr;
     Face4<float> *g32 = nullptr;
     // q64 (base state), dq64 (the solve's increment), dstar*/delta*, e32 and
-    // qb32: split layout; r32 and sres*: [var][cell]
+    // qb32: split layout; r32: [var][cell]
     double *d64 = nullptr, *q64 = nullptr, *dq64 = nullptr, *dstar64 = nullptr,
            *delta64 = nullptr;
     float *d32 = nullptr, *qb32 = nullptr, *r32 = nullptr, *e32 = nullptr, *dstar32 = nullptr,
           *delta32 = nullptr;
-    // the first colour's residual for the next pass, formed by its upper launch
-    double *sres64 = nullptr;
-    float *sres32 = nullptr;
     double *part = nullptr, *norm = nullptr, *hnorm = nullptr;
     double *hnorm_dev = nullptr;   // device address of the pinned hnorm
     int nparts = 0;
@@ -756,14 +742,13 @@ int copy_in(void *dst, const void *src, size_t bytes, hipStream_t st, const char
 }
 
 // lower (with residual) over all colours, norm of s to *norm.  pre0: the
-// previous pass's upper launch formed the first colour's residual and block
-// partials (k_upper_upd_res into sres), so that colour only divides by D.
-// With more than one colour the last colour's launch also does that
+// previous pass's upper launch formed the first colour's residual, block
+// partials and dstar (k_upper_upd_res), so the pass starts at the second
+// colour.  With more than one colour the last colour's launch also does that
 // colour's upper step, writing delta and e (upper_pass then skips it).
 template <typename T>
 int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb, const T *rhs,
-               T *e, T *dstar, T *delta, const Face4<T> *geo, const T *diag, bool pre0 = false,
-               const T *sres = nullptr)
+               T *e, T *dstar, T *delta, const Face4<T> *geo, const T *diag, bool pre0 = false)
 {
     int pofs = 0;
     for (int c = 0; c < s->nc; ++c) {
@@ -771,7 +756,6 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
         if (cnt <= 0) continue;
         const dim3 g((cnt + kSB - 1)/kSB);
         if (c == 0 && pre0) {
-            hipLaunchKernelGGL(k_lower_pre<T>, g, dim3(kSB), 0, st, cnt, s->N, diag, sres, dstar);
             pofs += (int)g.x;
             continue;
         }
@@ -796,12 +780,12 @@ int lower_pass(ucfd_lusgs_solver *s, hipStream_t st, bool with_res, const T *qb,
 
 // upper sweep and update over all colours (last colour first).  rhs given:
 // the first colour's launch also forms the next pass's residual of its
-// cells into sres, with their block partials at the start of part.
-// The last colour's upper step was done by the lower pass (more than one
-// colour).
+// cells, their block partials (at the start of part) and their dstar for
+// the next pass.  The last colour's upper step was done by the lower pass
+// (more than one colour).
 template <typename T>
-int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar, T *delta, T *e,
-               const Face4<T> *geo, const T *diag, const T *rhs = nullptr, T *sres = nullptr)
+int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, T *dstar, T *delta, T *e,
+               const Face4<T> *geo, const T *diag, const T *rhs = nullptr)
 {
     for (int c = s->nc - 1; c >= 0; --c) {
         const int n0 = s->off[c], cnt = s->off[c + 1] - n0;
@@ -811,10 +795,10 @@ int upper_pass(ucfd_lusgs_solver *s, hipStream_t st, const T *qb, const T *dstar
         if (c == 0 && rhs) {
             if (s->F == 4)
                 hipLaunchKernelGGL((k_upper_upd_res<T, 4>), g, dim3(kSB), 0, st, cnt, s->N, s->nei,
-                                   geo, diag, qb, dstar, delta, e, rhs, sres, s->part, 0);
+                                   geo, diag, qb, dstar, delta, e, rhs, s->part, 0);
             else
                 hipLaunchKernelGGL((k_upper_upd_res<T, 6>), g, dim3(kSB), 0, st, cnt, s->N, s->nei,
-                                   geo, diag, qb, dstar, delta, e, rhs, sres, s->part, 0);
+                                   geo, diag, qb, dstar, delta, e, rhs, s->part, 0);
             continue;
         }
         if (s->F == 4)
@@ -922,7 +906,6 @@ UCFD_API int ucfd_lusgs_solver_create(ucfd_lusgs_solver **out, int neles, int nf
         (rc = dalloc(s, &s->delta64, NV*N, fn)) || (rc = dalloc(s, &s->qb32, NV*N, fn)) ||
         (rc = dalloc(s, &s->r32, NV*N, fn)) || (rc = dalloc(s, &s->e32, NV*N, fn)) ||
         (rc = dalloc(s, &s->dstar32, NV*N, fn)) || (rc = dalloc(s, &s->delta32, NV*N, fn)) ||
-        (rc = dalloc(s, &s->sres64, NV*N, fn)) || (rc = dalloc(s, &s->sres32, NV*N, fn)) ||
         (rc = dalloc(s, &s->part, (size_t)s->nparts, fn)) || (rc = dalloc(s, &s->norm, 8, fn)) ||
         (rc = dalloc(s, &tfnv, F*N, fn)) || (rc = dalloc(s, &tvfn, 3*F*N, fn)) ||
         (rc = dalloc(s, &tsp, F*N, fn)) || (rc = dalloc(s, &tdt, N, fn)) ||
@@ -1029,9 +1012,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
         // e = dq; iteration j: s = b - A(e) (in the lower launches), e += M^-1 s
         for (int it = 0; it < maxiter && !rc; ++it) {
             rc = lower_pass<double>(s, st, it > 0, s->q64, B, E, s->dstar64, s->delta64, s->g64,
-                                    s->d64, it > 0, s->sres64);
+                                    s->d64, it > 0);
             if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, E, s->g64,
-                                             s->d64, B, s->sres64);
+                                             s->d64, B);
             if (!rc) rc = read_norm(s, st, &rn, fn);
             if (rc) break;
             if (it == 0) bnorm = rn;
@@ -1061,9 +1044,9 @@ UCFD_API int ucfd_lusgs_solve(ucfd_lusgs_solver *s, int mode, const double *b, d
             double s0 = 0.0;
             for (int j = 0; info.passes < maxiter && !rc; ++j) {
                 rc = lower_pass<float>(s, st, j > 0, s->qb32, s->r32, s->e32, s->dstar32,
-                                       s->delta32, s->g32, s->d32, j > 0, s->sres32);
+                                       s->delta32, s->g32, s->d32, j > 0);
                 if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32,
-                                                s->g32, s->d32, s->r32, s->sres32);
+                                                s->g32, s->d32, s->r32);
                 double sn = 0.0;
                 if (!rc) rc = read_norm(s, st, &sn, fn);
                 info.passes++;
@@ -1123,25 +1106,25 @@ UCFD_API int ucfd_lusgs_solver_time_kernels(ucfd_lusgs_solver *s, const double *
     if (!rc) rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->delta64,
                                      s->g64, s->d64);
     if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
-                                     s->d64, b, s->sres64);
+                                     s->d64, b);
     if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32, s->delta32,
                                     s->g32, s->d32);
     if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32, s->d32,
-                                    s->r32, s->sres32);
+                                    s->r32);
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps && !rc; ++r) {
         (void)hipEventRecord(ev[0], st);
         rc = lower_pass<double>(s, st, true, s->q64, b, s->dq64, s->dstar64, s->delta64, s->g64,
-                                s->d64, true, s->sres64);
+                                s->d64, true);
         (void)hipEventRecord(ev[1], st);
         if (!rc) rc = upper_pass<double>(s, st, s->q64, s->dstar64, s->delta64, s->dq64, s->g64,
-                                         s->d64, b, s->sres64);
+                                         s->d64, b);
         (void)hipEventRecord(ev[2], st);
         if (!rc) rc = lower_pass<float>(s, st, true, s->qb32, s->r32, s->e32, s->dstar32,
-                                        s->delta32, s->g32, s->d32, true, s->sres32);
+                                        s->delta32, s->g32, s->d32, true);
         (void)hipEventRecord(ev[3], st);
         if (!rc) rc = upper_pass<float>(s, st, s->qb32, s->dstar32, s->delta32, s->e32, s->g32,
-                                        s->d32, s->r32, s->sres32);
+                                        s->d32, s->r32);
         (void)hipEventRecord(ev[4], st);
         if (!rc) rc = residual64(s, st, b, s->dq64, nullptr, false, false, true);
         (void)hipEventRecord(ev[5], st);
