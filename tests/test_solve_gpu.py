@@ -30,10 +30,17 @@ def need_gpu():
     L.call("ucfd_set_stream", None)
 
 
-def make(m, seed, hexmesh=False):
+def make(m, seed, hexmesh=False, coloring="greedy"):
     mesh = M.hex_mesh(m, m, m, seed=seed) if hexmesh else M.tet_mesh(m, seed=seed)
     flow = M.flow_fields(mesh, seed=seed)
-    return colored_layout(mesh, flow)
+    return colored_layout(mesh, flow, coloring=coloring)
+
+
+# greedy: 4-5 colours; greedy-bfs: 2 colours on these meshes (the first
+# colour's lower step and the last colour's upper step then sit in the
+# other colour's launch, as in the 96^3 x 6 measurement)
+SOLVE_CASES = [(6, 3, False), (8, 5, True), (12, 4, False), (6, 3, False, "greedy-bfs"),
+               (8, 5, True, "greedy-bfs")]
 
 
 def dev(a):
@@ -87,9 +94,11 @@ def test_gpu_residual_vanishes_on_first_colour_after_legacy_pass():
     s.close()
 
 
-@pytest.mark.parametrize("case", [(6, 3, False), (8, 5, True), (12, 4, False)])
+@pytest.mark.parametrize("case", SOLVE_CASES)
 def test_fp64_solve_matches_oracle_defect_correction(case):
     lay = make(*case)
+    if len(case) > 3:
+        assert lay.ncolors == 2
     m, f = lay.mesh, lay.flow
     s = LUSGSSolver.from_layout(lay)
     b = np.ascontiguousarray(f.rhsb[:5])
@@ -108,9 +117,11 @@ def test_fp64_solve_matches_oracle_defect_correction(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", [(6, 3, False), (8, 5, True), (12, 4, False)])
+@pytest.mark.parametrize("case", SOLVE_CASES)
 def test_mixed_solve_reaches_fp64_accuracy(case):
     lay = make(*case)
+    if len(case) > 3:
+        assert lay.ncolors == 2
     m, f = lay.mesh, lay.flow
     s = LUSGSSolver.from_layout(lay)
     b = dev(f.rhsb[:5])
