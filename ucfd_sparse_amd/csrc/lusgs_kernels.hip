@@ -19,7 +19,10 @@
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSweepBlock = 128;   // sweeps: more workgroups for the small colours
+// sweeps: 4 waves per workgroup; at the sweep's 3 waves per SIMD a CU holds
+// three such workgroups (12 waves).  Measured on the 2.65M-cell heavy
+// launch: 201 us against 210 us with 128 threads and 218 us with 64.
+constexpr int kSweepBlock = 256;
 
 __device__ __forceinline__ void raise_flag(int *flag, int bit)
 {
