@@ -148,7 +148,7 @@ int ucfd_lusgs_residual(ucfd_lusgs_solver *solver, const double *b, const double
  * memory.  The call allocates nothing: its scratch (block partial sums, norm,
  * error word) is a fixed library array of 1 MiB per GPU, held until the
  * process ends, so the colour ranges may hold at most 131072 workgroups of
- * 128 cells (sum over colours of ceil(cells/128); about 16.7M cells) and
+ * 256 cells (sum over colours of ceil(cells/256); about 33.5M cells) and
  * larger meshes are refused with status 1.  Every call on one GPU shares that
  * array: calls that overlap on one GPU (several host threads or streams)
  * race and must be serialised by the caller.  Runs on the library stream and

@@ -99,7 +99,7 @@ def test_noalloc_residual_refuses_host_arrays_and_oversize_meshes():
         m.vec_fnorm.ctypes.data, f.fspr.ctypes.data, f.dt.ctypes.data, f.upts.ctypes.data,
         b.ctypes.data, dq.ctypes.data, None, None)
     assert rc == 1
-    big = 131072 * 128 + 1
+    big = 131072 * 256 + 1
     off2 = np.array([0, big], np.int32)
     rc = L.load().ucfd_lusgs_residual_noalloc(big, 4, 1, off2.ctypes.data, 8, 8, 8, 8, 8, 8, 8,
                                               8, None, None)

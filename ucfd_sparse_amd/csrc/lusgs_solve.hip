@@ -36,7 +36,9 @@
 namespace {
 
 constexpr int kB = 256;
-constexpr int kSB = 128;    // sweep and residual kernels
+// sweep and residual kernels: 256-thread workgroups (FP64 pass 0.595 -> 0.574 ms,
+// FP32 0.297 -> 0.281 ms against 128 threads at 5.3M cells)
+constexpr int kSB = 256;
 constexpr int NV = 5;
 
 template <typename T> struct alignas(4*sizeof(T)) Face4 {
@@ -1213,8 +1215,8 @@ UCFD_API int ucfd_lusgs_residual_noalloc(int neles, int nface, int ncolors, cons
     }
     if (parts > kNoAllocParts)
         return ucfd_report(UCFD_ERR_ARG, fn, "mesh too large for the fixed scratch: the colour "
-                                             "ranges need more than 131072 workgroups of 128 "
-                                             "cells (about 16.7M cells)");
+                                             "ranges need more than 131072 workgroups of 256 "
+                                             "cells (about 33.5M cells)");
     if (ucfd_config().ndims != 3)
         return ucfd_report(UCFD_ERR_ARG, fn, "3-D Euler only (ndims = 3)");
     const void *arrays[] = {nei_ele, fnorm_vol, vec_fnorm, fspr, dt, upts, b, dq, r};
